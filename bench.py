@@ -1,0 +1,212 @@
+"""Headline benchmark: batched L^p-annealed projected-Adam search at n = 8400.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A bench "step" is one launch of the fused local-search kernel over the whole
+batch: P = 512 structured-prior candidates at n = 8400, 200 Adam iterations
+each with p annealed 4 -> 1e4 and lr 0.085 -> 0.0015 (the reference's cold
+tier schedule, src/main.cu:627).  value = P * 200 / (seconds per step)
+candidate-steps/s.  Inputs are resident in HBM before the timed region.
+
+Beside it: the plain form (one candidate per launch, same kernel, a bounded
+sample of candidates run back to back on the stream), the roofline of the
+local-search kernel (4 n^2 FLOP per candidate-step, src/main.cu:313, :625,
+against the 157.3 TFLOP/s dense FP32 matrix peak), and the CPU baseline: the
+reference's own fp64 CPU twin of the kernel (tools/probe.cpp --opt, built from
+source into oracle/_ref/) timed on this host, or the oracle port when that
+binary is absent.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+
+METRIC = "candidate-steps/s at n=8400, batched vs plain, F within 1e-12 of FP64"
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X dense FP32 (matrix and vector)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=8400)
+    ap.add_argument("--P", type=int, default=512)
+    ap.add_argument("--iters", type=int, default=200)
+    ap.add_argument("--plain-sample", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=20260905)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def traffic_from_profiles(n: int, P: int, iters: int):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    f = REPO / "profiles" / f"local_search_n{n}_pmc.md"
+    if not f.exists():
+        return None
+    m = re.search(r"TRAFFIC_BYTES_PER_LAUNCH\s+P=(\d+)\s+iters=(\d+)\s+bytes=([0-9.eE+]+)",
+                  f.read_text())
+    if not m or int(m.group(1)) != P or int(m.group(2)) != iters:
+        return None
+    return float(m.group(3))
+
+
+def cpu_baseline(n: int, seconds: float, gpu_pop, gpu_F):
+    """CPU leg: the reference's fp64 twin (or the oracle port) on a bounded
+    sample of the same workload, plus the fp64 F check of the GPU results."""
+    sys.path.insert(0, str(REPO / "oracle"))
+    import numpy as np
+    import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    # F check: GPU fp64 F of every checked candidate vs the oracle (hostF order)
+    k = min(8, gpu_pop.shape[0])
+    rel = 0.0
+    for p in range(k):
+        want = O.F_hostorder(gpu_pop[p].astype(np.float64))
+        rel = max(rel, abs(gpu_F[p] - want) / want)
+    fcheck = {"max_rel_err": rel, "sample": k, "oracle": "hostF order fp64"}
+
+    probe = REPO / "oracle" / "_ref" / "probe"
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    if probe.exists():
+        # calibrate: one iteration of every restart, then size the run
+        t0 = time.perf_counter()
+        subprocess.run([str(probe), "--opt", "--n", str(n), "--restarts", str(threads),
+                        "--iters", "2"], capture_output=True, env=env, check=True)
+        per = (time.perf_counter() - t0) / 3.0  # 2 iterations + final objective per restart
+        iters = max(2, int(seconds / max(per, 1e-6)))
+        t0 = time.perf_counter()
+        subprocess.run([str(probe), "--opt", "--n", str(n), "--restarts", str(threads),
+                        "--iters", str(iters)], capture_output=True, env=env, check=True)
+        dt = time.perf_counter() - t0
+        steps = threads * iters
+        return {"value": steps / dt, "unit": "candidate-steps/s", "cores": threads,
+                "kind": "reference",
+                "sample": f"tools/probe.cpp --opt (fp64 CPU twin of the kernel), n={n}, "
+                          f"{threads} restarts x {iters} iterations, OpenMP {threads} threads, "
+                          f"{dt:.1f} s wall"}, fcheck
+    W = importlib.import_module("sidon-autocorrelation-bound_amd.workloads")
+    pop = W.structured_prior(threads, n, seed=1).astype(np.float64)
+    t0 = time.perf_counter()
+    O.local_search_batch(pop, 1, 4.0, 4.0, 0.05, 0.05, nthreads=threads)
+    per = time.perf_counter() - t0
+    iters = max(1, int(seconds / max(per, 1e-6)))
+    t0 = time.perf_counter()
+    O.local_search_batch(pop, iters, 4.0, 1e4, 0.085, 0.0015, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": threads * iters / dt, "unit": "candidate-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle fp64 local search, n={n}, {threads} candidates x {iters} "
+                      f"iterations, {dt:.1f} s"}, fcheck
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); every box here "
+              "has one GPU and this build has no multi-GPU path", file=sys.stderr)
+        return 2
+    import numpy as np
+    import torch
+
+    S = importlib.import_module("sidon-autocorrelation-bound_amd")
+    W = importlib.import_module("sidon-autocorrelation-bound_amd.workloads")
+    S.init(0)
+    n, P, iters = args.n, args.P, args.iters
+    sched = (4.0, 1e4, 0.085, 0.0015)
+
+    start = torch.as_tensor(W.structured_prior(P, n, seed=args.seed), device="cuda")
+    pop = start.clone()
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        pop.copy_(start)
+        S.local_search(pop, iters, *sched)
+    torch.cuda.synchronize()
+
+    times = []
+    for _ in range(args.steps):
+        pop.copy_(start)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        S.local_search(pop, iters, *sched)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        times.append(ev0.elapsed_time(ev1) / 1e3)
+    ms = float(np.median(times)) * 1e3
+    value = P * iters / (ms / 1e3)
+
+    # plain form: one candidate per launch, back to back on the same stream
+    k = min(args.plain_sample, P)
+    plain = start[:k].clone()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for p in range(k):
+        S.local_search(plain[p:p + 1], iters, *sched)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    plain_s = ev0.elapsed_time(ev1) / 1e3
+    plain_value = k * iters / plain_s
+    same = bool(torch.equal(plain, pop[:k]))
+
+    F, _ = S.eval_f64(pop)
+    torch.cuda.synchronize()
+    F0, _ = S.eval_f64(start)
+    flop = 4.0 * n * n * iters * P
+    achieved = flop / (ms / 1e3) / 1e12
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "candidate-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"batched anneal: {P} structured-prior candidates at n={n}, "
+                               f"{iters} projected-Adam steps per launch, p 4->1e4, "
+                               f"lr 0.085->0.0015, one persistent launch",
+                   "P": P, "n": n, "iters_per_step": iters, "seed": args.seed,
+                   "block": S.get_ls_block(n)},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+                     "traffic": traffic_from_profiles(n, P, iters)},
+        "plain": {"value": plain_value, "unit": "candidate-steps/s", "sample": k,
+                  "form": "one candidate per launch (grid = 1 workgroup), launched back to "
+                          "back on one stream", "bitwise_equal_to_batched": same},
+        "batched_over_plain": value / plain_value,
+        "F_mean_before": float(F0.mean().item()),
+        "F_mean_after": float(F.mean().item()),
+    }
+    if not args.no_cpu:
+        cpu, fcheck = cpu_baseline(n, args.cpu_seconds, pop[:8].cpu().numpy(),
+                                   F[:8].cpu().numpy())
+        out["cpu_baseline"] = cpu
+        out["f_check"] = fcheck
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

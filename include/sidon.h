@@ -1,0 +1,185 @@
+/*
+ * sidon.h -- C ABI of the MI355X (gfx950) Sidon-autocorrelation search path.
+ *
+ * Objective: F(a) = 2n * max_k b_k / (sum_i a_i)^2, b_k = sum_{i+j=k} a_i a_j,
+ * a_i >= 0 (reference: src/kernels.cuh:1-13).
+ *
+ * Conventions (every entry point):
+ *  - all array arguments are DEVICE pointers on the current HIP device,
+ *    owned by the caller; population arrays are row-major [P x n];
+ *  - work is enqueued on `stream` (0 = the null stream) and is asynchronous;
+ *  - nothing here allocates device memory except sidon_init() (a small
+ *    per-device work-queue workspace, created once) and the generation-graph
+ *    constructor;
+ *  - return value: SIDON_OK (0) or one of the sidon_status codes; the entry
+ *    points never exit or throw.  sidon_last_error() describes the last error.
+ *
+ * The reference has no library API: its path sits behind static host helpers
+ * in src/main.cu that launch templated kernels from src/kernels.cuh.  Each
+ * entry point names the helper/kernel it replaces.
+ */
+#ifndef SIDON_H
+#define SIDON_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SIDON_OK = 0,
+  SIDON_ERR_ARG = 1,         /* bad size / null pointer / bad parameter */
+  SIDON_ERR_LDS = 2,         /* n needs more LDS than one workgroup may use
+                                (the reference returns false / exits, main.cu:173, :391-399) */
+  SIDON_ERR_HIP = 3,         /* a HIP runtime call failed */
+  SIDON_ERR_UNSUPPORTED = 4  /* configuration not built (e.g. block size) */
+} sidon_status;
+
+/* Initialise the per-device workspace of `device` (-1 = current device).
+ * Called implicitly on first use; no reference counterpart. */
+int sidon_init(int device);
+/* Human-readable text of the last error on this thread. */
+const char* sidon_last_error(void);
+const char* sidon_status_string(int status);
+/* Dynamic LDS bytes that `kernel` needs at length n with `block` threads
+ * (kernel: 0 local search fp32, 1 local search fp64, 2 eval, 3 Frank-Wolfe,
+ * 4 lp_grad fp32), or -1 if unsupported.  Counterpart of localSearchSmem /
+ * evalSmem / fwSmem (src/kernels.cuh:161-162, :379-380, :405-406). */
+long long sidon_lds_bytes(int kernel, int n, int block);
+/* Largest dynamic LDS per workgroup on the current device. */
+int sidon_lds_limit(void);
+
+/* ---- FP64 objective --------------------------------------------------- */
+
+/* F in fp64 from fp32 heights plus the count of near-active peaks
+ * b_k >= M (1 - activeTol); nActive may be NULL.
+ * Replaces runEval (src/main.cu:161-166) / kEval<float,double>
+ * (src/kernels.cuh:301-377).  F = 1e30 when sum a <= 0. */
+int sidon_eval_f64(const float* pop, int n, int P, double* F, int* nActive,
+                   double activeTol, hipStream_t stream);
+/* Uneven lengths: candidate p has ns[p] <= nmax heights at pop + p*stride
+ * (ns on the device).  No reference counterpart (one n per launch there). */
+int sidon_eval_f64_ragged(const float* pop, int stride, const int* ns, int P, int nmax,
+                          double* F, int* nActive, double activeTol, hipStream_t stream);
+/* Same for fp64 heights: kEval<double,double> (src/main.cu:843). */
+int sidon_eval_f64_d(const double* pop, int n, int P, double* F, int* nActive,
+                     double activeTol, hipStream_t stream);
+/* Correctly rounded F: every b_k and S rounded once from their exact values
+ * (error-free products, double-double sums with a certified error bound).
+ * ambiguous[p] counts sums whose rounding the bound could not certify (0 in
+ * practice).  With correctly rounded b and S the n -> 2n cell split preserves
+ * F bit for bit.  M, S, ambiguous may be NULL.  No reference counterpart
+ * (the reference checks refinement to 1e-12, src/main.cu:208-219). */
+int sidon_eval_exact(const float* pop, int n, int P, double* F, double* M, double* S,
+                     int* ambiguous, hipStream_t stream);
+
+/* ---- L^p-annealed projected Adam / mirror descent -------------------- */
+
+/* `iters` fused steps per candidate with `a` resident in LDS:
+ * b = a*a, M = max b, c = (b/M)^(p-1) (log1p form), R = sum u^p, G = corr(c,a),
+ * direction g = G/R - M; mode 0 projected Adam (step (lr/n) mhat/(sqrt(vhat)+1e-30),
+ * clamp >= 0), mode 1 mirror descent (a exp(clamp(-lr g/M, +-4))); then
+ * renormalise sum a = 1.  p and lr are annealed geometrically from
+ * (pStart, lrStart) to (pEnd, lrEnd).  Adam moments restart at zero every call;
+ * mom/vel (may be NULL) receive the final moments.  skip[p] != 0 leaves
+ * candidate p untouched; mode (may be NULL) overrides modeDefault per candidate.
+ * Replaces runLocalSearch (src/main.cu:152-159) / kLocalSearch<float>
+ * (src/kernels.cuh:177-289). */
+int sidon_local_search(float* pop, float* mom, float* vel, const int* skip, const int* mode,
+                       int n, int P, int iters, float pStart, float pEnd, float lrStart,
+                       float lrEnd, float beta1, float beta2, int modeDefault,
+                       hipStream_t stream);
+/* fp64 form: kLocalSearch<double> (src/main.cu:837-839). */
+int sidon_local_search_f64(double* pop, double* mom, double* vel, const int* skip,
+                           const int* mode, int n, int P, int iters, double pStart,
+                           double pEnd, double lrStart, double lrEnd, double beta1,
+                           double beta2, int modeDefault, hipStream_t stream);
+/* Uneven lengths in one persistent launch: candidate p occupies
+ * pop[p*stride .. p*stride + ns[p]) (ns on the device, ns[p] <= nmax <= stride);
+ * `order` (may be NULL) lists the processing order, longest first gives the
+ * best balance.  Workgroups pull candidates from a device work queue.  No
+ * reference counterpart (the reference runs one n per launch). */
+int sidon_local_search_ragged(float* pop, int stride, const int* ns, int P, int nmax,
+                              const int* order, const int* skip, const int* mode, int iters,
+                              float pStart, float pEnd, float lrStart, float lrEnd,
+                              float beta1, float beta2, int modeDefault, hipStream_t stream);
+/* Choose the workgroup size the fp32 local search uses at length n
+ * (256, 512 or 1024; 0 restores the built-in rule).  Set by the autotuner. */
+int sidon_set_ls_block(int n, int block);
+int sidon_get_ls_block(int n);
+
+/* ---- gradient of the smoothed objective ------------------------------ */
+
+/* J_p(a) = 2n ||a*a||_p / (sum a)^2 and its true gradient
+ * dJ/da_i = (4 n R^{1/p}/S^2)(G_i/R - M/S), recomputing b (never stored).
+ * a is [B x n]; J (length B) and grad ([B x n]) may be NULL independently.
+ * No reference counterpart (kLocalSearch forms only the scaled direction,
+ * src/kernels.cuh:165-172, :244). */
+int sidon_lp_grad(const float* a, int n, int B, float p, float* J, float* grad,
+                  hipStream_t stream);
+int sidon_lp_grad_f64(const double* a, int n, int B, double p, double* J, double* grad,
+                      hipStream_t stream);
+
+/* ---- Frank-Wolfe step solved by Chambolle-Pock ------------------------ */
+
+/* `rounds` Frank-Wolfe rounds per candidate: normalise, M = max a*a, solve
+ * max sum g s.t. (a*g)_k <= M, g >= 0 by `cpIters` PDHG iterations
+ * (tau = 0.9/n, sigma = 0.9n; conv + dual update, corr + primal update,
+ * extrapolation), normalise g, grid line search over t = tMax*s/nT,
+ * accept the best strict improvement.  Replaces runFrankWolfe
+ * (src/main.cu:170-178) / kFrankWolfe<float> (src/kernels.cuh:408-532);
+ * returns SIDON_ERR_LDS where the reference returns false. */
+int sidon_frank_wolfe(float* pop, const int* skip, int n, int P, int rounds, int cpIters,
+                      int nT, float tMax, hipStream_t stream);
+
+/* ---- cell split, variation, selection --------------------------------- */
+
+/* n -> 2n cell split dst[p][2i] = dst[p][2i+1] = src[p][i]; F is preserved.
+ * Replaces kRefine (src/kernels.cuh:538-547) and the host refine loop
+ * (src/main.cu:491-502). */
+int sidon_refine(const float* src, int n, int P, float* dst, hipStream_t stream);
+
+/* Offspring j = operator op[j] applied to parent p1[j]: op 2 multiplies each
+ * height by exp(strength[j] * z), z ~ N(0,1) from Philox4x32-10 keyed by
+ * (seed, gen, j, i); any other op clones.  Then clamp >= 0 and renormalise
+ * sum = 1 (uniform 1/n if the sum is not positive).  Replaces kVariation<float>
+ * op 2 and the clone branch (src/kernels.cuh:582-801, launch src/main.cu:737-742). */
+int sidon_variation(const float* pop, float* out, int n, int nOff, const int* p1,
+                    const int* p2, const int* op, const float* strength,
+                    unsigned long long seed, unsigned long long gen, hipStream_t stream);
+/* Uneven-length form: parent p1[j] has ns[p1[j]] heights at pop + p1[j]*stride;
+ * the offspring inherits that length (ns_out[j]).  `gen` is read from device
+ * memory (*gen_dev) so the launch can be replayed from a graph. */
+int sidon_variation_ragged(const float* pop, const int* ns, float* out, int* ns_out,
+                           int stride, int nOff, const int* p1, const int* p2, const int* op,
+                           const float* strength, unsigned long long seed,
+                           const unsigned long long* gen_dev, hipStream_t stream);
+
+/* Island selection plan on the device: rank each island (P/NI members) by
+ * fit (ties by index); every migrate_every generations (gen > 0) the best of
+ * island i replaces the worst of island i+1 (vector, length and fit; pass
+ * pop/ns/stride, ns may be NULL for uniform n); the first `elite` slots of each
+ * island clone its ranked elites (op 20); the others take two binary
+ * tournaments inside the island (op 2, strength sigma) drawn from Philox keyed
+ * by (seed, gen, slot).  `gen` is read from *gen_dev.  fit is updated in place
+ * for migrated slots.  Replaces the host selection of src/main.cu:650-735
+ * (the operator is fixed to 2, as the configurations specify). */
+int sidon_select(double* fit, float* pop, int* ns, int stride, int P, int NI, int elite,
+                 int migrate_every, float sigma, unsigned long long seed,
+                 const unsigned long long* gen_dev, int* p1, int* p2, int* op,
+                 float* strength, hipStream_t stream);
+
+/* Adds `delta` to the device generation counter *gen_dev (one-thread kernel,
+ * graph-safe). */
+int sidon_advance_gen(unsigned long long* gen_dev, unsigned long long delta,
+                      hipStream_t stream);
+
+/* Orders candidates by decreasing length (ties by index) for the ragged
+ * local search: order[0..P) on the device. */
+int sidon_order_by_length(const int* ns, int P, int* order, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SIDON_H */

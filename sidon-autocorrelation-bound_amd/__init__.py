@@ -1,0 +1,375 @@
+"""Host-side mirror of the reference's search-path interface over the gfx950 C ABI.
+
+The reference keeps its device state in ``struct Gpu`` (src/main.cu:107-144)
+and drives the kernels through ``runLocalSearch`` / ``runEval`` /
+``runFrankWolfe`` (src/main.cu:146-184) plus an inline ``kVariation`` launch
+(src/main.cu:737-742).  This module offers the same names and argument meaning
+over ``libsidon_amd.so`` (include/sidon.h): torch tensors provide device memory
+and the current HIP stream, every compute call goes through the C ABI, and a
+failing status raises :class:`SidonError`.  There is no CPU fallback: if the
+HIP library is missing or no GPU is visible, calls raise.
+
+Import it with ``importlib.import_module("sidon-autocorrelation-bound_amd")``
+(the directory name is not a Python identifier).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import torch
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libsidon_amd.so"
+INCLUDE = HERE.parent / "include" / "sidon.h"
+
+SIDON_OK, SIDON_ERR_ARG, SIDON_ERR_LDS, SIDON_ERR_HIP, SIDON_ERR_UNSUPPORTED = range(5)
+
+# every entry point include/sidon.h declares
+EXPORTS = (
+    "sidon_init", "sidon_last_error", "sidon_status_string", "sidon_lds_bytes",
+    "sidon_lds_limit", "sidon_eval_f64", "sidon_eval_f64_ragged", "sidon_eval_f64_d",
+    "sidon_eval_exact", "sidon_local_search", "sidon_local_search_f64",
+    "sidon_local_search_ragged", "sidon_set_ls_block", "sidon_get_ls_block", "sidon_lp_grad",
+    "sidon_lp_grad_f64", "sidon_frank_wolfe", "sidon_refine", "sidon_variation",
+    "sidon_variation_ragged", "sidon_select", "sidon_advance_gen", "sidon_order_by_length",
+)
+
+
+class SidonError(RuntimeError):
+    def __init__(self, status: int, where: str, detail: str = ""):
+        self.status = status
+        names = {1: "SIDON_ERR_ARG", 2: "SIDON_ERR_LDS", 3: "SIDON_ERR_HIP",
+                 4: "SIDON_ERR_UNSUPPORTED"}
+        super().__init__(f"{where}: {names.get(status, status)} {detail}".strip())
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile libsidon_amd.so for gfx950 (always recompiles out-of-date objects)."""
+    out = subprocess.run(["make", "-C", str(HERE), "-j8"], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("HIP build failed:\n" + out.stdout[-4000:] + out.stderr[-4000:])
+    if verbose:
+        print(out.stdout)
+    return LIB_PATH
+
+
+_lib = None
+
+_vp = C.c_void_p
+_i, _f, _d, _u64 = C.c_int, C.c_float, C.c_double, C.c_uint64
+
+
+def _declare(L):
+    sig = {
+        "sidon_init": ([_i], _i),
+        "sidon_last_error": ([], C.c_char_p),
+        "sidon_status_string": ([_i], C.c_char_p),
+        "sidon_lds_bytes": ([_i, _i, _i], C.c_longlong),
+        "sidon_lds_limit": ([], _i),
+        "sidon_eval_f64": ([_vp, _i, _i, _vp, _vp, _d, _vp], _i),
+        "sidon_eval_f64_ragged": ([_vp, _i, _vp, _i, _i, _vp, _vp, _d, _vp], _i),
+        "sidon_eval_f64_d": ([_vp, _i, _i, _vp, _vp, _d, _vp], _i),
+        "sidon_eval_exact": ([_vp, _i, _i, _vp, _vp, _vp, _vp, _vp], _i),
+        "sidon_local_search": ([_vp, _vp, _vp, _vp, _vp, _i, _i, _i] + [_f] * 6 + [_i, _vp], _i),
+        "sidon_local_search_f64": ([_vp, _vp, _vp, _vp, _vp, _i, _i, _i] + [_d] * 6 + [_i, _vp],
+                                   _i),
+        "sidon_local_search_ragged": ([_vp, _i, _vp, _i, _i, _vp, _vp, _vp, _i] + [_f] * 6 +
+                                      [_i, _vp], _i),
+        "sidon_set_ls_block": ([_i, _i], _i),
+        "sidon_get_ls_block": ([_i], _i),
+        "sidon_lp_grad": ([_vp, _i, _i, _f, _vp, _vp, _vp], _i),
+        "sidon_lp_grad_f64": ([_vp, _i, _i, _d, _vp, _vp, _vp], _i),
+        "sidon_frank_wolfe": ([_vp, _vp, _i, _i, _i, _i, _i, _f, _vp], _i),
+        "sidon_refine": ([_vp, _i, _i, _vp, _vp], _i),
+        "sidon_variation": ([_vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _u64, _u64, _vp], _i),
+        "sidon_variation_ragged": ([_vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _u64, _vp,
+                                    _vp], _i),
+        "sidon_select": ([_vp, _vp, _vp, _i, _i, _i, _i, _i, _f, _u64, _vp, _vp, _vp, _vp, _vp,
+                          _vp], _i),
+        "sidon_advance_gen": ([_vp, _u64, _vp], _i),
+        "sidon_order_by_length": ([_vp, _i, _vp, _vp], _i),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+
+
+def lib():
+    """The loaded C ABI.  Raises if the HIP library has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build the HIP library first "
+                               "(make -C sidon-autocorrelation-bound_amd)")
+        L = C.CDLL(str(LIB_PATH))
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _ptr(t) -> int | None:
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("device tensor required")
+    if not t.is_contiguous():
+        raise ValueError("contiguous tensor required")
+    return t.data_ptr()
+
+
+def _stream(stream=None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def _check(rc: int, where: str) -> None:
+    if rc != SIDON_OK:
+        detail = lib().sidon_last_error().decode(errors="replace")
+        raise SidonError(rc, where, detail)
+
+
+def _need_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("the Sidon HIP path needs a visible GPU; there is no CPU fallback")
+
+
+def init(device: int = -1) -> None:
+    _need_gpu()
+    _check(lib().sidon_init(device), "sidon_init")
+
+
+def lds_limit() -> int:
+    return lib().sidon_lds_limit()
+
+
+def lds_bytes(kernel: int, n: int, block: int) -> int:
+    return lib().sidon_lds_bytes(kernel, n, block)
+
+
+# ------------------------------------------------------------- raw calls ---
+
+def eval_f64(pop: torch.Tensor, active_tol: float = 1e-4, with_active: bool = True,
+             stream=None):
+    """F (fp64) and near-active peak counts of each row: kEval / runEval."""
+    _need_gpu()
+    P, n = pop.shape
+    F = torch.empty(P, dtype=torch.float64, device=pop.device)
+    na = torch.empty(P, dtype=torch.int32, device=pop.device) if with_active else None
+    if pop.dtype == torch.float32:
+        rc = lib().sidon_eval_f64(_ptr(pop), n, P, _ptr(F), _ptr(na), active_tol, _stream(stream))
+    elif pop.dtype == torch.float64:
+        rc = lib().sidon_eval_f64_d(_ptr(pop), n, P, _ptr(F), _ptr(na), active_tol,
+                                    _stream(stream))
+    else:
+        raise TypeError("pop must be float32 or float64")
+    _check(rc, "sidon_eval_f64")
+    return F, na
+
+
+def eval_f64_ragged(pop: torch.Tensor, ns: torch.Tensor, nmax: int, active_tol: float = 1e-4,
+                    stream=None):
+    _need_gpu()
+    P, stride = pop.shape
+    F = torch.empty(P, dtype=torch.float64, device=pop.device)
+    na = torch.empty(P, dtype=torch.int32, device=pop.device)
+    _check(lib().sidon_eval_f64_ragged(_ptr(pop), stride, _ptr(ns), P, nmax, _ptr(F), _ptr(na),
+                                       active_tol, _stream(stream)), "sidon_eval_f64_ragged")
+    return F, na
+
+
+def eval_exact(pop: torch.Tensor, stream=None):
+    """Correctly rounded F, M, S and the ambiguity count of each fp32 row."""
+    _need_gpu()
+    P, n = pop.shape
+    dev = pop.device
+    F = torch.empty(P, dtype=torch.float64, device=dev)
+    M = torch.empty(P, dtype=torch.float64, device=dev)
+    S = torch.empty(P, dtype=torch.float64, device=dev)
+    amb = torch.empty(P, dtype=torch.int32, device=dev)
+    _check(lib().sidon_eval_exact(_ptr(pop), n, P, _ptr(F), _ptr(M), _ptr(S), _ptr(amb),
+                                  _stream(stream)), "sidon_eval_exact")
+    return F, M, S, amb
+
+
+def local_search(pop: torch.Tensor, iters: int, p_start: float, p_end: float, lr_start: float,
+                 lr_end: float, beta1: float = 0.9, beta2: float = 0.999, mode: int = 0,
+                 mode_arr: torch.Tensor | None = None, skip: torch.Tensor | None = None,
+                 mom: torch.Tensor | None = None, vel: torch.Tensor | None = None,
+                 stream=None) -> None:
+    """In-place fused local search on every row of pop (float32 or float64)."""
+    _need_gpu()
+    P, n = pop.shape
+    if pop.dtype == torch.float32:
+        rc = lib().sidon_local_search(_ptr(pop), _ptr(mom), _ptr(vel), _ptr(skip), _ptr(mode_arr),
+                                      n, P, iters, p_start, p_end, lr_start, lr_end, beta1, beta2,
+                                      mode, _stream(stream))
+    elif pop.dtype == torch.float64:
+        rc = lib().sidon_local_search_f64(_ptr(pop), _ptr(mom), _ptr(vel), _ptr(skip),
+                                          _ptr(mode_arr), n, P, iters, p_start, p_end, lr_start,
+                                          lr_end, beta1, beta2, mode, _stream(stream))
+    else:
+        raise TypeError("pop must be float32 or float64")
+    _check(rc, "sidon_local_search")
+
+
+def local_search_ragged(pop: torch.Tensor, ns: torch.Tensor, nmax: int, iters: int,
+                        p_start: float, p_end: float, lr_start: float, lr_end: float,
+                        beta1: float = 0.9, beta2: float = 0.999, mode: int = 0,
+                        order: torch.Tensor | None = None, skip: torch.Tensor | None = None,
+                        mode_arr: torch.Tensor | None = None, stream=None) -> None:
+    _need_gpu()
+    P, stride = pop.shape
+    _check(lib().sidon_local_search_ragged(_ptr(pop), stride, _ptr(ns), P, nmax, _ptr(order),
+                                           _ptr(skip), _ptr(mode_arr), iters, p_start, p_end,
+                                           lr_start, lr_end, beta1, beta2, mode, _stream(stream)),
+           "sidon_local_search_ragged")
+
+
+def set_ls_block(n: int, block: int) -> None:
+    _check(lib().sidon_set_ls_block(n, block), "sidon_set_ls_block")
+
+
+def get_ls_block(n: int) -> int:
+    return lib().sidon_get_ls_block(n)
+
+
+def lp_grad(a: torch.Tensor, p: float, want_J: bool = True, want_grad: bool = True,
+            stream=None):
+    """J_p and dJ_p/da for every row (b recomputed on the device, never stored)."""
+    _need_gpu()
+    B, n = a.shape
+    J = torch.empty(B, dtype=a.dtype, device=a.device) if want_J else None
+    g = torch.empty_like(a) if want_grad else None
+    fn = lib().sidon_lp_grad if a.dtype == torch.float32 else lib().sidon_lp_grad_f64
+    _check(fn(_ptr(a), n, B, p, _ptr(J), _ptr(g), _stream(stream)), "sidon_lp_grad")
+    return J, g
+
+
+def frank_wolfe(pop: torch.Tensor, rounds: int, cp_iters: int, nT: int = 12,
+                t_max: float = 0.30, skip: torch.Tensor | None = None, stream=None) -> None:
+    _need_gpu()
+    P, n = pop.shape
+    _check(lib().sidon_frank_wolfe(_ptr(pop), _ptr(skip), n, P, rounds, cp_iters, nT, t_max,
+                                   _stream(stream)), "sidon_frank_wolfe")
+
+
+def refine(src: torch.Tensor, stream=None) -> torch.Tensor:
+    """n -> 2n cell split of every row."""
+    _need_gpu()
+    P, n = src.shape
+    dst = torch.empty(P, 2 * n, dtype=torch.float32, device=src.device)
+    _check(lib().sidon_refine(_ptr(src), n, P, _ptr(dst), _stream(stream)), "sidon_refine")
+    return dst
+
+
+def variation(pop: torch.Tensor, out: torch.Tensor, p1, p2, op, strength, seed: int, gen: int,
+              stream=None) -> None:
+    _need_gpu()
+    P, n = pop.shape
+    _check(lib().sidon_variation(_ptr(pop), _ptr(out), n, out.shape[0], _ptr(p1), _ptr(p2),
+                                 _ptr(op), _ptr(strength), seed, gen, _stream(stream)),
+           "sidon_variation")
+
+
+def variation_ragged(pop, ns, out, ns_out, p1, p2, op, strength, seed: int, gen_dev,
+                     stream=None) -> None:
+    _need_gpu()
+    stride = pop.shape[1]
+    _check(lib().sidon_variation_ragged(_ptr(pop), _ptr(ns), _ptr(out), _ptr(ns_out), stride,
+                                        out.shape[0], _ptr(p1), _ptr(p2), _ptr(op),
+                                        _ptr(strength), seed, _ptr(gen_dev), _stream(stream)),
+           "sidon_variation_ragged")
+
+
+def select(fit, pop, ns, NI: int, elite: int, migrate_every: int, sigma: float, seed: int,
+           gen_dev, p1, p2, op, strength, stream=None) -> None:
+    _need_gpu()
+    P = fit.shape[0]
+    stride = pop.shape[1] if pop is not None else 0
+    _check(lib().sidon_select(_ptr(fit), _ptr(pop), _ptr(ns), stride, P, NI, elite,
+                              migrate_every, sigma, seed, _ptr(gen_dev), _ptr(p1), _ptr(p2),
+                              _ptr(op), _ptr(strength), _stream(stream)), "sidon_select")
+
+
+def advance_gen(gen_dev, delta: int = 1, stream=None) -> None:
+    _check(lib().sidon_advance_gen(_ptr(gen_dev), delta, _stream(stream)), "sidon_advance_gen")
+
+
+def order_by_length(ns: torch.Tensor, order: torch.Tensor, stream=None) -> None:
+    _check(lib().sidon_order_by_length(_ptr(ns), ns.shape[0], _ptr(order), _stream(stream)),
+           "sidon_order_by_length")
+
+
+# ------------------------------------------- reference-shaped device state --
+
+class Gpu:
+    """Device state of one ladder level: the SoA buffers of src/main.cu:107-144."""
+
+    def __init__(self, n: int = 0, P: int = 0, device: str | torch.device = "cuda"):
+        self.device = torch.device(device)
+        self.n = self.P = 0
+        if n and P:
+            self.alloc(n, P)
+
+    def alloc(self, n: int, P: int) -> None:
+        _need_gpu()
+        dev = self.device
+        self.n, self.P = n, P
+        f32 = dict(dtype=torch.float32, device=dev)
+        i32 = dict(dtype=torch.int32, device=dev)
+        self.pop = torch.zeros(P, n, **f32)
+        self.nxt = torch.zeros(P, n, **f32)
+        self.mom = torch.zeros(P, n, **f32)
+        self.vel = torch.zeros(P, n, **f32)
+        self.fit = torch.zeros(P, dtype=torch.float64, device=dev)
+        self.nact = torch.zeros(P, **i32)
+        self.modeArr = torch.zeros(P, **i32)
+        self.skip = torch.zeros(P, **i32)
+        self.p1 = torch.zeros(P, **i32)
+        self.p2 = torch.zeros(P, **i32)
+        self.op = torch.zeros(P, **i32)
+        self.strength = torch.zeros(P, **f32)
+
+    def free(self) -> None:
+        for k in ("pop", "nxt", "mom", "vel", "fit", "nact", "modeArr", "skip", "p1", "p2", "op",
+                  "strength"):
+            setattr(self, k, None)
+
+
+def runLocalSearch(g: Gpu, iters: int, pA: float, pB: float, lrA: float, lrB: float,
+                   modeArr: torch.Tensor | None = None, skip: torch.Tensor | None = None,
+                   stream=None) -> None:
+    """src/main.cu:152-159: beta = (0.9, 0.999), default mode 0, moments into g.mom/g.vel."""
+    local_search(g.pop, iters, pA, pB, lrA, lrB, 0.9, 0.999, 0, mode_arr=modeArr, skip=skip,
+                 mom=g.mom, vel=g.vel, stream=stream)
+
+
+def runEval(g: Gpu, activeTol: float = 1e-4, stream=None) -> None:
+    """src/main.cu:161-166: fills g.fit (fp64 F) and g.nact."""
+    _need_gpu()
+    _check(lib().sidon_eval_f64(_ptr(g.pop), g.n, g.P, _ptr(g.fit), _ptr(g.nact), activeTol,
+                                _stream(stream)), "runEval")
+
+
+def runFrankWolfe(g: Gpu, rounds: int, pdhgIters: int, nT: int, tMax: float,
+                  skip: torch.Tensor | None = None, stream=None) -> bool:
+    """src/main.cu:170-178: False when n does not fit the workgroup's LDS."""
+    _need_gpu()
+    rc = lib().sidon_frank_wolfe(_ptr(g.pop), _ptr(skip), g.n, g.P, rounds, pdhgIters, nT, tMax,
+                                 _stream(stream))
+    if rc == SIDON_ERR_LDS:
+        return False
+    _check(rc, "runFrankWolfe")
+    return True
+
+
+def runVariation(g: Gpu, seed: int, gen: int, stream=None) -> None:
+    """src/main.cu:737-742: offspring of the (p1, op, strength) plan into g.nxt, then swap."""
+    variation(g.pop, g.nxt, g.p1, g.p2, g.op, g.strength, seed, gen, stream=stream)
+    g.pop, g.nxt = g.nxt, g.pop
+
+
+from .lp import LpObjective, lp_objective  # noqa: E402  (autograd harness)

@@ -1,0 +1,304 @@
+// common.hpp -- device building blocks shared by every Sidon kernel (gfx950).
+//
+// Objective: F(a) = 2n * max_k b_k / (sum a)^2, b = a*a (autoconvolution),
+// a >= 0.  The hot loops are the autoconvolution b_k = sum_i a_i a_{k-i} and
+// its adjoint, the correlation G_i = sum_j c_{i+j} a_j
+// (reference: src/kernels.cuh:92-158).
+//
+// MI355X design (not a translation of the reference's warp32 register tiles):
+//  * wave64 butterfly reductions; every lane ends with the identical value, so
+//    block reductions are deterministic;
+//  * both O(n^2) products run on the matrix cores as blocked Toeplitz /
+//    Hankel GEMMs (v_mfma_f32_16x16x4f32, v_mfma_f64_16x16x4f64):
+//      conv:  b[16K + t] = sum_d sum_s a[16d + t - s] * a[16(K-d) + s]
+//             -> for a fixed block offset d the 16x16 Toeplitz block
+//                H_d[t][s] = a[16d + t - s] multiplies 16 consecutive input
+//                segments at once (one MFMA tile = 16 output segments);
+//      corr:  G[16I + t] = sum_e sum_s c[16e + t + s] * a[16(e-I) + s]
+//             -> Hankel block C_e[t][s] = c[16e + t + s] times 16 segments;
+//  * the vector `a` lives in LDS in a padded segment layout (17 floats per
+//    16-element segment, 16 zero segments on each side) so that the MFMA
+//    B-operand reads (stride 16 across lanes in element space) hit distinct
+//    banks and every out-of-range index reads an exact zero;
+//  * the work of each product is cut into W equal contiguous ranges of
+//    (tile, block-offset) units, one per wave ("stream-K").  A tile is never
+//    split among more than two waves, and two partial sums added onto an
+//    exactly-zero buffer commute, so results are bit-reproducible no matter
+//    which wave finishes first.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sidon {
+
+constexpr int kWave = 64;
+constexpr int kSeg = 16;          // MFMA tile edge / segment length
+constexpr int kSegStride = 17;    // padded floats per segment in LDS
+constexpr int kPadSeg = 16;       // zero segments on each side of `a`
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Acc4;
+template <> struct Acc4<float> { using type = f32x4; };
+template <> struct Acc4<double> { using type = f64x4; };
+
+__device__ __forceinline__ f32x4 mfma16x16x4(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f64x4 mfma16x16x4(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// ------------------------------------------------------------- reductions --
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const T w = __shfl_xor(v, o, kWave);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Block reductions.  `slots` holds NW values; callers alternate between two
+// slot arrays so that no trailing barrier is needed (a slot array is only
+// rewritten after every thread has passed the barrier of the reduction that
+// used the other array).  Result is identical in every thread.
+template <typename T, int NW>
+__device__ __forceinline__ T block_sum(T v, T* slots) {
+  v = wave_sum(v);
+  if ((threadIdx.x & (kWave - 1)) == 0) slots[threadIdx.x / kWave] = v;
+  __syncthreads();
+  T r = slots[0];
+#pragma unroll
+  for (int i = 1; i < NW; ++i) r += slots[i];
+  return r;
+}
+template <typename T, int NW>
+__device__ __forceinline__ T block_max(T v, T* slots) {
+  v = wave_max(v);
+  if ((threadIdx.x & (kWave - 1)) == 0) slots[threadIdx.x / kWave] = v;
+  __syncthreads();
+  T r = slots[0];
+#pragma unroll
+  for (int i = 1; i < NW; ++i) r = slots[i] > r ? slots[i] : r;
+  return r;
+}
+
+// ------------------------------------------------------------ LDS layout --
+// Padded segment layout of a length-n vector: element x (x may be negative
+// or >= n) lives at pad_addr(x); valid for x in [-256, 16*(N16+16)).
+__host__ __device__ __forceinline__ int pad_addr(int x) {
+  const int y = x + kPadSeg * kSeg;
+  return (y >> 4) * kSegStride + (y & 15);
+}
+__host__ __device__ __forceinline__ int nseg16(int n) { return (n + kSeg - 1) / kSeg; }
+// floats (elements) of the padded vector buffer
+__host__ __device__ __forceinline__ int pad_len(int n) {
+  return (nseg16(n) + 2 * kPadSeg) * kSegStride;
+}
+// elements of the weight / autoconvolution buffer (normal layout, zero tail)
+__host__ __device__ __forceinline__ int cbuf_len(int n) { return kSeg * (2 * nseg16(n) + 16); }
+// correlation output tiles and buffer
+__host__ __device__ __forceinline__ int corr_tiles(int n) { return (nseg16(n) + 15) / 16; }
+__host__ __device__ __forceinline__ int gbuf_len(int n) { return 256 * corr_tiles(n); }
+// autoconvolution output tiles (16 output segments each)
+__host__ __device__ __forceinline__ int conv_tiles(int n) {
+  const int NK = (2 * n - 1 + 15) / 16;
+  return (NK + 15) / 16;
+}
+
+// Block offsets d of the Toeplitz blocks that tile j (output segments
+// K in [16j, 16j+16)) needs: d in [lo, hi].
+__device__ __forceinline__ void conv_drange(int j, int N16, int& lo, int& hi) {
+  const int K0 = 16 * j;
+  lo = K0 - N16 + 1 > 0 ? K0 - N16 + 1 : 0;
+  hi = K0 + 15 < N16 ? K0 + 15 : N16;
+}
+__device__ __forceinline__ int conv_cost(int j, int N16) {
+  int lo, hi;
+  conv_drange(j, N16, lo, hi);
+  return hi - lo + 1;
+}
+// corr tile j (outputs I in [16j, 16j+16)) needs Hankel blocks e in
+// [16j, 16j + 15 + N16 - 1].
+__device__ __forceinline__ int corr_cost(int N16) { return N16 + 15; }
+
+// A wave's share of a product: units [u0, u1) of the concatenated tile costs.
+struct Share {
+  int u0, u1;     // unit range
+  int j, base;    // first tile overlapping u0 and its starting unit
+  int active;     // 0: this wave has no share
+};
+
+// Splits total units U over at most `nwaves` waves so that no tile (cost <=
+// maxcost) is shared by more than two waves: each share is >= maxcost.
+__device__ __forceinline__ Share make_share(int U, int maxcost, int wave, int nwaves) {
+  Share s;
+  int W = maxcost > 0 ? U / maxcost : 1;
+  W = W < 1 ? 1 : (W > nwaves ? nwaves : W);
+  s.active = wave < W;
+  s.u0 = (int)((long long)U * wave / W);
+  s.u1 = (int)((long long)U * (wave + 1) / W);
+  s.j = 0;
+  s.base = 0;
+  return s;
+}
+
+// ------------------------------------------------- MFMA tile accumulators --
+// Toeplitz tile: acc[r] (lane = 16*kk + t) += sum over block offsets
+// d in [da, db) of sum_s sA[16d + t' - s] * sB[16(K0 + t - d) + s], i.e. the
+// 16x16 output block D[4kk + r][t] = (sA (*) sB)[16(K0 + t) + 4kk + r].
+template <typename T>
+__device__ __forceinline__ typename Acc4<T>::type conv_tile(const T* __restrict__ sA,
+                                                           const T* __restrict__ sB, int K0,
+                                                           int da, int db) {
+  using A4 = typename Acc4<T>::type;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int t = lane & 15, kk = lane >> 4;
+  A4 acc = {0, 0, 0, 0};
+  // A operand: H_d[t][s] = a[16d + t - s], s = s0 + kk
+  // B operand: b[16(K0 + t - d) + s]   (column = t)
+  int pa0 = pad_addr(16 * da + t - 0 - kk);
+  int pa1 = pad_addr(16 * da + t - 4 - kk);
+  int pa2 = pad_addr(16 * da + t - 8 - kk);
+  int pa3 = pad_addr(16 * da + t - 12 - kk);
+  int pb = pad_addr(16 * (K0 + t - da)) + kk;
+  for (int d = da; d < db; ++d) {
+    const T a0 = sA[pa0], a1 = sA[pa1], a2 = sA[pa2], a3 = sA[pa3];
+    const T b0 = sB[pb], b1 = sB[pb + 4], b2 = sB[pb + 8], b3 = sB[pb + 12];
+    acc = mfma16x16x4(a0, b0, acc);
+    acc = mfma16x16x4(a1, b1, acc);
+    acc = mfma16x16x4(a2, b2, acc);
+    acc = mfma16x16x4(a3, b3, acc);
+    pa0 += kSegStride; pa1 += kSegStride; pa2 += kSegStride; pa3 += kSegStride;
+    pb -= kSegStride;
+  }
+  return acc;
+}
+
+// Hankel tile: D[4kk + r][t] = sum over e in [ea, eb) of
+// sum_s c[16e + 4kk + r + s] * a[16(e - I0 - t) + s]  -> G[16(I0 + t) + 4kk + r].
+template <typename T>
+__device__ __forceinline__ typename Acc4<T>::type corr_tile(const T* __restrict__ c,
+                                                           const T* __restrict__ sA, int I0,
+                                                           int ea, int eb) {
+  using A4 = typename Acc4<T>::type;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int t = lane & 15, kk = lane >> 4;
+  A4 acc = {0, 0, 0, 0};
+  // A operand: C_e[t][s] = c[16e + t + s], s = s0 + kk
+  // B operand: a[16(e - I0 - t) + s]    (column = t)
+  int pc = 16 * ea + t + kk;
+  int pb = pad_addr(16 * (ea - I0 - t)) + kk;
+  for (int e = ea; e < eb; ++e) {
+    const T a0 = c[pc], a1 = c[pc + 4], a2 = c[pc + 8], a3 = c[pc + 12];
+    const T b0 = sA[pb], b1 = sA[pb + 4], b2 = sA[pb + 8], b3 = sA[pb + 12];
+    acc = mfma16x16x4(a0, b0, acc);
+    acc = mfma16x16x4(a1, b1, acc);
+    acc = mfma16x16x4(a2, b2, acc);
+    acc = mfma16x16x4(a3, b3, acc);
+    pc += 16;
+    pb += kSegStride;
+  }
+  return acc;
+}
+
+// Stores (whole tile) or atomically adds (half of a shared tile) a lane's 4
+// results at o[0..3].
+template <typename T, typename A4>
+__device__ __forceinline__ void put4(T* o, const A4& acc, bool whole) {
+  if (whole) {
+    o[0] = acc[0]; o[1] = acc[1]; o[2] = acc[2]; o[3] = acc[3];
+  } else {
+    atomicAdd(o + 0, acc[0]); atomicAdd(o + 1, acc[1]);
+    atomicAdd(o + 2, acc[2]); atomicAdd(o + 3, acc[3]);
+  }
+}
+
+// -------------------------------------------- MFMA autoconvolution (split) --
+// out[k] += (sA (*) sB)[k] for all k < 16 * 16 * conv_tiles(n), where sA and
+// sB are padded vectors of length n.  `out` must be exactly zero before the
+// call (two commuting partial sums per element at most).  Every wave calls
+// this with its wave index; returns without a barrier.
+template <typename T>
+__device__ void conv_mfma(const T* __restrict__ sA, const T* __restrict__ sB, T* out, int n,
+                          int wave, int nwaves) {
+  const int N16 = nseg16(n);
+  const int NT = conv_tiles(n);
+  int U = 0, maxc = 0;
+  for (int j = 0; j < NT; ++j) {
+    const int c = conv_cost(j, N16);
+    U += c;
+    maxc = c > maxc ? c : maxc;
+  }
+  Share sh = make_share(U, maxc, wave, nwaves);
+  if (!sh.active) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int t = lane & 15, kk = lane >> 4;
+  int j = 0, base = 0;
+  while (base + conv_cost(j, N16) <= sh.u0) { base += conv_cost(j, N16); ++j; }
+  while (base < sh.u1 && j < NT) {
+    int dlo, dhi;
+    conv_drange(j, N16, dlo, dhi);
+    const int cost = dhi - dlo + 1;
+    const int ua = sh.u0 > base ? sh.u0 : base;
+    const int ub = sh.u1 < base + cost ? sh.u1 : base + cost;
+    const int K0 = 16 * j;
+    const auto acc = conv_tile<T>(sA, sB, K0, dlo + (ua - base), dlo + (ub - base));
+    put4(out + 16 * (K0 + t) + 4 * kk, acc, (ua == base) && (ub == base + cost));
+    base += cost;
+    ++j;
+  }
+}
+
+// ---------------------------------------------- MFMA correlation (split) ---
+// out[i] += sum_j c[i + j] a[j] for i < 256 * corr_tiles(n); c in the normal
+// layout, zero for indices >= 2n-1 up to cbuf_len(n); a padded.  `out` must
+// be exactly zero before the call.
+template <typename T>
+__device__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* out, int n,
+                          int wave, int nwaves) {
+  const int N16 = nseg16(n);
+  const int NT = corr_tiles(n);
+  const int cost = corr_cost(N16);
+  const int U = NT * cost;
+  Share sh = make_share(U, cost, wave, nwaves);
+  if (!sh.active) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int t = lane & 15, kk = lane >> 4;
+  int j = sh.u0 / cost;
+  int base = j * cost;
+  while (base < sh.u1 && j < NT) {
+    const int ua = sh.u0 > base ? sh.u0 : base;
+    const int ub = sh.u1 < base + cost ? sh.u1 : base + cost;
+    const int I0 = 16 * j;
+    const auto acc = corr_tile<T>(c, sA, I0, I0 + (ua - base), I0 + (ub - base));
+    put4(out + 16 * (I0 + t) + 4 * kk, acc, (ua == base) && (ub == base + cost));
+    base += cost;
+    ++j;
+  }
+}
+
+// ------------------------------------------------------------ L^p weights --
+// c = (b/M)^(p-1) in the log1p-stable form exp((p-1) log1p((b-M)/M))
+// (reference: src/kernels.cuh:35-44, :221-230); u^p = c (1 + d).
+__device__ __forceinline__ float lp_weight(float d, float pm1) {
+  return d > -1.0f ? expf(pm1 * log1pf(d)) : 0.0f;
+}
+__device__ __forceinline__ double lp_weight(double d, double pm1) {
+  return d > -1.0 ? exp(pm1 * log1p(d)) : 0.0;
+}
+__device__ __forceinline__ float dexp(float x) { return expf(x); }
+__device__ __forceinline__ double dexp(double x) { return exp(x); }
+__device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
+
+}  // namespace sidon

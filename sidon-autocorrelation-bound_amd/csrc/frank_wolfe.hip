@@ -1,0 +1,250 @@
+// frank_wolfe.hip -- Frank-Wolfe step whose LP direction is solved by
+// Chambolle-Pock (PDHG), reusing the MFMA convolution / correlation tiles.
+//
+// Reference: src/kernels.cuh:382-532 (kFrankWolfe), launch src/main.cu:170-178.
+// For a (sum 1) and M = max a*a the LP  max sum g  s.t. (a*g)_k <= M, g >= 0
+// is solved as the saddle point max_g min_{lambda>=0} 1'g - lambda'(A g - M)
+// with tau = 0.9/n, sigma = 0.9 n (|A|_2 <= sum a = 1):
+//   lambda <- [lambda + sigma (a * gbar - M)]_+      (Toeplitz tiles, fused)
+//   g_new  <- [g + tau (1 - A' lambda)]_+            (Hankel tiles, fused)
+//   gbar   <- 2 g_new - g
+// then g is normalised and a grid line search picks t = tMax*s/nT.
+//
+// LDS (n = 8400: 144 KB): a and gbar in the padded segment layout, lambda in
+// the normal layout.  g lives in registers: the wave that owns a correlation
+// tile owns those 256 heights for the whole solve, so the primal update,
+// the extrapolation and the write of gbar are all in the tile epilogue.  The
+// dual update is likewise fused into the Toeplitz-tile epilogue; Toeplitz
+// tiles (trapezoidal costs) are dealt to waves longest-first.
+#include "common.hpp"
+#include "runtime.hpp"
+
+namespace sidon {
+namespace {
+
+constexpr int kMaxConvTiles = 80;   // n <= 10240
+
+template <int BD>
+__host__ __device__ inline long long fw_lds_bytes(int n) {
+  const long long f = 2LL * pad_len(n) + cbuf_len(n) + 2 * (BD / kWave);
+  return f * 4 + 4LL * kMaxConvTiles;
+}
+
+struct FwArgs {
+  float* pop;
+  const int* skip;
+  int n, rounds, cpIters, nT;
+  float tMax;
+};
+
+// GT: correlation tiles owned per wave (ceil(corr_tiles / waves)).
+template <int BD, int GT>
+__global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
+  constexpr int NW = BD / kWave;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int cand = blockIdx.x;
+  if (A.skip && A.skip[cand]) return;
+  const int n = A.n;
+  float* a = A.pop + (size_t)cand * n;
+  float* sA = reinterpret_cast<float*>(smem);
+  float* sH = sA + pad_len(n);     // gbar during PDHG, h during the line search
+  float* sL = sH + pad_len(n);     // lambda / autoconvolution
+  float* slots = sL + cbuf_len(n);
+  int* owner = reinterpret_cast<int*>(slots + 2 * NW);  // Toeplitz tile -> wave
+  const int tid = threadIdx.x, wave = tid / kWave;
+  const int lane = tid & (kWave - 1), t = lane & 15, kk = lane >> 4;
+  const int LA = pad_len(n), LC = cbuf_len(n);
+  const int N16 = nseg16(n), NTc = conv_tiles(n), NTg = corr_tiles(n);
+  const int nb = 2 * n - 1;
+
+  for (int i = tid; i < LA; i += BD) { sA[i] = 0.0f; sH[i] = 0.0f; }
+  for (int i = tid; i < LC; i += BD) sL[i] = 0.0f;
+  if (tid == 0) {  // longest-processing-time deal of Toeplitz tiles to waves
+    int load[NW];
+    for (int w = 0; w < NW; ++w) load[w] = 0;
+    for (int j = 0; j < NTc; ++j) owner[j] = -1;
+    for (int r = 0; r < NTc; ++r) {
+      int best = -1, bc = -1;
+      for (int j = 0; j < NTc; ++j)
+        if (owner[j] < 0 && conv_cost(j, N16) > bc) { bc = conv_cost(j, N16); best = j; }
+      int wmin = 0;
+      for (int w = 1; w < NW; ++w) if (load[w] < load[wmin]) wmin = w;
+      owner[best] = wmin;
+      load[wmin] += bc;
+    }
+  }
+  __syncthreads();
+  float sl = 0.0f;
+  for (int i = tid; i < n; i += BD) { const float x = a[i]; sA[pad_addr(i)] = x; sl += x; }
+  const float s = block_sum<float, NW>(sl, slots);
+  if (!(s > 0.0f)) return;
+  {
+    const float inv = 1.0f / s;
+    for (int i = tid; i < n; i += BD) sA[pad_addr(i)] *= inv;
+  }
+  __syncthreads();
+
+  const float tau = 0.9f / (float)n, sig = 0.9f * (float)n;
+  float g[GT][4];
+
+  for (int r = 0; r < A.rounds; ++r) {
+    conv_mfma<float>(sA, sA, sL, n, wave, NW);
+    __syncthreads();
+    float loc = 0.0f;
+    for (int k = tid; k < nb; k += BD) loc = sL[k] > loc ? sL[k] : loc;
+    const float M = block_max<float, NW>(loc, slots + NW);
+    if (!(M > 0.0f)) break;
+
+    // g = gbar = a, lambda = 0
+#pragma unroll
+    for (int u = 0; u < GT; ++u) {
+      const int j = wave + NW * u;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = 16 * (16 * j + t) + 4 * kk + q;
+        g[u][q] = (j < NTg && i < n) ? sA[pad_addr(i)] : 0.0f;
+      }
+    }
+    for (int i = tid; i < n; i += BD) sH[pad_addr(i)] = sA[pad_addr(i)];
+    for (int k = tid; k < LC; k += BD) sL[k] = 0.0f;
+    __syncthreads();
+
+    for (int it = 0; it < A.cpIters; ++it) {
+      // dual: lambda_k <- [lambda_k + sigma ((a * gbar)_k - M)]_+
+      for (int j = 0; j < NTc; ++j) {
+        if (owner[j] != wave) continue;
+        int dlo, dhi;
+        conv_drange(j, N16, dlo, dhi);
+        const f32x4 acc = conv_tile<float>(sA, sH, 16 * j, dlo, dhi + 1);
+        const int k0 = 16 * (16 * j + t) + 4 * kk;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int k = k0 + q;
+          if (k < nb) {
+            const float l = sL[k] + sig * (acc[q] - M);
+            sL[k] = l > 0.0f ? l : 0.0f;
+          }
+        }
+      }
+      __syncthreads();
+      // primal: g_new <- [g + tau (1 - A' lambda)]_+, gbar <- 2 g_new - g
+#pragma unroll
+      for (int u = 0; u < GT; ++u) {
+        const int j = wave + NW * u;
+        if (j >= NTg) continue;
+        const int I0 = 16 * j;
+        const f32x4 c = corr_tile<float>(sL, sA, I0, I0, I0 + N16 + 15);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int i = 16 * (I0 + t) + 4 * kk + q;
+          if (i < n) {
+            float gn = g[u][q] + tau * (1.0f - c[q]);
+            gn = gn > 0.0f ? gn : 0.0f;
+            sH[pad_addr(i)] = 2.0f * gn - g[u][q];
+            g[u][q] = gn;
+          }
+        }
+      }
+      __syncthreads();
+    }
+
+    // normalise g
+    float sgl = 0.0f;
+#pragma unroll
+    for (int u = 0; u < GT; ++u)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sgl += g[u][q];
+    const float sg = block_sum<float, NW>(sgl, slots);
+    if (!(sg > 0.0f)) break;
+    const float ig = 1.0f / sg;
+#pragma unroll
+    for (int u = 0; u < GT; ++u)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) g[u][q] *= ig;
+
+    // line search over t = tMax * s / nT (sum h = 1 for every t)
+    float bestT = 0.0f, bestM = M;
+    for (int st = 1; st <= A.nT; ++st) {
+      const float tt = A.tMax * (float)st / (float)A.nT;
+#pragma unroll
+      for (int u = 0; u < GT; ++u) {
+        const int j = wave + NW * u;
+        if (j >= NTg) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int i = 16 * (16 * j + t) + 4 * kk + q;
+          if (i < n) sH[pad_addr(i)] = (1.0f - tt) * sA[pad_addr(i)] + tt * g[u][q];
+        }
+      }
+      for (int k = tid; k < LC; k += BD) sL[k] = 0.0f;
+      __syncthreads();
+      conv_mfma<float>(sH, sH, sL, n, wave, NW);
+      __syncthreads();
+      float lc = 0.0f;
+      for (int k = tid; k < nb; k += BD) lc = sL[k] > lc ? sL[k] : lc;
+      const float mm = block_max<float, NW>(lc, (st & 1) ? slots + NW : slots);
+      if (mm < bestM) { bestM = mm; bestT = tt; }
+    }
+    if (!(bestT > 0.0f)) break;
+#pragma unroll
+    for (int u = 0; u < GT; ++u) {
+      const int j = wave + NW * u;
+      if (j >= NTg) continue;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = 16 * (16 * j + t) + 4 * kk + q;
+        if (i < n) sA[pad_addr(i)] = (1.0f - bestT) * sA[pad_addr(i)] + bestT * g[u][q];
+      }
+    }
+    for (int k = tid; k < LC; k += BD) sL[k] = 0.0f;
+    __syncthreads();
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += BD) a[i] = sA[pad_addr(i)];
+}
+
+template <int BD, int GT>
+int launch_fw(const FwArgs& A, int P, hipStream_t stream) {
+  int rc = SIDON_OK;
+  DeviceState* ds = device_state(&rc);
+  if (!ds) return rc;
+  if (corr_tiles(A.n) > GT * (BD / kWave) || conv_tiles(A.n) > kMaxConvTiles)
+    return SIDON_ERR_UNSUPPORTED;
+  const long long lds = fw_lds_bytes<BD>(A.n);
+  if (lds > ds->lds_max) {
+    set_last_error_msg("frank_wolfe: n needs more LDS than a workgroup may use");
+    return SIDON_ERR_LDS;
+  }
+  auto kfn = k_frank_wolfe<BD, GT>;
+  SIDON_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kfn),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kfn, dim3(P), dim3(BD), (size_t)lds, stream, A);
+  SIDON_HIP_TRY(hipGetLastError());
+  return SIDON_OK;
+}
+
+}  // namespace
+}  // namespace sidon
+
+using namespace sidon;
+
+extern "C" long long sidon_fw_lds_bytes_impl(int n, int block) {
+  if (n < 1) return -1;
+  switch (block) {
+    case 256: return fw_lds_bytes<256>(n);
+    case 512: return fw_lds_bytes<512>(n);
+    case 1024: return fw_lds_bytes<1024>(n);
+    default: return -1;
+  }
+}
+
+extern "C" int sidon_frank_wolfe(float* pop, const int* skip, int n, int P, int rounds,
+                                 int cpIters, int nT, float tMax, hipStream_t stream) {
+  if (!pop || n < 1 || P < 0 || rounds < 0 || cpIters < 0 || nT < 1 || !(tMax > 0.0f))
+    return SIDON_ERR_ARG;
+  if (P == 0 || rounds == 0) return SIDON_OK;
+  FwArgs A{pop, skip, n, rounds, cpIters, nT, tMax};
+  if (n >= 2048) return launch_fw<1024, 3>(A, P, stream);
+  if (n >= 512) return launch_fw<512, 2>(A, P, stream);
+  return launch_fw<256, 2>(A, P, stream);
+}

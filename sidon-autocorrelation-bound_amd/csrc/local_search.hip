@@ -1,0 +1,406 @@
+// local_search.hip -- fused L^p-annealed projected Adam / mirror descent and
+// the gradient of the smoothed objective, on gfx950 matrix cores.
+//
+// Reference: src/kernels.cuh:160-289 (kLocalSearch), launch src/main.cu:152-159.
+// One workgroup owns one candidate for all `iters` steps; a (padded), the
+// autoconvolution / weight vector and the correlation output live in LDS
+// (about 140 KB at n = 8400), Adam moments and the new heights live in
+// registers (each thread owns heights i = tid + BD*q), so a step moves no HBM
+// bytes.  Per step:
+//   conv (MFMA) -> barrier -> M = max b -> weights c, R -> corr (MFMA) ->
+//   barrier -> Adam/mirror update -> S -> a = x/S, clear buffers -> barrier.
+#include <cmath>
+#include <cstdio>
+
+#include "common.hpp"
+#include "runtime.hpp"
+
+namespace sidon {
+
+template <typename T>
+struct LsArgs {
+  T* pop;
+  long long stride;     // elements between candidates
+  const int* ns;        // per-candidate length (nullptr: uniform n)
+  int n;                // uniform length
+  int P;
+  T* mom;
+  T* vel;
+  const int* skip;
+  const int* mode;
+  int modeDefault;
+  const int* order;     // processing order (nullptr: identity)
+  int* queue;           // work-queue counter (nullptr: one candidate per block)
+  int iters;
+  T pStart, pEnd, lrStart, lrEnd, beta1, beta2;
+};
+
+// Bytes of dynamic LDS for one candidate of length n.
+template <typename T>
+__host__ __device__ inline long long ls_lds_bytes(int n, int block) {
+  const long long elems = (long long)pad_len(n) + cbuf_len(n) + gbuf_len(n) + 2 * (block / kWave);
+  return elems * (long long)sizeof(T);
+}
+
+template <typename T, int BD, int QMAX>
+__device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) {
+  constexpr int NW = BD / kWave;
+  const int n = A.ns ? A.ns[cand] : A.n;
+  if (A.skip && A.skip[cand]) return;
+  if (n <= 0) return;
+  const int mode = A.mode ? A.mode[cand] : A.modeDefault;
+  T* __restrict__ a = A.pop + (size_t)cand * (size_t)A.stride;
+
+  T* sA = reinterpret_cast<T*>(smem);
+  T* sC = sA + pad_len(n);
+  T* sG = sC + cbuf_len(n);
+  T* slots = sG + gbuf_len(n);  // 2 * NW
+  const int tid = threadIdx.x, wave = tid / kWave;
+  const int LA = pad_len(n), LC = cbuf_len(n), LG = gbuf_len(n);
+
+  for (int i = tid; i < LA; i += BD) sA[i] = T(0);
+  for (int i = tid; i < LC; i += BD) sC[i] = T(0);
+  for (int i = tid; i < LG; i += BD) sG[i] = T(0);
+  __syncthreads();
+  for (int i = tid; i < n; i += BD) sA[pad_addr(i)] = a[i];
+
+  T m[QMAX], v[QMAX];
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) { m[q] = T(0); v[q] = T(0); }
+  __syncthreads();
+
+  const int nb = 2 * n - 1;
+  const T lnPr = T(log((double)A.pEnd / (double)A.pStart));
+  const T lnLr = T(log((double)A.lrEnd / (double)A.lrStart));
+  const T beta1 = A.beta1, beta2 = A.beta2;
+  T b1t = T(1), b2t = T(1);
+  const T nT = T(n);
+
+  for (int it = 0; it < A.iters; ++it) {
+    const T frac = A.iters > 1 ? T(it) / T(A.iters - 1) : T(0);
+    const T pexp = A.pStart * dexp(lnPr * frac);
+    const T lr = A.lrStart * dexp(lnLr * frac);
+
+    conv_mfma<T>(sA, sA, sC, n, wave, NW);
+    __syncthreads();
+
+    T loc = T(0);
+    for (int k = tid; k < nb; k += BD) loc = sC[k] > loc ? sC[k] : loc;
+    const T M = block_max<T, NW>(loc, slots);
+    if (!(M > T(0))) break;
+
+    const T pm1 = pexp - T(1);
+    T rloc = T(0);
+    for (int k = tid; k < nb; k += BD) {
+      const T d = (sC[k] - M) / M;
+      const T c = lp_weight(d, pm1);
+      rloc += c * (T(1) + d);
+      sC[k] = c;
+    }
+    const T R = block_sum<T, NW>(rloc, slots + NW);
+
+    b1t *= beta1;
+    b2t *= beta2;
+    const T bc1 = T(1) / (T(1) - b1t), bc2 = T(1) / (T(1) - b2t);
+
+    corr_mfma<T>(sC, sA, sG, n, wave, NW);
+    __syncthreads();
+
+    T sloc = T(0);
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) {
+      const int i = tid + BD * q;
+      if (i < n) {
+        const T ai = sA[pad_addr(i)];
+        const T gi = sG[i] / R - M;
+        T xx;
+        if (mode == 0) {
+          const T mm = beta1 * m[q] + (T(1) - beta1) * gi;
+          const T vv = beta2 * v[q] + (T(1) - beta2) * gi * gi;
+          m[q] = mm;
+          v[q] = vv;
+          const T step = (lr / nT) * (mm * bc1) / (dsqrt(vv * bc2) + T(1e-30));
+          xx = ai - step;
+          if (xx < T(0)) xx = T(0);
+        } else {
+          T e = -lr * (gi / M);
+          if (e > T(4)) e = T(4);
+          if (e < -T(4)) e = -T(4);
+          xx = ai * dexp(e);
+        }
+        sG[i] = xx;  // the owner thread parks the new height in place of G_i
+        sloc += xx;
+      }
+    }
+    T S = block_sum<T, NW>(sloc, slots);
+    const bool reset = !(S > T(0));
+    if (reset) S = T(1);
+    const T inv = T(1) / S;
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) {
+      const int i = tid + BD * q;
+      if (i < n) sA[pad_addr(i)] = reset ? T(1) / nT : sG[i] * inv;
+    }
+    for (int i = tid; i < LC; i += BD) sC[i] = T(0);
+    for (int i = tid; i < LG; i += BD) sG[i] = T(0);
+    __syncthreads();
+  }
+
+  for (int i = tid; i < n; i += BD) a[i] = sA[pad_addr(i)];
+  if (A.mom) {
+    T* mo = A.mom + (size_t)cand * (size_t)A.stride;
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) { const int i = tid + BD * q; if (i < n) mo[i] = m[q]; }
+  }
+  if (A.vel) {
+    T* vo = A.vel + (size_t)cand * (size_t)A.stride;
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) { const int i = tid + BD * q; if (i < n) vo[i] = v[q]; }
+  }
+  __syncthreads();  // LDS is reused by the next candidate of a persistent block
+}
+
+template <typename T, int BD, int QMAX>
+__global__ __launch_bounds__(BD) void k_local_search(LsArgs<T> A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int s_item;
+  if (!A.queue) {
+    const int cand = A.order ? A.order[blockIdx.x] : (int)blockIdx.x;
+    ls_candidate<T, BD, QMAX>(A, cand, smem);
+    return;
+  }
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(A.queue, 1);
+    __syncthreads();
+    const int item = s_item;
+    __syncthreads();
+    if (item >= A.P) break;
+    const int cand = A.order ? A.order[item] : item;
+    ls_candidate<T, BD, QMAX>(A, cand, smem);
+  }
+}
+
+// ---------------------------------------------------------------- lp_grad --
+template <typename T>
+struct GradArgs {
+  const T* a;
+  int n, B;
+  T p;
+  T* J;
+  T* grad;
+};
+
+template <typename T, int BD>
+__global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
+  constexpr int NW = BD / kWave;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = A.n, cand = blockIdx.x;
+  const T* a = A.a + (size_t)cand * n;
+  T* sA = reinterpret_cast<T*>(smem);
+  T* sC = sA + pad_len(n);
+  T* sG = sC + cbuf_len(n);
+  T* slots = sG + gbuf_len(n);
+  const int tid = threadIdx.x, wave = tid / kWave;
+  const int LA = pad_len(n), LC = cbuf_len(n), LG = gbuf_len(n);
+  for (int i = tid; i < LA; i += BD) sA[i] = T(0);
+  for (int i = tid; i < LC; i += BD) sC[i] = T(0);
+  for (int i = tid; i < LG; i += BD) sG[i] = T(0);
+  __syncthreads();
+  T sl = T(0);
+  for (int i = tid; i < n; i += BD) { const T x = a[i]; sA[pad_addr(i)] = x; sl += x; }
+  const T S = block_sum<T, NW>(sl, slots);  // barrier also publishes sA
+  const int nb = 2 * n - 1;
+  conv_mfma<T>(sA, sA, sC, n, wave, NW);
+  __syncthreads();
+  T loc = T(0);
+  for (int k = tid; k < nb; k += BD) loc = sC[k] > loc ? sC[k] : loc;
+  const T M = block_max<T, NW>(loc, slots + NW);
+  const T pm1 = A.p - T(1);
+  T rloc = T(0);
+  for (int k = tid; k < nb; k += BD) {
+    const T d = (sC[k] - M) / M;
+    const T c = lp_weight(d, pm1);
+    rloc += c * (T(1) + d);
+    sC[k] = c;
+  }
+  const T R = block_sum<T, NW>(rloc, slots);
+  const T Rp = dexp(log(R) / A.p);
+  if (A.J && tid == 0) A.J[cand] = T(2) * T(n) * M * Rp / (S * S);
+  if (!A.grad) return;
+  corr_mfma<T>(sC, sA, sG, n, wave, NW);
+  __syncthreads();
+  const T scale = T(4) * T(n) * Rp / (S * S);
+  const T MS = M / S;
+  T* g = A.grad + (size_t)cand * n;
+  for (int i = tid; i < n; i += BD) g[i] = scale * (sG[i] / R - MS);
+}
+
+// ------------------------------------------------------------- host side --
+namespace {
+
+// Per-length block-size override set by the autotuner (n -> block).
+struct BlockRule { int n, block; };
+BlockRule g_rules[64];
+int g_nrules = 0;
+
+int default_block(int n) { return n >= 3072 ? 1024 : (n >= 1024 ? 512 : 256); }
+
+int pick_block(int nmax) {
+  for (int r = 0; r < g_nrules; ++r)
+    if (g_rules[r].n == nmax) return g_rules[r].block;
+  return default_block(nmax);
+}
+
+template <typename T, int BD, int QMAX>
+int launch_ls(const LsArgs<T>& A, int nmax, int grid, hipStream_t stream) {
+  if (nmax > BD * QMAX) return SIDON_ERR_UNSUPPORTED;
+  int rc = SIDON_OK;
+  DeviceState* ds = device_state(&rc);
+  if (!ds) return rc;
+  const long long lds = ls_lds_bytes<T>(nmax, BD);
+  if (lds > ds->lds_max) {
+    set_last_error_msg("local search: n needs more LDS than a workgroup may use");
+    return SIDON_ERR_LDS;
+  }
+  auto kfn = k_local_search<T, BD, QMAX>;
+  SIDON_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kfn),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kfn, dim3(grid), dim3(BD), (size_t)lds, stream, A);
+  SIDON_HIP_TRY(hipGetLastError());
+  return SIDON_OK;
+}
+
+template <typename T>
+int dispatch_ls(LsArgs<T>& A, int nmax, int block, bool persistent, hipStream_t stream) {
+  int rc = SIDON_OK;
+  DeviceState* ds = device_state(&rc);
+  if (!ds) return rc;
+  int grid = A.P;
+  if (persistent) {
+    rc = reset_queue(ds, 0, stream);
+    if (rc) return rc;
+    A.queue = ds->queue;
+    grid = ds->cu_count < A.P ? ds->cu_count : A.P;  // LDS allows one block per CU at large n
+  }
+  switch (block) {
+    case 256: return launch_ls<T, 256, 16>(A, nmax, grid, stream);
+    case 512:
+      if constexpr (sizeof(T) == 4) return launch_ls<T, 512, 20>(A, nmax, grid, stream);
+      return launch_ls<T, 512, 12>(A, nmax, grid, stream);
+    case 1024:
+      if constexpr (sizeof(T) == 4) return launch_ls<T, 1024, 10>(A, nmax, grid, stream);
+      return SIDON_ERR_UNSUPPORTED;
+    default: return SIDON_ERR_UNSUPPORTED;
+  }
+}
+
+template <typename T>
+int check_schedule(int iters, T pS, T pE, T lS, T lE) {
+  if (iters < 0) return SIDON_ERR_ARG;
+  if (iters == 0) return SIDON_OK;
+  if (!(pS > T(0)) || !(pE > T(0)) || !(lS > T(0)) || !(lE > T(0))) return SIDON_ERR_ARG;
+  return SIDON_OK;
+}
+
+template <typename T, int BD>
+int launch_grad(const GradArgs<T>& A, hipStream_t stream) {
+  int rc = SIDON_OK;
+  DeviceState* ds = device_state(&rc);
+  if (!ds) return rc;
+  const long long lds = ls_lds_bytes<T>(A.n, BD);
+  if (lds > ds->lds_max) {
+    set_last_error_msg("lp_grad: n needs more LDS than a workgroup may use");
+    return SIDON_ERR_LDS;
+  }
+  auto kfn = k_lp_grad<T, BD>;
+  SIDON_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kfn),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kfn, dim3(A.B), dim3(BD), (size_t)lds, stream, A);
+  SIDON_HIP_TRY(hipGetLastError());
+  return SIDON_OK;
+}
+
+}  // namespace
+}  // namespace sidon
+
+using namespace sidon;
+
+extern "C" int sidon_set_ls_block(int n, int block) {
+  if (block != 0 && block != 256 && block != 512 && block != 1024) return SIDON_ERR_ARG;
+  for (int r = 0; r < g_nrules; ++r)
+    if (g_rules[r].n == n) {
+      if (block == 0) { g_rules[r] = g_rules[--g_nrules]; return SIDON_OK; }
+      g_rules[r].block = block;
+      return SIDON_OK;
+    }
+  if (block == 0) return SIDON_OK;
+  if (g_nrules >= 64) return SIDON_ERR_ARG;
+  g_rules[g_nrules++] = BlockRule{n, block};
+  return SIDON_OK;
+}
+
+extern "C" int sidon_get_ls_block(int n) { return pick_block(n); }
+
+extern "C" int sidon_local_search(float* pop, float* mom, float* vel, const int* skip,
+                                  const int* mode, int n, int P, int iters, float pStart,
+                                  float pEnd, float lrStart, float lrEnd, float beta1,
+                                  float beta2, int modeDefault, hipStream_t stream) {
+  if (!pop || n < 1 || P < 0) return SIDON_ERR_ARG;
+  if (int rc = check_schedule<float>(iters, pStart, pEnd, lrStart, lrEnd)) return rc;
+  if (P == 0 || iters == 0) return SIDON_OK;
+  LsArgs<float> A{pop, n, nullptr, n, P, mom, vel, skip, mode, modeDefault, nullptr, nullptr,
+                  iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2};
+  return dispatch_ls<float>(A, n, pick_block(n), false, stream);
+}
+
+extern "C" int sidon_local_search_f64(double* pop, double* mom, double* vel, const int* skip,
+                                      const int* mode, int n, int P, int iters, double pStart,
+                                      double pEnd, double lrStart, double lrEnd, double beta1,
+                                      double beta2, int modeDefault, hipStream_t stream) {
+  if (!pop || n < 1 || P < 0) return SIDON_ERR_ARG;
+  if (int rc = check_schedule<double>(iters, pStart, pEnd, lrStart, lrEnd)) return rc;
+  if (P == 0 || iters == 0) return SIDON_OK;
+  LsArgs<double> A{pop, n, nullptr, n, P, mom, vel, skip, mode, modeDefault, nullptr, nullptr,
+                   iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2};
+  const int block = n >= 1024 ? 512 : 256;
+  return dispatch_ls<double>(A, n, block, false, stream);
+}
+
+extern "C" int sidon_local_search_ragged(float* pop, int stride, const int* ns, int P, int nmax,
+                                         const int* order, const int* skip, const int* mode,
+                                         int iters, float pStart, float pEnd, float lrStart,
+                                         float lrEnd, float beta1, float beta2, int modeDefault,
+                                         hipStream_t stream) {
+  if (!pop || !ns || nmax < 1 || stride < nmax || P < 0) return SIDON_ERR_ARG;
+  if (int rc = check_schedule<float>(iters, pStart, pEnd, lrStart, lrEnd)) return rc;
+  if (P == 0 || iters == 0) return SIDON_OK;
+  LsArgs<float> A{pop, stride, ns, nmax, P, nullptr, nullptr, skip, mode, modeDefault, order,
+                  nullptr, iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2};
+  return dispatch_ls<float>(A, nmax, pick_block(nmax), true, stream);
+}
+
+extern "C" int sidon_lp_grad(const float* a, int n, int B, float p, float* J, float* grad,
+                             hipStream_t stream) {
+  if (!a || n < 1 || B < 0 || !(p >= 1.0f)) return SIDON_ERR_ARG;
+  if (B == 0 || (!J && !grad)) return SIDON_OK;
+  GradArgs<float> A{a, n, B, p, J, grad};
+  return n >= 2048 ? launch_grad<float, 1024>(A, stream) : launch_grad<float, 256>(A, stream);
+}
+
+extern "C" int sidon_lp_grad_f64(const double* a, int n, int B, double p, double* J,
+                                 double* grad, hipStream_t stream) {
+  if (!a || n < 1 || B < 0 || !(p >= 1.0)) return SIDON_ERR_ARG;
+  if (B == 0 || (!J && !grad)) return SIDON_OK;
+  GradArgs<double> A{a, n, B, p, J, grad};
+  return n >= 2048 ? launch_grad<double, 1024>(A, stream) : launch_grad<double, 256>(A, stream);
+}
+
+extern "C" long long sidon_ls_lds_bytes_impl(int kernel, int n, int block) {
+  if (n < 1) return -1;
+  switch (kernel) {
+    case 0: return ls_lds_bytes<float>(n, block);
+    case 1: return ls_lds_bytes<double>(n, block);
+    case 4: return ls_lds_bytes<float>(n, block);
+    default: return -1;
+  }
+}

@@ -1,0 +1,45 @@
+// runtime.hpp -- host-side helpers shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/sidon.h"
+
+namespace sidon {
+
+// Per-device persistent workspace (allocated once by sidon_init or on first
+// use on a device): work-queue counters for persistent launches.
+struct DeviceState {
+  int device = -1;
+  int lds_max = 0;        // max dynamic LDS bytes per workgroup
+  int cu_count = 0;
+  int* queue = nullptr;   // kQueueSlots work-queue counters
+};
+constexpr int kQueueSlots = 64;
+
+// Returns the state of the current device (initialising it on first use), or
+// nullptr with *rc set on failure.
+DeviceState* device_state(int* rc);
+
+inline int hip_status(hipError_t e) { return e == hipSuccess ? SIDON_OK : SIDON_ERR_HIP; }
+
+// Records the last HIP error string for sidon_last_error().
+void set_last_error(const char* where, hipError_t e);
+void set_last_error_msg(const char* msg);
+
+#define SIDON_HIP_TRY(call)                                    \
+  do {                                                         \
+    hipError_t _e = (call);                                    \
+    if (_e != hipSuccess) {                                    \
+      ::sidon::set_last_error(#call, _e);                      \
+      return SIDON_ERR_HIP;                                    \
+    }                                                          \
+  } while (0)
+
+// Resets work-queue counter `slot` to zero with a one-thread kernel (no
+// hipMemsetAsync, so the sequence is safe inside captured graphs).
+int reset_queue(DeviceState* ds, int slot, hipStream_t stream);
+
+}  // namespace sidon

@@ -1,0 +1,105 @@
+"""GPU fp64 objective vs the oracle: bit-exact for fp32 heights (same exact
+products, same ascending-i sums, sequential S as hostF), KATs, edge cases,
+certificates, correctly rounded F and the bit-exact cell split."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(x, dtype=torch.float32):
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype, device="cuda")
+
+
+@pytest.mark.parametrize("n", [512, 4096, 8400])
+def test_certificates_bitwise(S, O, certs, n):
+    print(f"[eval] certificate n={n}", flush=True)
+    a = certs[n]["a"]
+    F, na = S.eval_f64(dev(a[None, :]), active_tol=1e-9)
+    torch.cuda.synchronize()
+    assert F.item() == certs[n]["F"]          # stored value, bit for bit
+    assert F.item() == O.F_hostorder(a)
+    assert na.item() == O.verify_ld(a)["nActive"]
+
+
+def test_plain_config_vectors_bitwise(S, O, W):
+    """64 vectors at n=512 (a_i = 0 w.p. 0.33 else U(0,1)), rounded to fp32."""
+    print("[eval] plain config 64 x 512", flush=True)
+    a = W.plain_vectors(64, 512).astype(np.float32).astype(np.float64)
+    a[0] = 1.0  # constant -> F = 2
+    F, na = S.eval_f64(dev(a), active_tol=1e-4)
+    F = F.cpu().numpy()
+    want = np.array([O.F_hostorder(r) for r in a])
+    assert F[0] == 2.0
+    np.testing.assert_array_equal(F, want)
+    cnt = []
+    for r in a:
+        b = np.convolve(r, r)
+        cnt.append(int((b >= b.max() * (1 - 1e-4)).sum()))
+    np.testing.assert_array_equal(na.cpu().numpy(), cnt)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 15, 16, 17, 31, 33, 257, 1000, 2100, 8400, 9000])
+def test_ragged_sizes_bitwise(S, O, n):
+    print(f"[eval] size n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    a = rng.uniform(size=(3, n)).astype(np.float32).astype(np.float64)
+    a[1, rng.uniform(size=n) < 0.5] = 0.0
+    F, _ = S.eval_f64(dev(a), with_active=False)
+    np.testing.assert_array_equal(F.cpu().numpy(), [O.F_hostorder(r) for r in a])
+
+
+def test_edge_cases(S):
+    print("[eval] edge cases", flush=True)
+    a = np.zeros((4, 4))
+    a[1] = (1, 2, 2, 1)
+    a[2] = (1, 0, 0, 1)
+    a[3] = (0, 1, 0, 0)
+    F, _ = S.eval_f64(dev(a))
+    F = F.cpu().numpy()
+    assert F[0] == 1e30                         # sum <= 0 (src/kernels.cuh:376)
+    assert F[1] == 2 * 4 * 10 / 36.0 and F[2] == 4.0 and F[3] == 8.0
+
+
+def test_fp64_input(S, O, W):
+    print("[eval] fp64 input", flush=True)
+    a = W.plain_vectors(8, 300, seed=9)
+    F, _ = S.eval_f64(dev(a, torch.float64))
+    want = np.array([O.F_hostorder(r) for r in a])
+    np.testing.assert_allclose(F.cpu().numpy(), want, rtol=1e-13, atol=0)
+
+
+def test_ragged_entry(S, O):
+    print("[eval] ragged entry", flush=True)
+    rng = np.random.default_rng(4)
+    ns = np.array([600, 1200, 2100, 600, 17], dtype=np.int32)
+    stride = 2200
+    pop = np.zeros((ns.size, stride), np.float32)
+    for p, n in enumerate(ns):
+        pop[p, :n] = rng.uniform(size=n)
+        pop[p, n:] = 7.0  # garbage beyond the row length must be ignored
+    F, na = S.eval_f64_ragged(dev(pop), dev(ns, torch.int32), int(ns.max()))
+    want = [O.F_hostorder(pop[p, :n].astype(np.float64)) for p, n in enumerate(ns)]
+    np.testing.assert_array_equal(F.cpu().numpy(), want)
+
+
+def test_exact_F_bitwise_and_cell_split(S, O, W, certs):
+    print("[eval] exact F + split", flush=True)
+    rows = [W.structured_prior(1, 1050, seed=s)[0].astype(np.float64) for s in range(3)]
+    rows.append(certs[512]["a"])
+    for a in rows:
+        n = a.size
+        cur = dev(a[None, :])
+        want = O.F_exact(a)
+        for level in range(4):  # n -> 2n -> 4n -> 8n
+            F, M, Ssum, amb = S.eval_exact(cur)
+            assert amb.item() == 0
+            assert F.item() == want["F"], level    # correctly rounded, bit for bit
+            assert M.item() == want["M"] * 2 ** level   # b' = 2b exactly per split
+            cur = S.refine(cur)
+        assert cur.shape[1] == 16 * n
+    Fc, *_ = S.eval_exact(dev(certs[8400]["a"][None, :]))
+    assert Fc.item() == O.F_exact(certs[8400]["a"])["F"]

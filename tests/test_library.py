@@ -1,0 +1,61 @@
+"""The C-ABI library loads and exports every entry point include/sidon.h
+declares; argument validation runs before any device call.  No GPU."""
+from __future__ import annotations
+
+import ctypes
+import importlib
+import re
+
+import pytest
+
+from conftest import REPO
+
+
+def declared_entry_points():
+    text = (REPO / "include" / "sidon.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sidon_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_and_module_agree():
+    S = importlib.import_module("sidon-autocorrelation-bound_amd")
+    assert declared_entry_points() == sorted(S.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    S = importlib.import_module("sidon-autocorrelation-bound_amd")
+    L = ctypes.CDLL(str(S.LIB_PATH))
+    for name in declared_entry_points():
+        assert hasattr(L, name), name
+
+
+def test_argument_validation_without_device():
+    S = importlib.import_module("sidon-autocorrelation-bound_amd")
+    L = S.lib()
+    fake = ctypes.c_void_p(0x1000)
+    # n < 1, null population, bad schedule -> SIDON_ERR_ARG before any HIP call
+    assert L.sidon_local_search(fake, None, None, None, None, 0, 4, 10, 4.0, 8.0, 0.1, 0.01,
+                                0.9, 0.999, 0, None) == S.SIDON_ERR_ARG
+    assert L.sidon_local_search(None, None, None, None, None, 64, 4, 10, 4.0, 8.0, 0.1, 0.01,
+                                0.9, 0.999, 0, None) == S.SIDON_ERR_ARG
+    assert L.sidon_local_search(fake, None, None, None, None, 64, 4, 10, -4.0, 8.0, 0.1, 0.01,
+                                0.9, 0.999, 0, None) == S.SIDON_ERR_ARG
+    # P == 0 / iters == 0 are no-ops
+    assert L.sidon_local_search(fake, None, None, None, None, 64, 0, 10, 4.0, 8.0, 0.1, 0.01,
+                                0.9, 0.999, 0, None) == S.SIDON_OK
+    assert L.sidon_eval_f64(fake, 0, 1, fake, None, 1e-4, None) == S.SIDON_ERR_ARG
+    assert L.sidon_frank_wolfe(fake, None, 64, 1, 1, 10, 0, 0.3, None) == S.SIDON_ERR_ARG
+    assert L.sidon_select(fake, None, None, 0, 10, 3, 1, 0, 0.05, 1, fake, fake, fake, fake,
+                          fake, None) == S.SIDON_ERR_ARG  # P not divisible by NI
+    assert L.sidon_set_ls_block(4096, 300) == S.SIDON_ERR_ARG
+    assert S.lib().sidon_status_string(S.SIDON_ERR_LDS).startswith(b"length needs more LDS")
+
+
+def test_lds_footprints():
+    S = importlib.import_module("sidon-autocorrelation-bound_amd")
+    ls = S.lds_bytes(0, 8400, 1024)
+    assert 130_000 < ls <= 160 * 1024          # fits one MI355X workgroup (160 KiB)
+    assert S.lds_bytes(3, 8400, 1024) <= 160 * 1024
+    assert S.lds_bytes(2, 8400, 256) <= 160 * 1024
+    assert S.lds_bytes(1, 8400, 512) > 160 * 1024   # fp64 local search does not fit at 8400
+    assert S.lds_bytes(9, 100, 256) == -1
