@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=20260905)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-copy-mib", type=int, default=0,
+                    help="run one device copy of this many MiB first (calibrates the HBM "
+                         "byte counters of a profiler run)")
     return ap.parse_args()
 
 
@@ -126,6 +129,13 @@ def main() -> int:
     S.init(0)
     n, P, iters = args.n, args.P, args.iters
     sched = (4.0, 1e4, 0.085, 0.0015)
+
+    if args.profile_copy_mib:
+        src = torch.ones(args.profile_copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     start = torch.as_tensor(W.structured_prior(P, n, seed=args.seed), device="cuda")
     pop = start.clone()

@@ -39,9 +39,17 @@ constexpr int kPadSeg = 16;       // zero segments on each side of `a`
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
+// 16x16x4 accumulator and its lane layout: lane (kk = lane/16, t = lane%16)
+// holds output rows kRowMul*kk + kRowStride*r (r = 0..3) of column t.
 template <typename T> struct Acc4;
-template <> struct Acc4<float> { using type = f32x4; };
-template <> struct Acc4<double> { using type = f64x4; };
+template <> struct Acc4<float> {
+  using type = f32x4;
+  static constexpr int kRowMul = 4, kRowStride = 1;
+};
+template <> struct Acc4<double> {
+  using type = f64x4;
+  static constexpr int kRowMul = 1, kRowStride = 4;
+};
 
 __device__ __forceinline__ f32x4 mfma16x16x4(float a, float b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -153,9 +161,8 @@ __device__ __forceinline__ Share make_share(int U, int maxcost, int wave, int nw
 }
 
 // ------------------------------------------------- MFMA tile accumulators --
-// Toeplitz tile: acc[r] (lane = 16*kk + t) += sum over block offsets
-// d in [da, db) of sum_s sA[16d + t' - s] * sB[16(K0 + t - d) + s], i.e. the
-// 16x16 output block D[4kk + r][t] = (sA (*) sB)[16(K0 + t) + 4kk + r].
+// Toeplitz tile: the 16x16 output block D[row][t] = (sA (*) sB)[16(K0+t) + row]
+// accumulated over block offsets d in [da, db) (rows per lane: Acc4<T>).
 template <typename T>
 __device__ __forceinline__ typename Acc4<T>::type conv_tile(const T* __restrict__ sA,
                                                            const T* __restrict__ sB, int K0,
@@ -184,8 +191,8 @@ __device__ __forceinline__ typename Acc4<T>::type conv_tile(const T* __restrict_
   return acc;
 }
 
-// Hankel tile: D[4kk + r][t] = sum over e in [ea, eb) of
-// sum_s c[16e + 4kk + r + s] * a[16(e - I0 - t) + s]  -> G[16(I0 + t) + 4kk + r].
+// Hankel tile: D[row][t] = sum over e in [ea, eb) of
+// sum_s c[16e + row + s] * a[16(e - I0 - t) + s]  = G[16(I0 + t) + row].
 template <typename T>
 __device__ __forceinline__ typename Acc4<T>::type corr_tile(const T* __restrict__ c,
                                                            const T* __restrict__ sA, int I0,
@@ -212,14 +219,17 @@ __device__ __forceinline__ typename Acc4<T>::type corr_tile(const T* __restrict_
 }
 
 // Stores (whole tile) or atomically adds (half of a shared tile) a lane's 4
-// results at o[0..3].
+// results of output column t: out[16 * col + row(kk, r)].
 template <typename T, typename A4>
-__device__ __forceinline__ void put4(T* o, const A4& acc, bool whole) {
+__device__ __forceinline__ void put4(T* out, int col, const A4& acc, bool whole) {
+  const int kk = (threadIdx.x & (kWave - 1)) >> 4;
+  constexpr int RS = Acc4<T>::kRowStride;
+  T* o = out + 16 * col + Acc4<T>::kRowMul * kk;
   if (whole) {
-    o[0] = acc[0]; o[1] = acc[1]; o[2] = acc[2]; o[3] = acc[3];
+    o[0] = acc[0]; o[RS] = acc[1]; o[2 * RS] = acc[2]; o[3 * RS] = acc[3];
   } else {
-    atomicAdd(o + 0, acc[0]); atomicAdd(o + 1, acc[1]);
-    atomicAdd(o + 2, acc[2]); atomicAdd(o + 3, acc[3]);
+    atomicAdd(o, acc[0]); atomicAdd(o + RS, acc[1]);
+    atomicAdd(o + 2 * RS, acc[2]); atomicAdd(o + 3 * RS, acc[3]);
   }
 }
 
@@ -242,7 +252,7 @@ __device__ void conv_mfma(const T* __restrict__ sA, const T* __restrict__ sB, T*
   Share sh = make_share(U, maxc, wave, nwaves);
   if (!sh.active) return;
   const int lane = threadIdx.x & (kWave - 1);
-  const int t = lane & 15, kk = lane >> 4;
+  const int t = lane & 15;
   int j = 0, base = 0;
   while (base + conv_cost(j, N16) <= sh.u0) { base += conv_cost(j, N16); ++j; }
   while (base < sh.u1 && j < NT) {
@@ -253,7 +263,7 @@ __device__ void conv_mfma(const T* __restrict__ sA, const T* __restrict__ sB, T*
     const int ub = sh.u1 < base + cost ? sh.u1 : base + cost;
     const int K0 = 16 * j;
     const auto acc = conv_tile<T>(sA, sB, K0, dlo + (ua - base), dlo + (ub - base));
-    put4(out + 16 * (K0 + t) + 4 * kk, acc, (ua == base) && (ub == base + cost));
+    put4(out, K0 + t, acc, (ua == base) && (ub == base + cost));
     base += cost;
     ++j;
   }
@@ -273,7 +283,7 @@ __device__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* 
   Share sh = make_share(U, cost, wave, nwaves);
   if (!sh.active) return;
   const int lane = threadIdx.x & (kWave - 1);
-  const int t = lane & 15, kk = lane >> 4;
+  const int t = lane & 15;
   int j = sh.u0 / cost;
   int base = j * cost;
   while (base < sh.u1 && j < NT) {
@@ -281,7 +291,7 @@ __device__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* 
     const int ub = sh.u1 < base + cost ? sh.u1 : base + cost;
     const int I0 = 16 * j;
     const auto acc = corr_tile<T>(c, sA, I0, I0 + (ua - base), I0 + (ub - base));
-    put4(out + 16 * (I0 + t) + 4 * kk, acc, (ua == base) && (ub == base + cost));
+    put4(out, I0 + t, acc, (ua == base) && (ub == base + cost));
     base += cost;
     ++j;
   }

@@ -55,25 +55,41 @@ def plant_peaks(a: np.ndarray, n_peaks: int = 5, spread: float = 1e-7, cells: in
     values.  Returns (a_new, peak_indices)."""
     rng = np.random.default_rng(seed)
     a = np.array(a, dtype=np.float64)
+    n = a.size
     b = autoconv64(a)
     ks = np.argsort(b)[::-1][:n_peaks]
-    L = b[ks[0]]
-    target = L * (1.0 - spread * np.arange(n_peaks))
-    support = np.flatnonzero(a > 0)
-    for _ in range(8):
+    for _ in range(6):
+        # bring the top values down to the level of the n_peaks-th largest
+        # (lowering b can push no other value above the planted level)
+        L = b[ks[-1]]
+        target = L * (1.0 - spread * np.arange(n_peaks))
+        # free cells: for every target k, cells i with a_i > 0 and a_{k-i} > 0
+        # (so that db_k/da_i = 2 a_{k-i} is non-zero for that row)
+        free = set()
+        per = max(2, cells // n_peaks)
+        for k in ks:
+            i = np.arange(max(0, k - n + 1), min(k, n - 1) + 1)
+            ok = i[(a[i] > 0) & (a[k - i] > 0)]
+            if ok.size:
+                free.update(rng.choice(ok, size=min(per, ok.size), replace=False).tolist())
+        free = np.array(sorted(free))
+        for _ in range(12):
+            b = autoconv64(a)
+            r = target - b[ks]
+            if np.max(np.abs(r) / L) < 1e-14:
+                break
+            J = np.zeros((n_peaks, free.size))
+            for row, k in enumerate(ks):
+                idx = k - free
+                ok = (idx >= 0) & (idx < n)
+                J[row, ok] = 2.0 * a[idx[ok]]
+            delta = J.T @ np.linalg.lstsq(J @ J.T, r, rcond=None)[0]
+            a[free] = np.maximum(a[free] + delta, 0.0)
         b = autoconv64(a)
-        r = target - b[ks]
-        if np.max(np.abs(r) / L) < 1e-14:
+        if b.max() <= target[0] * (1.0 + 1e-15):
             break
-        free = rng.choice(support, size=min(cells, support.size), replace=False)
-        # db_k / da_i = 2 a_{k-i}
-        J = np.zeros((n_peaks, free.size))
-        for row, k in enumerate(ks):
-            idx = k - free
-            ok = (idx >= 0) & (idx < a.size)
-            J[row, ok] = 2.0 * a[idx[ok]]
-        delta = J.T @ np.linalg.lstsq(J @ J.T, r, rcond=None)[0]
-        a[free] = np.maximum(a[free] + delta, 0.0)
+        # another k rose above the planted level: plant on the new top values
+        ks = np.argsort(b)[::-1][:n_peaks]
     if round_fp32:
         a = a.astype(np.float32).astype(np.float64)
     b = autoconv64(a)

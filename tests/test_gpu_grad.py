@@ -25,12 +25,15 @@ def test_adjoint_planted_ties_n8400(S, O, W, p):
     the weights (b/M)^(p-1) near the planted ties move by ~(p-1) 1e-6: the
     tolerance grows with p."""
     print(f"[grad] n=8400 p={p}", flush=True)
-    base = W.structured_prior(4, 8400, seed=77)
+    base = W.structured_prior(8, 8400, seed=77)
     rows = []
-    for r in range(4):
+    for r in range(8):  # the first four rows where planting reached five ties
         a, near = W.plant_peaks(base[r], seed=r)
-        assert near.size >= 5
-        rows.append(a)
+        if near.size >= 5:
+            rows.append(a)
+        if len(rows) == 4:
+            break
+    assert len(rows) == 4
     A = np.stack(rows)
     t = dev(A).requires_grad_(True)
     J = S.lp_objective(t, p)
