@@ -16,10 +16,11 @@
 //                segments at once (one MFMA tile = 16 output segments);
 //      corr:  G[16I + t] = sum_e sum_s c[16e + t + s] * a[16(e-I) + s]
 //             -> Hankel block C_e[t][s] = c[16e + t + s] times 16 segments;
-//  * the vector `a` lives in LDS in a padded segment layout (17 floats per
+//  * the vector `a` lives in LDS in a padded segment layout (20 elements per
 //    16-element segment, 16 zero segments on each side) so that the MFMA
-//    B-operand reads (stride 16 across lanes in element space) hit distinct
-//    banks and every out-of-range index reads an exact zero;
+//    B-operand reads (one 16-byte read per lane, stride 16 heights across
+//    lanes) hit disjoint banks and every out-of-range index reads an exact
+//    zero;
 //  * the work of each product is cut into W equal contiguous ranges of
 //    (tile, block-offset) units, one per wave ("stream-K").  A tile is never
 //    split among more than two waves, and two partial sums added onto an
@@ -33,7 +34,7 @@ namespace sidon {
 
 constexpr int kWave = 64;
 constexpr int kSeg = 16;          // MFMA tile edge / segment length
-constexpr int kSegStride = 17;    // padded floats per segment in LDS
+constexpr int kSegStride = 20;    // padded elements per segment in LDS (16-byte aligned)
 constexpr int kPadSeg = 16;       // zero segments on each side of `a`
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -56,6 +57,12 @@ __device__ __forceinline__ f32x4 mfma16x16x4(float a, float b, f32x4 c) {
 }
 __device__ __forceinline__ f64x4 mfma16x16x4(double a, double b, f64x4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Wave index as a wave-uniform (scalar) value, so loops over a wave's share
+// of tiles compile to scalar control flow.
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
 }
 
 // ------------------------------------------------------------- reductions --
@@ -161,61 +168,170 @@ __device__ __forceinline__ Share make_share(int U, int maxcost, int wave, int nw
 }
 
 // ------------------------------------------------- MFMA tile accumulators --
+// The 16 reduction indices s of a block are dealt to the four MFMAs as
+// s = 4*kk + q (kk = lane/16 is the operand's k row, q = 0..3 the MFMA), so
+// the four B-operand values a lane feeds are four consecutive heights of one
+// segment: a single 16-byte LDS read (segments are 20 elements apart, so the
+// 16 columns of a row hit disjoint banks).  Operand reads for step d+1 are
+// issued before the MFMAs of step d, and even / odd steps accumulate into
+// separate registers so consecutive MFMAs do not wait on each other.
+
+__device__ __forceinline__ void load4(const float* p, float (&o)[4]) {
+  const float4 v = *reinterpret_cast<const float4*>(p);
+  o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+__device__ __forceinline__ void load4(const double* p, double (&o)[4]) {
+  const double2 v0 = *reinterpret_cast<const double2*>(p);
+  const double2 v1 = *reinterpret_cast<const double2*>(p + 2);
+  o[0] = v0.x; o[1] = v0.y; o[2] = v1.x; o[3] = v1.y;
+}
+
+// Two independent accumulations issued MFMA by MFMA in turn: a wave running
+// alone on its SIMD (the tail of a phase) still has two MFMAs in flight.
+template <typename A4, typename T>
+__device__ __forceinline__ void mfma4x2(const T (&a0)[4], const T (&b0)[4], A4& acc0,
+                                        const T (&a1)[4], const T (&b1)[4], A4& acc1) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    acc0 = mfma16x16x4(a0[q], b0[q], acc0);
+    acc1 = mfma16x16x4(a1[q], b1[q], acc1);
+  }
+}
+template <typename A4, typename T>
+__device__ __forceinline__ A4 mfma4(const T (&a)[4], const T (&b)[4], A4 acc) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) acc = mfma16x16x4(a[q], b[q], acc);
+  return acc;
+}
+
 // Toeplitz tile: the 16x16 output block D[row][t] = (sA (*) sB)[16(K0+t) + row]
 // accumulated over block offsets d in [da, db) (rows per lane: Acc4<T>).
+//   A operand (lane t, kk; MFMA q): H_d[t][s] = sA[16d + t - s]
+//   B operand:                      sB[16(K0 + t - d) + s]      (column = t)
+// Steps run in pairs (d -> acc0, d+1 -> acc1) with two operand sets used in
+// turn: the next pair's operands are read while the current pair multiplies.
+// Nothing is read outside [da, db).  LDS addresses are kept as integer
+// element indices so the reads stay ds_read.
 template <typename T>
 __device__ __forceinline__ typename Acc4<T>::type conv_tile(const T* __restrict__ sA,
                                                            const T* __restrict__ sB, int K0,
                                                            int da, int db) {
   using A4 = typename Acc4<T>::type;
+  constexpr int S = kSegStride;
   const int lane = threadIdx.x & (kWave - 1);
   const int t = lane & 15, kk = lane >> 4;
-  A4 acc = {0, 0, 0, 0};
-  // A operand: H_d[t][s] = a[16d + t - s], s = s0 + kk
-  // B operand: b[16(K0 + t - d) + s]   (column = t)
-  int pa0 = pad_addr(16 * da + t - 0 - kk);
-  int pa1 = pad_addr(16 * da + t - 4 - kk);
-  int pa2 = pad_addr(16 * da + t - 8 - kk);
-  int pa3 = pad_addr(16 * da + t - 12 - kk);
-  int pb = pad_addr(16 * (K0 + t - da)) + kk;
-  for (int d = da; d < db; ++d) {
-    const T a0 = sA[pa0], a1 = sA[pa1], a2 = sA[pa2], a3 = sA[pa3];
-    const T b0 = sB[pb], b1 = sB[pb + 4], b2 = sB[pb + 8], b3 = sB[pb + 12];
-    acc = mfma16x16x4(a0, b0, acc);
-    acc = mfma16x16x4(a1, b1, acc);
-    acc = mfma16x16x4(a2, b2, acc);
-    acc = mfma16x16x4(a3, b3, acc);
-    pa0 += kSegStride; pa1 += kSegStride; pa2 += kSegStride; pa3 += kSegStride;
-    pb -= kSegStride;
+  A4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+  int pa[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) pa[q] = pad_addr(16 * da + t - 4 * kk - q);
+  int pb = pad_addr(16 * (K0 + t - da)) + 4 * kk;
+  // operands of the pair of steps starting `off` steps after the current one
+  auto load_pair = [&](int off, T (&x0)[4], T (&y0)[4], T (&x1)[4], T (&y1)[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { x0[q] = sA[pa[q] + off * S]; x1[q] = sA[pa[q] + (off + 1) * S]; }
+    load4(sB + pb - off * S, y0);
+    load4(sB + pb - (off + 1) * S, y1);
+  };
+  const int cnt = db - da, pairs = cnt >> 1;
+  T a0[4], b0[4], a1[4], b1[4], a2[4], b2[4], a3[4], b3[4];
+  if (pairs > 0) {
+    load_pair(0, a0, b0, a1, b1);
+    int k = 0;
+    for (; k + 2 < pairs; k += 2) {
+      load_pair(2, a2, b2, a3, b3);
+      __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the MFMAs
+      mfma4x2(a0, b0, acc0, a1, b1, acc1);
+      __builtin_amdgcn_sched_barrier(0);
+      load_pair(4, a0, b0, a1, b1);
+      __builtin_amdgcn_sched_barrier(0);
+      mfma4x2(a2, b2, acc0, a3, b3, acc1);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) pa[q] += 4 * S;
+      pb -= 4 * S;
+    }
+    if (k + 1 < pairs) {
+      load_pair(2, a2, b2, a3, b3);
+      __builtin_amdgcn_sched_barrier(0);
+      mfma4x2(a0, b0, acc0, a1, b1, acc1);
+      mfma4x2(a2, b2, acc0, a3, b3, acc1);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) pa[q] += 4 * S;
+      pb -= 4 * S;
+    } else {
+      mfma4x2(a0, b0, acc0, a1, b1, acc1);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) pa[q] += 2 * S;
+      pb -= 2 * S;
+    }
   }
-  return acc;
+  if (cnt & 1) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a0[q] = sA[pa[q]];
+    load4(sB + pb, b0);
+    acc0 = mfma4(a0, b0, acc0);
+  }
+  return acc0 + acc1;
 }
 
 // Hankel tile: D[row][t] = sum over e in [ea, eb) of
 // sum_s c[16e + row + s] * a[16(e - I0 - t) + s]  = G[16(I0 + t) + row].
+//   A operand: C_e[t][s] = c[16e + t + s]       (four consecutive elements)
+//   B operand: sA[16(e - I0 - t) + s]           (column = t)
 template <typename T>
 __device__ __forceinline__ typename Acc4<T>::type corr_tile(const T* __restrict__ c,
                                                            const T* __restrict__ sA, int I0,
                                                            int ea, int eb) {
   using A4 = typename Acc4<T>::type;
+  constexpr int S = kSegStride;
   const int lane = threadIdx.x & (kWave - 1);
   const int t = lane & 15, kk = lane >> 4;
-  A4 acc = {0, 0, 0, 0};
-  // A operand: C_e[t][s] = c[16e + t + s], s = s0 + kk
-  // B operand: a[16(e - I0 - t) + s]    (column = t)
-  int pc = 16 * ea + t + kk;
-  int pb = pad_addr(16 * (ea - I0 - t)) + kk;
-  for (int e = ea; e < eb; ++e) {
-    const T a0 = c[pc], a1 = c[pc + 4], a2 = c[pc + 8], a3 = c[pc + 12];
-    const T b0 = sA[pb], b1 = sA[pb + 4], b2 = sA[pb + 8], b3 = sA[pb + 12];
-    acc = mfma16x16x4(a0, b0, acc);
-    acc = mfma16x16x4(a1, b1, acc);
-    acc = mfma16x16x4(a2, b2, acc);
-    acc = mfma16x16x4(a3, b3, acc);
-    pc += 16;
-    pb += kSegStride;
+  A4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+  int pc = 16 * ea + t + 4 * kk;
+  int pb = pad_addr(16 * (ea - I0 - t)) + 4 * kk;
+  auto load_pair = [&](int off, T (&x0)[4], T (&y0)[4], T (&x1)[4], T (&y1)[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { x0[q] = c[pc + 16 * off + q]; x1[q] = c[pc + 16 * (off + 1) + q]; }
+    load4(sA + pb + off * S, y0);
+    load4(sA + pb + (off + 1) * S, y1);
+  };
+  const int cnt = eb - ea, pairs = cnt >> 1;
+  T a0[4], b0[4], a1[4], b1[4], a2[4], b2[4], a3[4], b3[4];
+  if (pairs > 0) {
+    load_pair(0, a0, b0, a1, b1);
+    int k = 0;
+    for (; k + 2 < pairs; k += 2) {
+      load_pair(2, a2, b2, a3, b3);
+      __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the MFMAs
+      mfma4x2(a0, b0, acc0, a1, b1, acc1);
+      __builtin_amdgcn_sched_barrier(0);
+      load_pair(4, a0, b0, a1, b1);
+      __builtin_amdgcn_sched_barrier(0);
+      mfma4x2(a2, b2, acc0, a3, b3, acc1);
+      __builtin_amdgcn_sched_barrier(0);
+      pc += 64;
+      pb += 4 * S;
+    }
+    if (k + 1 < pairs) {
+      load_pair(2, a2, b2, a3, b3);
+      __builtin_amdgcn_sched_barrier(0);
+      mfma4x2(a0, b0, acc0, a1, b1, acc1);
+      mfma4x2(a2, b2, acc0, a3, b3, acc1);
+      pc += 64;
+      pb += 4 * S;
+    } else {
+      mfma4x2(a0, b0, acc0, a1, b1, acc1);
+      pc += 32;
+      pb += 2 * S;
+    }
   }
-  return acc;
+  if (cnt & 1) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a0[q] = c[pc + q];
+    load4(sA + pb, b0);
+    acc0 = mfma4(a0, b0, acc0);
+  }
+  return acc0 + acc1;
 }
 
 // Stores (whole tile) or atomically adds (half of a shared tile) a lane's 4

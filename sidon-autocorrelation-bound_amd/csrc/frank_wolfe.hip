@@ -51,7 +51,7 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
   float* sL = sH + pad_len(n);     // lambda / autoconvolution
   float* slots = sL + cbuf_len(n);
   int* owner = reinterpret_cast<int*>(slots + 2 * NW);  // Toeplitz tile -> wave
-  const int tid = threadIdx.x, wave = tid / kWave;
+  const int tid = threadIdx.x, wave = wave_id();
   const int lane = tid & (kWave - 1), t = lane & 15, kk = lane >> 4;
   const int LA = pad_len(n), LC = cbuf_len(n);
   const int N16 = nseg16(n), NTc = conv_tiles(n), NTg = corr_tiles(n);

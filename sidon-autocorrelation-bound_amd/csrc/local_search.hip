@@ -55,7 +55,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
   T* sC = sA + pad_len(n);
   T* sG = sC + cbuf_len(n);
   T* slots = sG + gbuf_len(n);  // 2 * NW
-  const int tid = threadIdx.x, wave = tid / kWave;
+  const int tid = threadIdx.x, wave = wave_id();
   const int LA = pad_len(n), LC = cbuf_len(n), LG = gbuf_len(n);
 
   for (int i = tid; i < LA; i += BD) sA[i] = T(0);
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   T* sC = sA + pad_len(n);
   T* sG = sC + cbuf_len(n);
   T* slots = sG + gbuf_len(n);
-  const int tid = threadIdx.x, wave = tid / kWave;
+  const int tid = threadIdx.x, wave = wave_id();
   const int LA = pad_len(n), LC = cbuf_len(n), LG = gbuf_len(n);
   for (int i = tid; i < LA; i += BD) sA[i] = T(0);
   for (int i = tid; i < LC; i += BD) sC[i] = T(0);
