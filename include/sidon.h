@@ -179,6 +179,54 @@ int sidon_advance_gen(unsigned long long* gen_dev, unsigned long long delta,
  * local search: order[0..P) on the device. */
 int sidon_order_by_length(const int* ns, int P, int* order, hipStream_t stream);
 
+/* ---- one generation as a HIP graph ------------------------------------ */
+
+/* Device buffers (caller-owned) and parameters of a generation of the island
+ * search.  Populations are [P x stride] fp32 with per-slot lengths ns (<= nmax);
+ * a generation reads population A (ns_a) and writes population B (ns_b), the
+ * next one the other way round.  gen points to a device uint64 counter. */
+typedef struct {
+  float* pop_a; float* pop_b;
+  int* ns_a; int* ns_b;
+  double* fit; int* nact;
+  int* p1; int* p2; int* op; float* strength;
+  int* order;
+  unsigned long long* gen;
+  int P, stride, nmax, islands, elite, migrate_every, iters;
+  float pStart, pEnd, lrStart, lrEnd, sigma;
+  double activeTol;
+  unsigned long long seed;
+} sidon_generation_desc;
+
+typedef struct sidon_generation sidon_generation;
+
+/* Captures the two generation graphs (A -> B and B -> A):
+ * order by length -> sidon_local_search_ragged (iters fused steps) ->
+ * sidon_eval_f64_ragged -> sidon_select -> sidon_variation_ragged ->
+ * sidon_advance_gen.  Allocates only the graph objects.  Replaces the host
+ * generation loop body of src/main.cu:586-785, whose ~10 blocking transfers
+ * per generation kept it off the device. */
+int sidon_generation_create(const sidon_generation_desc* desc, sidon_generation** out,
+                            hipStream_t stream);
+/* Replays `generations` generations on `stream`, alternating the graphs. */
+int sidon_generation_run(sidon_generation* g, int generations, hipStream_t stream);
+/* Which population holds the current generation: 0 = A, 1 = B. */
+int sidon_generation_current(const sidon_generation* g, int* parity);
+/* The same sequence enqueued eagerly (no graph), reading A (parity 0) or B. */
+int sidon_generation_enqueue(const sidon_generation_desc* desc, int parity, hipStream_t stream);
+int sidon_generation_destroy(sidon_generation* g);
+
+/* ---- autotuner ---------------------------------------------------------- */
+
+/* Times the fused local search at length n for each workgroup size (4, 8 or
+ * 16 waves per candidate) on `scratch` (P x n, overwritten; reset from
+ * `reference` before every trial when non-NULL), `repeats` timed runs of
+ * `iters` steps after one warm-up, and keeps the fastest as the rule for n
+ * (sidon_get_ls_block).  No reference counterpart (tile sizes are
+ * compile-time constants there, src/kernels.cuh:30-33). */
+int sidon_tune_local_search(float* scratch, const float* reference, int n, int P, int iters,
+                            int repeats, int* best_block, float* best_ms, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,7 +34,19 @@ EXPORTS = (
     "sidon_local_search_ragged", "sidon_set_ls_block", "sidon_get_ls_block", "sidon_lp_grad",
     "sidon_lp_grad_f64", "sidon_frank_wolfe", "sidon_refine", "sidon_variation",
     "sidon_variation_ragged", "sidon_select", "sidon_advance_gen", "sidon_order_by_length",
+    "sidon_generation_create", "sidon_generation_run", "sidon_generation_current",
+    "sidon_generation_enqueue", "sidon_generation_destroy", "sidon_tune_local_search",
 )
+
+
+class GenerationDesc(C.Structure):
+    """sidon_generation_desc (include/sidon.h)."""
+    _fields_ = [(name, C.c_void_p) for name in (
+        "pop_a", "pop_b", "ns_a", "ns_b", "fit", "nact", "p1", "p2", "op", "strength", "order",
+        "gen")] + [(name, C.c_int) for name in (
+            "P", "stride", "nmax", "islands", "elite", "migrate_every", "iters")] + [
+        (name, C.c_float) for name in ("pStart", "pEnd", "lrStart", "lrEnd", "sigma")] + [
+        ("activeTol", C.c_double), ("seed", C.c_uint64)]
 
 
 class SidonError(RuntimeError):
@@ -90,6 +102,13 @@ def _declare(L):
                           _vp], _i),
         "sidon_advance_gen": ([_vp, _u64, _vp], _i),
         "sidon_order_by_length": ([_vp, _i, _vp, _vp], _i),
+        "sidon_generation_create": ([C.POINTER(GenerationDesc), C.POINTER(_vp), _vp], _i),
+        "sidon_generation_run": ([_vp, _i, _vp], _i),
+        "sidon_generation_current": ([_vp, C.POINTER(_i)], _i),
+        "sidon_generation_enqueue": ([C.POINTER(GenerationDesc), _i, _vp], _i),
+        "sidon_generation_destroy": ([_vp], _i),
+        "sidon_tune_local_search": ([_vp, _vp, _i, _i, _i, _i, C.POINTER(_i), C.POINTER(_f), _vp],
+                                    _i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -301,6 +320,102 @@ def advance_gen(gen_dev, delta: int = 1, stream=None) -> None:
 def order_by_length(ns: torch.Tensor, order: torch.Tensor, stream=None) -> None:
     _check(lib().sidon_order_by_length(_ptr(ns), ns.shape[0], _ptr(order), _stream(stream)),
            "sidon_order_by_length")
+
+
+def tune_local_search(n: int, P: int = 64, iters: int = 20, repeats: int = 2, seed: int = 1,
+                      stream=None):
+    """Autotune the waves per candidate (workgroup size) of the fused local
+    search at length n on a structured-prior sample; the choice is kept by the
+    library for n.  Returns (block, ms per trial)."""
+    _need_gpu()
+    from .workloads import structured_prior
+    ref = torch.as_tensor(structured_prior(P, n, seed=seed), device="cuda")
+    scratch = torch.empty_like(ref)
+    blk, ms = C.c_int(), C.c_float()
+    _check(lib().sidon_tune_local_search(_ptr(scratch), _ptr(ref), n, P, iters, repeats,
+                                         C.byref(blk), C.byref(ms), _stream(stream)),
+           "sidon_tune_local_search")
+    return blk.value, ms.value
+
+
+class Generation:
+    """Island-search generations captured as HIP graphs (sidon_generation_*).
+
+    Buffers are torch tensors owned here; populations are [P x stride] with
+    per-slot lengths.  `run(k)` replays k generations; `population()` returns
+    the current (pop, ns) pair."""
+
+    def __init__(self, pop: torch.Tensor, ns: torch.Tensor, islands: int, iters: int,
+                 p_start: float = 4.0, p_end: float = 1e4, lr_start: float = 0.085,
+                 lr_end: float = 0.0015, sigma: float = 0.05, elite: int = 2,
+                 migrate_every: int = 12, seed: int = 20260905, active_tol: float = 1e-4,
+                 graph: bool = True, stream=None):
+        _need_gpu()
+        P, stride = pop.shape
+        dev = pop.device
+        i32 = dict(dtype=torch.int32, device=dev)
+        self.pop_a = pop.contiguous()
+        self.pop_b = torch.zeros_like(self.pop_a)
+        self.ns_a = ns.to(torch.int32).contiguous()
+        self.ns_b = self.ns_a.clone()
+        self.fit = torch.zeros(P, dtype=torch.float64, device=dev)
+        self.nact = torch.zeros(P, **i32)
+        self.p1 = torch.zeros(P, **i32)
+        self.p2 = torch.zeros(P, **i32)
+        self.op = torch.zeros(P, **i32)
+        self.strength = torch.zeros(P, dtype=torch.float32, device=dev)
+        self.order = torch.zeros(P, **i32)
+        self.gen = torch.zeros(1, dtype=torch.int64, device=dev)
+        nmax = int(self.ns_a.max().item())
+        d = GenerationDesc()
+        for name in ("pop_a", "pop_b", "ns_a", "ns_b", "fit", "nact", "p1", "p2", "op",
+                     "strength", "order", "gen"):
+            setattr(d, name, _ptr(getattr(self, name)))
+        d.P, d.stride, d.nmax, d.islands, d.elite = P, stride, nmax, islands, elite
+        d.migrate_every, d.iters = migrate_every, iters
+        d.pStart, d.pEnd, d.lrStart, d.lrEnd, d.sigma = p_start, p_end, lr_start, lr_end, sigma
+        d.activeTol, d.seed = active_tol, seed
+        self.desc = d
+        self.stream = stream if stream is not None else torch.cuda.current_stream()
+        self.handle = None
+        self.eager_parity = 0
+        if graph:
+            h = C.c_void_p()
+            _check(lib().sidon_generation_create(C.byref(d), C.byref(h), self.stream.cuda_stream),
+                   "sidon_generation_create")
+            self.handle = h
+
+    def run(self, generations: int = 1) -> None:
+        if self.handle is not None:
+            _check(lib().sidon_generation_run(self.handle, generations, self.stream.cuda_stream),
+                   "sidon_generation_run")
+        else:
+            for _ in range(generations):
+                _check(lib().sidon_generation_enqueue(C.byref(self.desc), self.eager_parity,
+                                                      self.stream.cuda_stream),
+                       "sidon_generation_enqueue")
+                self.eager_parity ^= 1
+
+    def parity(self) -> int:
+        if self.handle is None:
+            return self.eager_parity
+        p = C.c_int()
+        _check(lib().sidon_generation_current(self.handle, C.byref(p)), "sidon_generation_current")
+        return p.value
+
+    def population(self):
+        return (self.pop_b, self.ns_b) if self.parity() else (self.pop_a, self.ns_a)
+
+    def close(self) -> None:
+        if self.handle is not None:
+            lib().sidon_generation_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------- reference-shaped device state --

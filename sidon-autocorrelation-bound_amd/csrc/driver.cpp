@@ -1,0 +1,536 @@
+// driver.cpp -- C++ host driver over the C ABI (include/sidon.h).
+//
+// The counterpart of the reference's sidon.exe host side (src/main.cu:40-876)
+// for the paths this library implements.  Modes:
+//   --selftest               known-answer tests of src/main.cu:186-291 against the device
+//   --uneven                 island generations of uneven lengths as HIP graphs
+//   --ladder                 Frank-Wolfe (Chambolle-Pock) rounds + n -> 2n cell split
+//   --search                 ladder search: generations per level, split, best vector out
+// Every mode prints one JSON line per result.  All device work goes through
+// the C ABI; host code here only seeds inputs, reads results and writes files.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/sidon.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double seconds_since(Clock::time_point t0) {
+  return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
+#define CHECK_HIP(x)                                                                  \
+  do {                                                                                \
+    hipError_t _e = (x);                                                              \
+    if (_e != hipSuccess) {                                                           \
+      std::fprintf(stderr, "HIP error %s at %s:%d: %s\n", #x, __FILE__, __LINE__,     \
+                   hipGetErrorString(_e));                                            \
+      std::exit(3);                                                                   \
+    }                                                                                 \
+  } while (0)
+#define CHECK_SIDON(x)                                                                \
+  do {                                                                                \
+    int _r = (x);                                                                     \
+    if (_r != SIDON_OK) {                                                             \
+      std::fprintf(stderr, "%s failed (%s): %s\n", #x, sidon_status_string(_r),      \
+                   sidon_last_error());                                               \
+      std::exit(4);                                                                   \
+    }                                                                                 \
+  } while (0)
+
+// fp64 objective in hostF order (src/main.cu:94-105): the host-side recheck
+// the reference prints next to every reported bound.
+double host_F(const std::vector<double>& a) {
+  const int n = (int)a.size();
+  std::vector<double> b(2 * n - 1, 0.0);
+  for (int i = 0; i < n; ++i) {
+    if (a[i] == 0.0) continue;
+    for (int j = 0; j < n; ++j) b[i + j] += a[i] * a[j];
+  }
+  double M = 0.0;
+  for (double x : b) M = std::max(M, x);
+  double S = 0.0;
+  for (double x : a) S += x;
+  return 2.0 * n * M / (S * S);
+}
+
+// Structured prior: endpoint atom = 5 x body mean, zero cells [1, 0.1 n),
+// body LogNormal(0,1) (1 + 0.5 i/n) with 33 % exact zeros, sum 1.
+void structured_prior(std::mt19937_64& rng, int n, float* out) {
+  std::lognormal_distribution<double> LN(0.0, 1.0);
+  std::uniform_real_distribution<double> U(0.0, 1.0);
+  std::vector<double> a(n, 0.0);
+  const int gap = std::max(1, (int)(0.1 * n));
+  double sum = 0.0;
+  for (int i = gap; i < n; ++i) {
+    const double v = LN(rng) * (1.0 + 0.5 * i / n);
+    a[i] = U(rng) < 0.33 ? 0.0 : v;
+    sum += a[i];
+  }
+  a[0] = 5.0 * sum / std::max(1, n - gap);
+  double S = 0.0;
+  for (double x : a) S += x;
+  for (int i = 0; i < n; ++i) out[i] = (float)(a[i] / S);
+}
+
+template <typename T>
+T* dalloc(size_t count) {
+  T* p = nullptr;
+  CHECK_HIP(hipMalloc(&p, count * sizeof(T)));
+  CHECK_HIP(hipMemset(p, 0, count * sizeof(T)));
+  return p;
+}
+
+// saved-vector text format (src/main.cu:329-338)
+void save_vec(const std::string& path, const std::vector<double>& a, double F) {
+  FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) return;
+  std::fprintf(f, "# AlphaEvolve problem 2 -- autocorrelation / Sidon\n");
+  std::fprintf(f, "# F = 2*n*max_k b_k / (sum a)^2  is an upper bound on C\n");
+  std::fprintf(f, "n %zu\nF %.17e\n", a.size(), F);
+  for (double x : a) std::fprintf(f, "%.17g\n", x);
+  std::fclose(f);
+}
+
+// certificate JSON (the schema of certificates/*.json)
+void save_certificate(const std::string& path, const std::vector<double>& a, double F) {
+  FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) return;
+  int nz = 0;
+  for (double x : a) nz += x != 0.0;
+  std::fprintf(f,
+               "{\"problem\": \"AlphaEvolve problem 2 - an autocorrelation problem related to "
+               "Sidon sets\", \"claim\": \"C <= F, where C is the largest constant in the "
+               "problem statement\", \"formula\": \"F = 2*n*max_k b_k / (sum_i a_i)^2,  b_k = "
+               "sum_{i+j=k} a_i a_j,  a_i >= 0\", \"n\": %zu, \"F\": %.17g, "
+               "\"non_zero_cells\": %d, \"found_by\": \"sidon-autocorrelation-bound_amd "
+               "(MI355X)\", \"weights\": [",
+               a.size(), F, nz);
+  for (size_t i = 0; i < a.size(); ++i) std::fprintf(f, "%s%.17g", i ? ", " : "", a[i]);
+  std::fprintf(f, "]}\n");
+  std::fclose(f);
+}
+
+struct Args {
+  std::string mode;
+  int P = 512, islands = 8, iters = 200, gens = 3, elite = 2;
+  int fw_rounds = 50, cp_iters = 2000, nT = 12;
+  float sigma = 0.05f, t_max = 0.30f;
+  double seconds = 60.0;
+  unsigned long long seed = 20260905ULL;
+  std::vector<int> ladder;
+  std::vector<int> lengths{600, 1200, 2100, 4096, 8400};
+  std::string out = "best";
+  bool tune = true;
+};
+
+std::vector<int> parse_list(const char* s) {
+  std::vector<int> v;
+  std::string cur;
+  for (const char* p = s;; ++p) {
+    if (*p == ',' || *p == '\0') {
+      if (!cur.empty()) v.push_back(std::atoi(cur.c_str()));
+      cur.clear();
+      if (!*p) break;
+    } else {
+      cur.push_back(*p);
+    }
+  }
+  return v;
+}
+
+// ------------------------------------------------------------ self-test --
+int selftest() {
+  int bad = 0;
+  auto check = [&](const char* name, double got, double want, double tol) {
+    const bool ok = std::fabs(got - want) <= tol * std::max(1.0, std::fabs(want));
+    std::printf("  %-32s got %.15f want %.15f %s\n", name, got, want, ok ? "OK" : "FAIL");
+    bad += !ok;
+  };
+  auto gpu_F = [&](const std::vector<float>& h, int n, int P) {
+    float* d = dalloc<float>(h.size());
+    double* F = dalloc<double>(P);
+    CHECK_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    CHECK_SIDON(sidon_eval_f64(d, n, P, F, nullptr, 1e-9, 0));
+    std::vector<double> out(P);
+    CHECK_HIP(hipMemcpy(out.data(), F, P * sizeof(double), hipMemcpyDeviceToHost));
+    CHECK_HIP(hipFree(d));
+    CHECK_HIP(hipFree(F));
+    return out;
+  };
+  // (1)-(4) closed forms (src/main.cu:196-207), on the device
+  for (int n : {4, 8, 37, 128}) {
+    char nm[64];
+    std::snprintf(nm, sizeof nm, "constant n=%d", n);
+    check(nm, gpu_F(std::vector<float>(n, 1.0f), n, 1)[0], 2.0, 1e-15);
+  }
+  check("(1,2,2,1)", gpu_F({1, 2, 2, 1}, 4, 1)[0], 80.0 / 36.0, 1e-15);
+  check("(1,0,0,1)", gpu_F({1, 0, 0, 1}, 4, 1)[0], 4.0, 1e-15);
+  check("spike (0,1,0,0)", gpu_F({0, 1, 0, 0}, 4, 1)[0], 8.0, 1e-15);
+  // (5) refinement: correctly rounded F is preserved bit for bit
+  {
+    std::mt19937 rng(7);
+    std::uniform_real_distribution<float> U(0.0f, 1.0f);
+    for (int n : {23, 34, 45}) {
+      std::vector<float> h(n);
+      for (auto& x : h) x = U(rng);
+      float* d = dalloc<float>(n);
+      float* r = dalloc<float>(2 * n);
+      double* F = dalloc<double>(2);
+      int* amb = dalloc<int>(2);
+      CHECK_HIP(hipMemcpy(d, h.data(), n * sizeof(float), hipMemcpyHostToDevice));
+      CHECK_SIDON(sidon_refine(d, n, 1, r, 0));
+      CHECK_SIDON(sidon_eval_exact(d, n, 1, F, nullptr, nullptr, amb, 0));
+      CHECK_SIDON(sidon_eval_exact(r, 2 * n, 1, F + 1, nullptr, nullptr, amb + 1, 0));
+      double f[2];
+      int a2[2];
+      CHECK_HIP(hipMemcpy(f, F, sizeof f, hipMemcpyDeviceToHost));
+      CHECK_HIP(hipMemcpy(a2, amb, sizeof a2, hipMemcpyDeviceToHost));
+      char nm[64];
+      std::snprintf(nm, sizeof nm, "refine bit-exact n=%d", n);
+      check(nm, f[1], f[0], 0.0);
+      bad += (a2[0] != 0 || a2[1] != 0);
+      CHECK_HIP(hipFree(d)); CHECK_HIP(hipFree(r)); CHECK_HIP(hipFree(F)); CHECK_HIP(hipFree(amb));
+    }
+  }
+  // (7) device F vs host F on the same fp32 values (src/main.cu:227-256)
+  {
+    std::mt19937 rng(3);
+    std::uniform_real_distribution<double> U(0, 1);
+    const int n = 257, P = 5;
+    std::vector<float> h((size_t)n * P);
+    for (int p = 0; p < P; ++p)
+      for (int i = 0; i < n; ++i) h[(size_t)p * n + i] = p == 0 ? 1.0f : (float)U(rng);
+    const auto got = gpu_F(h, n, P);
+    for (int p = 0; p < P; ++p) {
+      std::vector<double> a(h.begin() + (size_t)p * n, h.begin() + (size_t)(p + 1) * n);
+      char nm[64];
+      std::snprintf(nm, sizeof nm, "device F individual %d", p);
+      check(nm, got[p], host_F(a), 0.0);  // identical order of operations: bit for bit
+    }
+  }
+  // (8) local search strictly improves (src/main.cu:257-288)
+  {
+    const int n = 128, P = 4;
+    std::mt19937 rng(5);
+    std::uniform_real_distribution<float> U(0.1f, 1.0f);
+    std::vector<float> h((size_t)n * P);
+    for (auto& x : h) x = U(rng);
+    const auto F0 = gpu_F(h, n, P);
+    float* d = dalloc<float>(h.size());
+    CHECK_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    CHECK_SIDON(sidon_local_search(d, nullptr, nullptr, nullptr, nullptr, n, P, 800, 4.0f,
+                                   300.0f, 0.06f, 0.002f, 0.9f, 0.999f, 0, 0));
+    CHECK_HIP(hipMemcpy(h.data(), d, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    CHECK_HIP(hipFree(d));
+    const auto F1 = gpu_F(h, n, P);
+    bool nonneg = true;
+    for (float x : h) nonneg &= (x >= 0.0f && std::isfinite(x));
+    for (int p = 0; p < P; ++p) {
+      std::printf("  local search %d: %.9f -> %.9f %s\n", p, F0[p], F1[p],
+                  F1[p] < F0[p] ? "OK" : "FAIL");
+      bad += !(F1[p] < F0[p]);
+    }
+    std::printf("  non-negativity preserved: %s\n", nonneg ? "OK" : "FAIL");
+    bad += !nonneg;
+  }
+  std::printf("{\"selftest\": \"%s\", \"failures\": %d}\n", bad ? "FAILED" : "passed", bad);
+  return bad ? 1 : 0;
+}
+
+// --------------------------------------------------- uneven island graph --
+int uneven(const Args& A) {
+  const int P = A.P, nmax = *std::max_element(A.lengths.begin(), A.lengths.end());
+  const int stride = nmax;
+  std::mt19937_64 rng(A.seed);
+  std::vector<int> ns(P);
+  std::uniform_int_distribution<int> pick(0, (int)A.lengths.size() - 1);
+  for (int p = 0; p < P; ++p) ns[p] = A.lengths[pick(rng)];
+  std::vector<float> h((size_t)P * stride, 0.0f);
+  for (int p = 0; p < P; ++p) structured_prior(rng, ns[p], &h[(size_t)p * stride]);
+
+  hipStream_t s;
+  CHECK_HIP(hipStreamCreate(&s));  // blocking: orders with the null-stream copies below
+  if (A.tune) {  // autotune waves per candidate at the longest length (it sets the launch)
+    float* scratch = dalloc<float>((size_t)64 * nmax);
+    float* ref = dalloc<float>((size_t)64 * nmax);
+    std::vector<float> r((size_t)64 * nmax);
+    for (int p = 0; p < 64; ++p) structured_prior(rng, nmax, &r[(size_t)p * nmax]);
+    CHECK_HIP(hipMemcpy(ref, r.data(), r.size() * sizeof(float), hipMemcpyHostToDevice));
+    int blk = 0;
+    float ms = 0.0f;
+    CHECK_SIDON(sidon_tune_local_search(scratch, ref, nmax, 64, 10, 2, &blk, &ms, s));
+    std::printf("{\"autotune\": {\"n\": %d, \"block\": %d, \"ms\": %.3f}}\n", nmax, blk, ms);
+    CHECK_HIP(hipFree(scratch));
+    CHECK_HIP(hipFree(ref));
+  }
+  sidon_generation_desc d{};
+  d.pop_a = dalloc<float>(h.size());
+  d.pop_b = dalloc<float>(h.size());
+  d.ns_a = dalloc<int>(P);
+  d.ns_b = dalloc<int>(P);
+  d.fit = dalloc<double>(P);
+  d.nact = dalloc<int>(P);
+  d.p1 = dalloc<int>(P);
+  d.p2 = dalloc<int>(P);
+  d.op = dalloc<int>(P);
+  d.strength = dalloc<float>(P);
+  d.order = dalloc<int>(P);
+  d.gen = dalloc<unsigned long long>(1);
+  d.P = P; d.stride = stride; d.nmax = nmax; d.islands = A.islands; d.elite = A.elite;
+  d.migrate_every = 12; d.iters = A.iters;
+  d.pStart = 400.0f; d.pEnd = 6e4f; d.lrStart = 0.012f; d.lrEnd = 0.0003f;  // warm tier, main.cu:629
+  d.sigma = A.sigma; d.activeTol = 1e-4; d.seed = A.seed;
+  CHECK_HIP(hipMemcpy(d.pop_a, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  CHECK_HIP(hipMemcpy(d.ns_a, ns.data(), P * sizeof(int), hipMemcpyHostToDevice));
+  CHECK_HIP(hipMemcpy(d.ns_b, ns.data(), P * sizeof(int), hipMemcpyHostToDevice));
+  sidon_generation* g = nullptr;
+  CHECK_SIDON(sidon_generation_create(&d, &g, s));
+  CHECK_SIDON(sidon_generation_run(g, 1, s));  // warm-up generation
+  CHECK_HIP(hipStreamSynchronize(s));
+  long long steps = 0;
+  for (int p = 0; p < P; ++p) steps += A.iters;  // candidate-steps per generation
+  hipEvent_t e0, e1;
+  CHECK_HIP(hipEventCreate(&e0));
+  CHECK_HIP(hipEventCreate(&e1));
+  CHECK_HIP(hipEventRecord(e0, s));
+  CHECK_SIDON(sidon_generation_run(g, A.gens, s));
+  CHECK_HIP(hipEventRecord(e1, s));
+  CHECK_HIP(hipEventSynchronize(e1));
+  CHECK_HIP(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+  // fp64 flops of the local search part: 4 n^2 per candidate-step
+  double flop = 0.0;
+  for (int p = 0; p < P; ++p) flop += 4.0 * ns[p] * (double)ns[p] * A.iters;
+  std::vector<double> fit(P);
+  CHECK_HIP(hipMemcpy(fit.data(), d.fit, P * sizeof(double), hipMemcpyDeviceToHost));
+  const double best = *std::min_element(fit.begin(), fit.end());
+  std::printf("{\"workload\": \"uneven islands: %d candidates, n in {600,1200,2100,4096,8400} "
+              "(initial draw), %d islands, %d Adam steps + mutation sigma=%.2f + tournament per "
+              "generation, HIP graph\", \"generations\": %d, \"ms_per_generation\": %.3f, "
+              "\"candidate_steps_per_s\": %.1f, \"local_search_tflops_initial_lengths\": %.2f, "
+              "\"best_F\": %.12f}\n",
+              P, A.islands, A.iters, A.sigma, A.gens, ms / A.gens,
+              (double)steps * A.gens / (ms / 1e3), flop * A.gens / (ms / 1e3) / 1e12, best);
+  CHECK_SIDON(sidon_generation_destroy(g));
+  return 0;
+}
+
+// ---------------------------------------------- Frank-Wolfe + split ladder --
+int ladder(const Args& A) {
+  const std::vector<int> L = A.ladder.empty() ? std::vector<int>{1050, 2100, 4200, 8400} : A.ladder;
+  const int P = A.P;
+  std::mt19937_64 rng(A.seed);
+  int n = L[0];
+  std::vector<float> h((size_t)P * n);
+  for (int p = 0; p < P; ++p) structured_prior(rng, n, &h[(size_t)p * n]);
+  float* pop = dalloc<float>((size_t)P * L.back());
+  float* nxt = dalloc<float>((size_t)P * L.back());
+  double* F = dalloc<double>(P);
+  double* F2 = dalloc<double>(P);
+  int* amb = dalloc<int>(P);
+  CHECK_HIP(hipMemcpy(pop, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  hipStream_t s;
+  CHECK_HIP(hipStreamCreate(&s));  // blocking: orders with the null-stream copies below
+  for (size_t lvl = 0; lvl < L.size(); ++lvl) {
+    n = L[lvl];
+    hipEvent_t e0, e1;
+    CHECK_HIP(hipEventCreate(&e0));
+    CHECK_HIP(hipEventCreate(&e1));
+    CHECK_SIDON(sidon_eval_f64(pop, n, P, F, nullptr, 1e-9, s));
+    CHECK_HIP(hipEventRecord(e0, s));
+    const int rc = sidon_frank_wolfe(pop, nullptr, n, P, A.fw_rounds, A.cp_iters, A.nT, A.t_max, s);
+    CHECK_HIP(hipEventRecord(e1, s));
+    if (rc == SIDON_ERR_LDS) {
+      std::printf("{\"level\": %d, \"frank_wolfe\": \"does not fit in LDS\"}\n", n);
+    } else {
+      CHECK_SIDON(rc);
+    }
+    CHECK_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    CHECK_SIDON(sidon_eval_f64(pop, n, P, F2, nullptr, 1e-9, s));
+    CHECK_HIP(hipStreamSynchronize(s));
+    std::vector<double> f0(P), f1(P);
+    CHECK_HIP(hipMemcpy(f0.data(), F, P * sizeof(double), hipMemcpyDeviceToHost));
+    CHECK_HIP(hipMemcpy(f1.data(), F2, P * sizeof(double), hipMemcpyDeviceToHost));
+    double m0 = 0, m1 = 0, b1 = 1e30;
+    for (int p = 0; p < P; ++p) { m0 += f0[p] / P; m1 += f1[p] / P; b1 = std::min(b1, f1[p]); }
+    const double cp_flop = 4.0 * n * (double)n * A.cp_iters * A.fw_rounds * P;
+    std::printf("{\"level\": %d, \"fw_rounds\": %d, \"cp_iters\": %d, \"ms\": %.1f, "
+                "\"cp_iterations_per_s\": %.1f, \"cp_tflops\": %.2f, \"mean_F_before\": %.9f, "
+                "\"mean_F_after\": %.9f, \"best_F\": %.12f}\n",
+                n, A.fw_rounds, A.cp_iters, ms, (double)A.cp_iters * A.fw_rounds * P / (ms / 1e3),
+                cp_flop / (ms / 1e3) / 1e12, m0, m1, b1);
+    if (lvl + 1 < L.size()) {
+      if (L[lvl + 1] != 2 * n) { std::fprintf(stderr, "ladder must double\n"); return 2; }
+      CHECK_SIDON(sidon_eval_exact(pop, n, P, F, nullptr, nullptr, nullptr, s));
+      CHECK_SIDON(sidon_refine(pop, n, P, nxt, s));
+      CHECK_SIDON(sidon_eval_exact(nxt, 2 * n, P, F2, nullptr, nullptr, amb, s));
+      CHECK_HIP(hipStreamSynchronize(s));
+      std::vector<double> a(P), b(P);
+      std::vector<int> am(P);
+      CHECK_HIP(hipMemcpy(a.data(), F, P * sizeof(double), hipMemcpyDeviceToHost));
+      CHECK_HIP(hipMemcpy(b.data(), F2, P * sizeof(double), hipMemcpyDeviceToHost));
+      CHECK_HIP(hipMemcpy(am.data(), amb, P * sizeof(int), hipMemcpyDeviceToHost));
+      int same = 0, ambig = 0;
+      for (int p = 0; p < P; ++p) { same += (a[p] == b[p]); ambig += am[p]; }
+      std::printf("{\"split\": \"%d -> %d\", \"F_bit_exact\": %d, \"of\": %d, \"ambiguous\": %d}\n",
+                  n, 2 * n, same, P, ambig);
+      std::swap(pop, nxt);
+    }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------- search --
+int search(const Args& A) {
+  const std::vector<int> L = A.ladder.empty() ? std::vector<int>{256, 512, 1024, 2048} : A.ladder;
+  const int P = A.P, nmax = L.back();
+  std::mt19937_64 rng(A.seed);
+  hipStream_t s;
+  CHECK_HIP(hipStreamCreate(&s));  // blocking: orders with the null-stream copies below
+  std::vector<float> h((size_t)P * nmax, 0.0f);
+  std::vector<int> ns(P, L[0]);
+  for (int p = 0; p < P; ++p) structured_prior(rng, L[0], &h[(size_t)p * nmax]);
+  sidon_generation_desc d{};
+  d.pop_a = dalloc<float>(h.size()); d.pop_b = dalloc<float>(h.size());
+  d.ns_a = dalloc<int>(P); d.ns_b = dalloc<int>(P);
+  d.fit = dalloc<double>(P); d.nact = dalloc<int>(P);
+  d.p1 = dalloc<int>(P); d.p2 = dalloc<int>(P); d.op = dalloc<int>(P);
+  d.strength = dalloc<float>(P); d.order = dalloc<int>(P);
+  d.gen = dalloc<unsigned long long>(1);
+  d.P = P; d.stride = nmax; d.islands = A.islands; d.elite = A.elite; d.migrate_every = 12;
+  d.iters = A.iters; d.sigma = A.sigma; d.activeTol = 1e-4; d.seed = A.seed;
+  const auto t0 = Clock::now();
+  double bestF = 1e30;
+  std::vector<double> bestA;
+  float* tmp = dalloc<float>(h.size());
+  for (size_t lvl = 0; lvl < L.size(); ++lvl) {
+    const int n = L[lvl];
+    if (lvl == 0) {
+      CHECK_HIP(hipMemcpy(d.pop_a, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    std::fill(ns.begin(), ns.end(), n);
+    CHECK_HIP(hipMemcpy(d.ns_a, ns.data(), P * sizeof(int), hipMemcpyHostToDevice));
+    CHECK_HIP(hipMemcpy(d.ns_b, ns.data(), P * sizeof(int), hipMemcpyHostToDevice));
+    d.nmax = n;
+    // cold anneal at the first level, warm refinement after a split (main.cu:627-629)
+    d.pStart = lvl == 0 ? 6.0f : 400.0f; d.pEnd = 2e4f;
+    d.lrStart = lvl == 0 ? 0.085f : 0.012f; d.lrEnd = lvl == 0 ? 0.0015f : 0.0003f;
+    sidon_generation* g = nullptr;
+    CHECK_SIDON(sidon_generation_create(&d, &g, s));
+    const double budget = A.seconds * (lvl + 1) / (L.size() * (L.size() + 1) / 2.0);
+    const auto tl = Clock::now();
+    int gens = 0;
+    while (gens < A.gens || seconds_since(tl) < budget) {
+      CHECK_SIDON(sidon_generation_run(g, 1, s));
+      ++gens;
+      if (gens >= A.gens && seconds_since(tl) >= budget) break;
+      if (gens % 4 == 0) CHECK_HIP(hipStreamSynchronize(s));
+    }
+    CHECK_HIP(hipStreamSynchronize(s));
+    int parity = 0;
+    CHECK_SIDON(sidon_generation_current(g, &parity));
+    float* cur = parity ? d.pop_b : d.pop_a;
+    // fp64 F of the current population (the fit buffer holds the pre-mutation values)
+    CHECK_SIDON(sidon_eval_f64_ragged(cur, nmax, parity ? d.ns_b : d.ns_a, P, n, d.fit, d.nact,
+                                      1e-4, s));
+    CHECK_HIP(hipStreamSynchronize(s));
+    std::vector<double> fit(P);
+    CHECK_HIP(hipMemcpy(fit.data(), d.fit, P * sizeof(double), hipMemcpyDeviceToHost));
+    const int bi = (int)(std::min_element(fit.begin(), fit.end()) - fit.begin());
+    std::vector<float> row(n);
+    CHECK_HIP(hipMemcpy(row.data(), cur + (size_t)bi * nmax, n * sizeof(float),
+                        hipMemcpyDeviceToHost));
+    std::vector<double> a(row.begin(), row.end());
+    const double hf = host_F(a);
+    if (fit[bi] < bestF) { bestF = fit[bi]; bestA = a; }
+    std::printf("{\"level\": %d, \"generations\": %d, \"seconds\": %.1f, \"best_F\": %.15f, "
+                "\"host_recheck\": %.15f}\n", n, gens, seconds_since(t0), fit[bi], hf);
+    CHECK_SIDON(sidon_generation_destroy(g));
+    if (lvl + 1 < L.size()) {  // cell split every candidate into population A
+      if (L[lvl + 1] != 2 * n) { std::fprintf(stderr, "ladder must double\n"); return 2; }
+      CHECK_HIP(hipMemcpy2DAsync(tmp, n * sizeof(float), cur, nmax * sizeof(float),
+                                 n * sizeof(float), P, hipMemcpyDeviceToDevice, s));
+      float* packed = dalloc<float>((size_t)P * 2 * n);
+      CHECK_SIDON(sidon_refine(tmp, n, P, packed, s));
+      CHECK_HIP(hipMemcpy2DAsync(d.pop_a, nmax * sizeof(float), packed, 2 * n * sizeof(float),
+                                 2 * n * sizeof(float), P, hipMemcpyDeviceToDevice, s));
+      CHECK_HIP(hipStreamSynchronize(s));
+      CHECK_HIP(hipFree(packed));
+      unsigned long long zero = 0;
+      CHECK_HIP(hipMemcpy(d.gen, &zero, sizeof zero, hipMemcpyHostToDevice));
+    }
+  }
+  const double hf = host_F(bestA);
+  save_vec(A.out + ".txt", bestA, std::max(bestF, hf));
+  save_certificate(A.out + ".json", bestA, hf);
+  std::printf("{\"best_F\": %.15f, \"host_recheck\": %.15f, \"n\": %zu, \"bound\": \"C <= %.9f\", "
+              "\"files\": [\"%s.txt\", \"%s.json\"], \"seconds\": %.1f}\n",
+              bestF, hf, bestA.size(), std::max(bestF, hf), A.out.c_str(), A.out.c_str(),
+              seconds_since(t0));
+  return 0;
+}
+
+void usage() {
+  std::printf(
+      "sidon_search MODE [options]\n"
+      "  --selftest                 device known-answer tests\n"
+      "  --uneven                   island generations of uneven n as HIP graphs\n"
+      "      --P 512 --islands 8 --iters 200 --gens 3 --sigma 0.05 --lengths 600,...,8400\n"
+      "  --ladder                   Frank-Wolfe + n -> 2n split ladder\n"
+      "      --P 256 --levels 1050,2100,4200,8400 --fw-rounds 50 --cp-iters 2000\n"
+      "  --search                   ladder search with generation graphs\n"
+      "      --P 256 --levels 256,512,1024,2048 --islands 8 --iters 200 --gens 4\n"
+      "      --seconds 60 --out best   (writes best.txt and best.json)\n"
+      "  --seed S  --no-tune\n");
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  setvbuf(stdout, nullptr, _IONBF, 0);
+  Args A;
+  for (int i = 1; i < argc; ++i) {
+    const std::string s = argv[i];
+    auto next = [&]() -> const char* {
+      if (i + 1 >= argc) { usage(); std::exit(2); }
+      return argv[++i];
+    };
+    if (s == "--selftest" || s == "--uneven" || s == "--ladder" || s == "--search") A.mode = s;
+    else if (s == "--P") A.P = std::atoi(next());
+    else if (s == "--islands") A.islands = std::atoi(next());
+    else if (s == "--iters") A.iters = std::atoi(next());
+    else if (s == "--gens") A.gens = std::atoi(next());
+    else if (s == "--elite") A.elite = std::atoi(next());
+    else if (s == "--sigma") A.sigma = (float)std::atof(next());
+    else if (s == "--lengths") A.lengths = parse_list(next());
+    else if (s == "--levels") A.ladder = parse_list(next());
+    else if (s == "--fw-rounds") A.fw_rounds = std::atoi(next());
+    else if (s == "--cp-iters") A.cp_iters = std::atoi(next());
+    else if (s == "--seconds") A.seconds = std::atof(next());
+    else if (s == "--seed") A.seed = std::strtoull(next(), nullptr, 10);
+    else if (s == "--out") A.out = next();
+    else if (s == "--no-tune") A.tune = false;
+    else { usage(); return 2; }
+  }
+  CHECK_SIDON(sidon_init(-1));
+  if (A.mode == "--selftest") return selftest();
+  if (A.mode == "--uneven") return uneven(A);
+  if (A.mode == "--ladder") return ladder(A);
+  if (A.mode == "--search") return search(A);
+  usage();
+  return 2;
+}

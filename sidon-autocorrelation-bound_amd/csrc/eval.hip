@@ -118,22 +118,47 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e
   e = (a - (s - bb)) + (b - bb);
 }
 
-// Rounds hi + lo (|lo| small) and decides whether the error bound delta can
-// move the exact value across a rounding boundary of the result.
-__device__ __forceinline__ double round_certified(double hi, double lo, double delta, int& amb) {
-  double s, err;
-  two_sum(hi, lo, s, err);
+// Triple-length accumulator: sum = hi + lo + lo2, where hi collects the
+// terms, lo the exact TwoSum errors of hi and lo2 the exact TwoSum errors of
+// lo; only lo2 is rounded.  `exact` stays true while every error reaching lo2
+// is zero, i.e. while hi + lo is the exact sum.
+struct Acc3 {
+  double hi, lo, lo2;
+  bool exact;
+};
+__device__ __forceinline__ void acc3_add(Acc3& a, double p) {
+  double s, e, l, e2;
+  two_sum(a.hi, p, s, e);
+  a.hi = s;
+  two_sum(a.lo, e, l, e2);
+  a.lo = l;
+  if (e2 != 0.0) { a.lo2 += e2; a.exact = false; }
+}
+
+// Rounds the accumulated sum once.  Exact accumulation: fl(hi + lo) is the
+// correctly rounded sum (ties to even included), certified.  Otherwise the
+// rounding is certified unless the remainder lies within the error bound of
+// half an ulp (rel: bound on the error of lo2 relative to the sum).
+__device__ __forceinline__ double round_certified(const Acc3& a, double rel, int& amb) {
+  if (a.exact) {
+    amb = 0;
+    return a.hi + a.lo;
+  }
+  double s1, r1, s, r2;
+  two_sum(a.lo, a.lo2, s1, r1);
+  two_sum(a.hi, s1, s, r2);
+  const double err = r2 + r1;
+  const double delta = rel * fabs(s) + 0x1p-52 * fabs(err);
   const double up = nextafter(s, DBL_MAX) - s;
   const double dn = s - nextafter(s, 0.0);
   amb = (err + delta >= 0.5 * up) || (err - delta <= -0.5 * dn);
   return s;
 }
 
-// Exact tile: double-double sums of exact fp32 x fp32 products.
-__device__ __forceinline__ void exact_tile(const double* sA, int n, int k0, double hi[16],
-                                           double lo[16]) {
+// Exact tile: triple-length sums of exact fp32 x fp32 products.
+__device__ __forceinline__ void exact_tile(const double* sA, int n, int k0, Acc3 acc[16]) {
 #pragma unroll
-  for (int t = 0; t < 16; ++t) { hi[t] = 0.0; lo[t] = 0.0; }
+  for (int t = 0; t < 16; ++t) acc[t] = Acc3{0.0, 0.0, 0.0, true};
   const int ilo = k0 - (n - 1) > 0 ? k0 - (n - 1) : 0;
   const int ihi = k0 + 15 < n - 1 ? k0 + 15 : n - 1;
   for (int J = ilo >> 4; J <= (ihi >> 4); ++J) {
@@ -146,13 +171,7 @@ __device__ __forceinline__ void exact_tile(const double* sA, int n, int k0, doub
 #pragma unroll
     for (int s = 0; s < 16; ++s)
 #pragma unroll
-      for (int t = 0; t < 16; ++t) {
-        const double p = x[s] * w[t - s + 15];  // exact: 24-bit x 24-bit mantissas
-        double sh, e;
-        two_sum(hi[t], p, sh, e);
-        hi[t] = sh;
-        lo[t] += e;
-      }
+      for (int t = 0; t < 16; ++t) acc3_add(acc[t], x[s] * w[t - s + 15]);  // exact products
   }
 }
 #pragma clang fp contract(on)
@@ -174,20 +193,15 @@ __global__ __launch_bounds__(BD) void k_eval_exact(const float* __restrict__ pop
   __syncthreads();
   for (int i = tid; i < n; i += BD) sA[i] = (double)a[i];
   __syncthreads();
-  // bound on the double-double accumulation error for m non-negative terms:
-  // 2 m^2 u^2 |sum| (u = 2^-53), plus slack.
+  // bound on the error of lo2 (the only rounded part) relative to a sum of m
+  // non-negative terms: m^3 u^3 (u = 2^-53), with slack.
   const double m = (double)n;
-  const double rel = (2.0 * m * m + 8.0) * 0x1p-106;
+  const double rel = (2.0 * m * m * m + 8.0) * 0x1p-159;
   if (tid == 0) {
-    double hi = 0.0, lo = 0.0;
-    for (int i = 0; i < n; ++i) {
-      double sh, e;
-      two_sum(hi, sA[i], sh, e);
-      hi = sh;
-      lo += e;
-    }
+    Acc3 acc{0.0, 0.0, 0.0, true};
+    for (int i = 0; i < n; ++i) acc3_add(acc, sA[i]);
     int ab = 0;
-    const double S = round_certified(hi, lo, rel * hi, ab);
+    const double S = round_certified(acc, rel, ab);
     sS[0] = S;
     sS[1] = (double)ab;
   }
@@ -197,14 +211,14 @@ __global__ __launch_bounds__(BD) void k_eval_exact(const float* __restrict__ pop
     for (int side = 0; side < 2; ++side) {
       const int tq = side ? NT - 1 - q : q;
       if (side && tq == q) break;
-      double hi[16], lo[16];
-      exact_tile(sA, n, 16 * tq, hi, lo);
+      Acc3 acc[16];
+      exact_tile(sA, n, 16 * tq, acc);
       double mx = 0.0;
 #pragma unroll
       for (int t = 0; t < 16; ++t) {
         if (16 * tq + t >= nb) continue;
         int ab = 0;
-        const double r = round_certified(hi[t], lo[t], rel * hi[t], ab);
+        const double r = round_certified(acc[t], rel, ab);
         mx = r > mx ? r : mx;
       }
       tmax[tq] = mx;
@@ -216,13 +230,13 @@ __global__ __launch_bounds__(BD) void k_eval_exact(const float* __restrict__ pop
   double cnt = 0.0;
   for (int q = tid; q < NT; q += BD) {
     if (tmax[q] != M) continue;
-    double hi[16], lo[16];
-    exact_tile(sA, n, 16 * q, hi, lo);
+    Acc3 acc[16];
+    exact_tile(sA, n, 16 * q, acc);
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
       if (16 * q + t >= nb) continue;
       int ab = 0;
-      const double r = round_certified(hi[t], lo[t], rel * hi[t], ab);
+      const double r = round_certified(acc[t], rel, ab);
       if (r == M && ab) cnt += 1.0;
     }
   }

@@ -1,0 +1,188 @@
+// generation.hip -- host side: one memetic generation captured as a HIP graph,
+// and the local-search autotuner.
+//
+// A generation of the island search (reference: the body of the generation
+// loop, src/main.cu:586-785, reduced to the operators the configurations use)
+// is, on the device and with no host round trip:
+//   order candidates longest-first -> fused local search (persistent, uneven
+//   lengths) -> fp64 F -> island ranking / migration / elites / tournaments ->
+//   Gaussian multiplicative mutation into the other buffer -> gen += 1.
+// Two graphs are captured (population A -> B and B -> A) and replayed in turn,
+// so a run of G generations is G graph launches.  Every kernel in the
+// sequence reads its generation number from device memory and clears its
+// work-queue counter with a kernel, so replays need no host updates and no
+// memset nodes.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "runtime.hpp"
+
+using namespace sidon;
+
+struct sidon_generation {
+  sidon_generation_desc d;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  unsigned long long replays = 0;
+};
+
+namespace {
+
+int enqueue_generation(const sidon_generation_desc& d, int parity, hipStream_t s) {
+  float* src = parity ? d.pop_b : d.pop_a;
+  float* dst = parity ? d.pop_a : d.pop_b;
+  int* ns_src = parity ? d.ns_b : d.ns_a;
+  int* ns_dst = parity ? d.ns_a : d.ns_b;
+  int rc;
+  if ((rc = sidon_order_by_length(ns_src, d.P, d.order, s))) return rc;
+  if ((rc = sidon_local_search_ragged(src, d.stride, ns_src, d.P, d.nmax, d.order, nullptr,
+                                      nullptr, d.iters, d.pStart, d.pEnd, d.lrStart, d.lrEnd,
+                                      0.9f, 0.999f, 0, s)))
+    return rc;
+  if ((rc = sidon_eval_f64_ragged(src, d.stride, ns_src, d.P, d.nmax, d.fit, d.nact,
+                                  d.activeTol, s)))
+    return rc;
+  if ((rc = sidon_select(d.fit, src, ns_src, d.stride, d.P, d.islands, d.elite, d.migrate_every,
+                         d.sigma, d.seed, d.gen, d.p1, d.p2, d.op, d.strength, s)))
+    return rc;
+  if ((rc = sidon_variation_ragged(src, ns_src, dst, ns_dst, d.stride, d.P, d.p1, d.p2, d.op,
+                                   d.strength, d.seed, d.gen, s)))
+    return rc;
+  return sidon_advance_gen(d.gen, 1, s);
+}
+
+bool desc_ok(const sidon_generation_desc& d) {
+  return d.pop_a && d.pop_b && d.ns_a && d.ns_b && d.fit && d.nact && d.p1 && d.p2 && d.op &&
+         d.strength && d.order && d.gen && d.P > 0 && d.islands > 0 && d.P % d.islands == 0 &&
+         d.nmax > 0 && d.stride >= d.nmax && d.iters >= 0 && d.elite >= 0 &&
+         d.elite <= d.P / d.islands;
+}
+
+}  // namespace
+
+extern "C" int sidon_generation_create(const sidon_generation_desc* desc,
+                                       sidon_generation** out, hipStream_t stream) {
+  if (!desc || !out || !desc_ok(*desc)) return SIDON_ERR_ARG;
+  *out = nullptr;
+  int rc = SIDON_OK;
+  if (!device_state(&rc)) return rc;
+  sidon_generation* g = new (std::nothrow) sidon_generation();
+  if (!g) return SIDON_ERR_HIP;
+  g->d = *desc;
+  hipStream_t cap = stream;
+  bool own = false;
+  if (!cap) {  // the null stream cannot be captured: use a private one
+    if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) {
+      delete g;
+      return SIDON_ERR_HIP;
+    }
+    own = true;
+  }
+  for (int parity = 0; parity < 2 && rc == SIDON_OK; ++parity) {
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) { set_last_error("hipStreamBeginCapture", e); rc = SIDON_ERR_HIP; break; }
+    rc = enqueue_generation(g->d, parity, cap);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(cap, &graph);
+    if (rc == SIDON_OK && e != hipSuccess) { set_last_error("hipStreamEndCapture", e); rc = SIDON_ERR_HIP; }
+    if (rc != SIDON_OK) { if (graph) (void)hipGraphDestroy(graph); break; }
+    g->graph[parity] = graph;
+    e = hipGraphInstantiate(&g->exec[parity], graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) { set_last_error("hipGraphInstantiate", e); rc = SIDON_ERR_HIP; }
+  }
+  if (own) (void)hipStreamDestroy(cap);
+  if (rc != SIDON_OK) {
+    sidon_generation_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return SIDON_OK;
+}
+
+extern "C" int sidon_generation_run(sidon_generation* g, int generations, hipStream_t stream) {
+  if (!g || generations < 0) return SIDON_ERR_ARG;
+  for (int k = 0; k < generations; ++k) {
+    const int parity = (int)(g->replays & 1ULL);
+    SIDON_HIP_TRY(hipGraphLaunch(g->exec[parity], stream));
+    ++g->replays;
+  }
+  return SIDON_OK;
+}
+
+extern "C" int sidon_generation_current(const sidon_generation* g, int* parity) {
+  if (!g || !parity) return SIDON_ERR_ARG;
+  *parity = (int)(g->replays & 1ULL);  // 0: population in pop_a / ns_a, 1: pop_b / ns_b
+  return SIDON_OK;
+}
+
+extern "C" int sidon_generation_enqueue(const sidon_generation_desc* desc, int parity,
+                                        hipStream_t stream) {
+  if (!desc || !desc_ok(*desc) || (parity != 0 && parity != 1)) return SIDON_ERR_ARG;
+  return enqueue_generation(*desc, parity, stream);
+}
+
+extern "C" int sidon_generation_destroy(sidon_generation* g) {
+  if (!g) return SIDON_OK;
+  for (int p = 0; p < 2; ++p) {
+    if (g->exec[p]) (void)hipGraphExecDestroy(g->exec[p]);
+    if (g->graph[p]) (void)hipGraphDestroy(g->graph[p]);
+  }
+  delete g;
+  return SIDON_OK;
+}
+
+// ------------------------------------------------------------ autotuner --
+// Times the fused local search at length n for every workgroup size on the
+// caller's scratch population (P x n, modified in place; restored from
+// `reference` before each trial when given) and keeps the fastest as the rule
+// for n.  Block size sets the waves per candidate (4, 8 or 16).
+extern "C" int sidon_tune_local_search(float* scratch, const float* reference, int n, int P,
+                                       int iters, int repeats, int* best_block, float* best_ms,
+                                       hipStream_t stream) {
+  if (!scratch || n < 1 || P < 1 || iters < 1 || repeats < 1) return SIDON_ERR_ARG;
+  static const int blocks[3] = {256, 512, 1024};
+  hipEvent_t e0, e1;
+  SIDON_HIP_TRY(hipEventCreate(&e0));
+  SIDON_HIP_TRY(hipEventCreate(&e1));
+  int best = 0;
+  float bms = 0.0f;
+  int rc = SIDON_OK;
+  const size_t bytes = (size_t)P * (size_t)n * sizeof(float);
+  for (int b : blocks) {
+    if ((rc = sidon_set_ls_block(n, b))) break;
+    float tot = 0.0f;
+    bool ok = true;
+    for (int r = 0; r <= repeats && ok; ++r) {  // r = 0 warms up
+      if (reference &&
+          hipMemcpyAsync(scratch, reference, bytes, hipMemcpyDeviceToDevice, stream) != hipSuccess) {
+        rc = SIDON_ERR_HIP;
+        break;
+      }
+      (void)hipEventRecord(e0, stream);
+      const int lrc = sidon_local_search(scratch, nullptr, nullptr, nullptr, nullptr, n, P, iters,
+                                         4.0f, 1e4f, 0.085f, 0.0015f, 0.9f, 0.999f, 0, stream);
+      (void)hipEventRecord(e1, stream);
+      if (lrc == SIDON_ERR_LDS || lrc == SIDON_ERR_UNSUPPORTED) { ok = false; break; }
+      if (lrc) { rc = lrc; break; }
+      if (hipEventSynchronize(e1) != hipSuccess) { rc = SIDON_ERR_HIP; break; }
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (r > 0) tot += ms;
+    }
+    if (rc) break;
+    if (!ok) continue;
+    const float ms = tot / (float)repeats;
+    if (!best || ms < bms) { best = b; bms = ms; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) { (void)sidon_set_ls_block(n, 0); return rc; }
+  if (!best) { (void)sidon_set_ls_block(n, 0); return SIDON_ERR_LDS; }
+  (void)sidon_set_ls_block(n, best);
+  if (best_block) *best_block = best;
+  if (best_ms) *best_ms = bms;
+  return SIDON_OK;
+}

@@ -107,3 +107,44 @@ def test_generation_counter_and_graph_replay(S, W):
         _ = cap["pop"].sum().item()       # eager work between replays
     assert cap["gen"].item() == 6 == eager["gen"].item()
     assert torch.equal(cap["pop"], eager["pop"]) and torch.equal(cap["nxt"], eager["nxt"])
+
+
+def test_generation_graph_matches_eager(S, W):
+    """Generations replayed from the captured HIP graphs equal the same
+    sequence enqueued eagerly, bit for bit, with uneven lengths (the lengths
+    travel with the vectors)."""
+    print("[evolve] generation graph vs eager", flush=True)
+    P, NI, stride = 64, 4, 320
+    ns = np.array([64, 128, 200, 320] * 16, np.int32)
+    pop = np.zeros((P, stride), np.float32)
+    for p, n in enumerate(ns):
+        pop[p, :n] = W.structured_prior(1, int(n), seed=200 + p)[0]
+    kw = dict(islands=NI, iters=6, sigma=0.05, elite=2, migrate_every=3, seed=77)
+    g = S.Generation(dev(pop), dev(ns, torch.int32), graph=True, **kw)
+    e = S.Generation(dev(pop), dev(ns, torch.int32), graph=False, **kw)
+    for rnd in range(3):
+        g.run(3)
+        e.run(3)
+        torch.cuda.synchronize()
+        _ = float(g.pop_a.sum().item())        # eager work between replays
+        gp, gn = g.population()
+        ep, en = e.population()
+        assert g.gen.item() == e.gen.item() == 3 * (rnd + 1)
+        assert torch.equal(gn, en) and torch.equal(gp, ep)
+        assert torch.equal(g.fit, e.fit)
+    gp, gn = g.population()
+    lens = gn.cpu().numpy()
+    assert set(lens.tolist()) <= {64, 128, 200, 320}
+    arr = gp.cpu().numpy()
+    for p in range(P):
+        row = arr[p, :lens[p]].astype(np.float64)
+        assert (row >= 0).all() and abs(row.sum() - 1) < 1e-5
+    g.close()
+
+
+def test_autotuner_picks_a_block(S):
+    print("[evolve] autotuner", flush=True)
+    blk, ms = S.tune_local_search(600, P=16, iters=5, repeats=1)
+    assert blk in (256, 512, 1024) and ms > 0
+    assert S.get_ls_block(600) == blk
+    S.set_ls_block(600, 0)
