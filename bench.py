@@ -10,11 +10,10 @@ candidate-steps/s.  Inputs are resident in HBM before the timed region.
 
 Beside it: the plain form (one candidate per launch, same kernel, a bounded
 sample of candidates run back to back on the stream), the roofline of the
-local-search kernel (4 n^2 FLOP per candidate-step, src/main.cu:313, :625,
-against the 157.3 TFLOP/s dense FP32 matrix peak), and the CPU baseline: the
-reference's own fp64 CPU twin of the kernel (tools/probe.cpp --opt, built from
-source into oracle/_ref/) timed on this host, or the oracle port when that
-binary is absent.  Prints one JSON line.
+local-search kernel against the 157.3 TFLOP/s dense FP32 matrix peak, and the
+CPU baseline: the reference's own fp64 CPU twin of the kernel (tools/probe.cpp
+--opt, built from source into oracle/_ref/) timed on this host, or the oracle
+port when that binary is absent.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -180,11 +179,17 @@ def main() -> int:
         plain_value = k * iters / (ev0.elapsed_time(ev1) / 1e3)
         same = bool(torch.equal(plain, pop[:k]))
 
+    valu_ceiling, mfma_ceiling = S.measure_fp32_ceilings()
+
     F, _ = S.eval_f64(pop)
     torch.cuda.synchronize()
     F0, _ = S.eval_f64(start)
-    flop = 4.0 * n * n * iters * P
+    # FLOP a candidate-step needs: the autoconvolution counts each product
+    # a_i a_j (i < j) once (n^2 / 2 multiply-adds), the correlation n^2.  The
+    # reference counts the full product, 4 n^2 (src/main.cu:313, :625).
+    flop = 3.0 * n * n * iters * P
     achieved = flop / (ms / 1e3) / 1e12
+    ref_accounting = 4.0 * n * n * iters * P / (ms / 1e3) / 1e12
 
     out = {
         "metric": METRIC,
@@ -206,11 +211,17 @@ def main() -> int:
                    "block": S.get_ls_block(n)},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
-                     "traffic": traffic_from_profiles(n, P, iters)},
+                     "traffic": traffic_from_profiles(n, P, iters),
+                     "flop_per_candidate_step": 3.0 * n * n,
+                     "reference_accounting_tflops": ref_accounting},
         "plain": {"value": plain_value, "unit": "candidate-steps/s", "sample": k,
                   "form": "one candidate per launch (grid = 1 workgroup), launched back to "
                           "back on one stream", "bitwise_equal_to_batched": same},
         "batched_over_plain": value / plain_value if plain_value else None,
+        "engine_ceilings": {"valu_fp32_tflops": valu_ceiling, "mfma_fp32_tflops": mfma_ceiling,
+                            "note": "register-only FMA kernels on every CU, this build's flags "
+                                    "(packed FP32 off); the local search runs on the matrix "
+                                    "cores"},
         "F_mean_before": float(F0.mean().item()),
         "F_mean_after": float(F.mean().item()),
     }

@@ -216,6 +216,17 @@ int sidon_generation_current(const sidon_generation* g, int* parity);
 int sidon_generation_enqueue(const sidon_generation_desc* desc, int parity, hipStream_t stream);
 int sidon_generation_destroy(sidon_generation* g);
 
+/* ---- engine ceilings ------------------------------------------------------ */
+
+/* Measures, on the current device and under this library's compile flags, the
+ * FP32 FMA rate of the vector ALUs (packed FP32 off) and of the matrix cores
+ * (v_mfma_f32_16x16x4f32), in TFLOP/s, with register-only kernels on every CU.
+ * `scratch` is a device buffer of at least (CUs * 8 * 256) floats.  The local
+ * search runs its products on the matrix cores because the measured matrix
+ * ceiling is above the vector one.  No reference counterpart. */
+int sidon_measure_fp32_ceilings(float* scratch, int scratch_len, double* valu_tflops,
+                                double* mfma_tflops, hipStream_t stream);
+
 /* ---- autotuner ---------------------------------------------------------- */
 
 /* Times the fused local search at length n for each workgroup size (4, 8 or

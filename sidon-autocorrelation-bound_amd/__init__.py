@@ -36,6 +36,7 @@ EXPORTS = (
     "sidon_variation_ragged", "sidon_select", "sidon_advance_gen", "sidon_order_by_length",
     "sidon_generation_create", "sidon_generation_run", "sidon_generation_current",
     "sidon_generation_enqueue", "sidon_generation_destroy", "sidon_tune_local_search",
+    "sidon_measure_fp32_ceilings",
 )
 
 
@@ -109,6 +110,7 @@ def _declare(L):
         "sidon_generation_destroy": ([_vp], _i),
         "sidon_tune_local_search": ([_vp, _vp, _i, _i, _i, _i, C.POINTER(_i), C.POINTER(_f), _vp],
                                     _i),
+        "sidon_measure_fp32_ceilings": ([_vp, _i, C.POINTER(_d), C.POINTER(_d), _vp], _i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -320,6 +322,18 @@ def advance_gen(gen_dev, delta: int = 1, stream=None) -> None:
 def order_by_length(ns: torch.Tensor, order: torch.Tensor, stream=None) -> None:
     _check(lib().sidon_order_by_length(_ptr(ns), ns.shape[0], _ptr(order), _stream(stream)),
            "sidon_order_by_length")
+
+
+def measure_fp32_ceilings(stream=None):
+    """(VALU, MFMA) FP32 TFLOP/s measured with register-only kernels on every CU."""
+    _need_gpu()
+    cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
+    scratch = torch.empty(cus * 8 * 256, dtype=torch.float32, device="cuda")
+    v, m = C.c_double(), C.c_double()
+    _check(lib().sidon_measure_fp32_ceilings(_ptr(scratch), scratch.numel(), C.byref(v),
+                                             C.byref(m), _stream(stream)),
+           "sidon_measure_fp32_ceilings")
+    return v.value, m.value
 
 
 def tune_local_search(n: int, P: int = 64, iters: int = 20, repeats: int = 2, seed: int = 1,

@@ -16,11 +16,13 @@
 //                segments at once (one MFMA tile = 16 output segments);
 //      corr:  G[16I + t] = sum_e sum_s c[16e + t + s] * a[16(e-I) + s]
 //             -> Hankel block C_e[t][s] = c[16e + t + s] times 16 segments;
-//  * the vector `a` lives in LDS in a padded segment layout (20 elements per
-//    16-element segment, 16 zero segments on each side) so that the MFMA
-//    B-operand reads (one 16-byte read per lane, stride 16 heights across
-//    lanes) hit disjoint banks and every out-of-range index reads an exact
-//    zero;
+//  * the vector `a` lives in LDS in an overlapped segment layout: 20
+//    elements per 16-element segment (the last four heights of the previous
+//    segment, then its own sixteen), 16 zero segments on each side.  The
+//    MFMA B-operand reads (one 16-byte read per lane, stride 16 heights
+//    across lanes) hit disjoint banks, the four Toeplitz A-operand heights
+//    of a lane are always four adjacent elements, and every out-of-range
+//    index reads an exact zero;
 //  * the work of each product is cut into W equal contiguous ranges of
 //    (tile, block-offset) units, one per wave ("stream-K").  A tile is never
 //    split among more than two waves, and two partial sums added onto an
@@ -108,11 +110,22 @@ __device__ __forceinline__ T block_max(T v, T* slots) {
 }
 
 // ------------------------------------------------------------ LDS layout --
-// Padded segment layout of a length-n vector: element x (x may be negative
-// or >= n) lives at pad_addr(x); valid for x in [-256, 16*(N16+16)).
+// Overlapped segment layout of a length-n vector: segment g (heights
+// 16(g-16) .. 16(g-16)+15) occupies elements [20g, 20g+20): first copies of
+// the previous segment's last four heights, then its own sixteen.  Element
+// x (x may be negative or >= n) lives at pad_addr(x), so x-3 .. x are the
+// four elements pad_addr(x)-3 .. pad_addr(x); valid for x in
+// [-256, 16*(N16+16)).  Writes go through pad_put, which also refreshes the
+// copy held by the next segment.
 __host__ __device__ __forceinline__ int pad_addr(int x) {
   const int y = x + kPadSeg * kSeg;
-  return (y >> 4) * kSegStride + (y & 15);
+  return (y >> 4) * kSegStride + 4 + (y & 15);
+}
+template <typename T>
+__device__ __forceinline__ void pad_put(T* s, int x, T v) {
+  const int p = pad_addr(x);
+  s[p] = v;
+  if (((x + kPadSeg * kSeg) & 15) >= 12) s[p + 4] = v;  // copy at the head of the next segment
 }
 __host__ __device__ __forceinline__ int nseg16(int n) { return (n + kSeg - 1) / kSeg; }
 // floats (elements) of the padded vector buffer
@@ -221,14 +234,12 @@ __device__ __forceinline__ typename Acc4<T>::type conv_tile(const T* __restrict_
   const int lane = threadIdx.x & (kWave - 1);
   const int t = lane & 15, kk = lane >> 4;
   A4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
-  int pa[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) pa[q] = pad_addr(16 * da + t - 4 * kk - q);
+  int pa = pad_addr(16 * da + t - 4 * kk);  // A operand q at pa - q
   int pb = pad_addr(16 * (K0 + t - da)) + 4 * kk;
   // operands of the pair of steps starting `off` steps after the current one
   auto load_pair = [&](int off, T (&x0)[4], T (&y0)[4], T (&x1)[4], T (&y1)[4]) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) { x0[q] = sA[pa[q] + off * S]; x1[q] = sA[pa[q] + (off + 1) * S]; }
+    for (int q = 0; q < 4; ++q) { x0[q] = sA[pa + off * S - q]; x1[q] = sA[pa + (off + 1) * S - q]; }
     load4(sB + pb - off * S, y0);
     load4(sB + pb - (off + 1) * S, y1);
   };
@@ -246,8 +257,7 @@ __device__ __forceinline__ typename Acc4<T>::type conv_tile(const T* __restrict_
       __builtin_amdgcn_sched_barrier(0);
       mfma4x2(a2, b2, acc0, a3, b3, acc1);
       __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) pa[q] += 4 * S;
+      pa += 4 * S;
       pb -= 4 * S;
     }
     if (k + 1 < pairs) {
@@ -255,19 +265,17 @@ __device__ __forceinline__ typename Acc4<T>::type conv_tile(const T* __restrict_
       __builtin_amdgcn_sched_barrier(0);
       mfma4x2(a0, b0, acc0, a1, b1, acc1);
       mfma4x2(a2, b2, acc0, a3, b3, acc1);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) pa[q] += 4 * S;
+      pa += 4 * S;
       pb -= 4 * S;
     } else {
       mfma4x2(a0, b0, acc0, a1, b1, acc1);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) pa[q] += 2 * S;
+      pa += 2 * S;
       pb -= 2 * S;
     }
   }
   if (cnt & 1) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) a0[q] = sA[pa[q]];
+    for (int q = 0; q < 4; ++q) a0[q] = sA[pa - q];
     load4(sB + pb, b0);
     acc0 = mfma4(a0, b0, acc0);
   }
@@ -383,6 +391,118 @@ __device__ void conv_mfma(const T* __restrict__ sA, const T* __restrict__ sB, T*
     base += cost;
     ++j;
   }
+}
+
+// ------------------------------------- symmetric MFMA autoconvolution -----
+// b_k = sum_i a_i a_{k-i} counts every product a_i a_j (i != j) twice, so
+//   b_k = 2 * sum_{i < k-i} a_i a_{k-i} + [k even] a_{k/2}^2.
+// In the blocked form, block d of output segment K holds the products with
+// i in [16d + row - 15, 16d + row], j = k - i in [16(K-d), 16(K-d) + 15]; all
+// of them have i < j while d < dp(K) = ceil(K/2), none once d > dp(K).  The
+// matrix cores therefore run only d < dp(K) (half the blocks): tile j needs
+// d <= 8j + 7, and inside the last eight offsets (d >= 8j, the "zone")
+// column t = K - 16j drops out once d >= dp(K).  What is left of each output,
+// the products with 16(dp-1) + row < i < k/2 (at most 15) and the square
+// term, is added by autoconv_finish.
+__device__ __forceinline__ void autoconv_drange(int j, int N16, int& lo, int& hi) {
+  conv_drange(j, N16, lo, hi);
+  const int z = 8 * j + 7;
+  hi = hi < z ? hi : z;
+}
+__device__ __forceinline__ int autoconv_cost(int j, int N16) {
+  int lo, hi;
+  autoconv_drange(j, N16, lo, hi);
+  return hi >= lo ? hi - lo + 1 : 0;
+}
+
+// Zone steps d in [da, db) of tile K0 = 16j (8j <= da): column t keeps its B
+// operand only while d < dp(K0 + t) = 8j + ceil(t/2).
+template <typename T, typename A4>
+__device__ __forceinline__ A4 autoconv_zone(const T* __restrict__ sA, int K0, int da, int db,
+                                            A4 acc) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int t = lane & 15, kk = lane >> 4;
+  const int lim = (K0 >> 1) + ((t + 1) >> 1);  // dp of this lane's column
+  for (int d = da; d < db; ++d) {
+    T x[4], y[4];
+    const int pa = pad_addr(16 * d + t - 4 * kk);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) x[q] = sA[pa - q];
+    load4(sA + pad_addr(16 * (K0 + t - d)) + 4 * kk, y);
+    if (d >= lim) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) y[q] = T(0);
+    }
+    acc = mfma4(x, y, acc);
+  }
+  return acc;
+}
+
+// out[k] += sum over the matrix-core share of the symmetric autoconvolution
+// (see above) for all k < 16 * 16 * conv_tiles(n); sA padded, `out` exactly
+// zero before the call.  Same split and determinism as conv_mfma.
+template <typename T>
+__device__ void autoconv_mfma(const T* __restrict__ sA, T* out, int n, int wave, int nwaves) {
+  const int N16 = nseg16(n);
+  const int NT = conv_tiles(n);
+  int U = 0, maxc = 0;
+  for (int j = 0; j < NT; ++j) {
+    const int c = autoconv_cost(j, N16);
+    U += c;
+    maxc = c > maxc ? c : maxc;
+  }
+  if (U == 0) return;
+  Share sh = make_share(U, maxc, wave, nwaves);
+  if (!sh.active) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int t = lane & 15;
+  int j = 0, base = 0;
+  while (base + autoconv_cost(j, N16) <= sh.u0) { base += autoconv_cost(j, N16); ++j; }
+  while (base < sh.u1 && j < NT) {
+    int dlo, dhi;
+    autoconv_drange(j, N16, dlo, dhi);
+    const int cost = dhi >= dlo ? dhi - dlo + 1 : 0;
+    if (cost > 0) {
+      const int ua = sh.u0 > base ? sh.u0 : base;
+      const int ub = sh.u1 < base + cost ? sh.u1 : base + cost;
+      const int K0 = 16 * j, z = 8 * j;
+      const int da = dlo + (ua - base), db = dlo + (ub - base);
+      const int dm = db < z ? db : z;
+      typename Acc4<T>::type acc = {0, 0, 0, 0};
+      if (da < dm) acc = conv_tile<T>(sA, sA, K0, da, dm);
+      const int dz = da > z ? da : z;
+      if (dz < db) acc = autoconv_zone<T>(sA, K0, dz, db, acc);
+      put4(out, K0 + t, acc, (ua == base) && (ub == base + cost));
+    }
+    base += cost;
+    ++j;
+  }
+}
+
+// b_k from the matrix-core share h = out[k] of autoconv_mfma: adds the
+// products 16(dp-1) + row < i < k/2 (dp = ceil(K/2), K = k/16, row = k%16;
+// at most 15 of them) in ascending i, doubles, and adds a_{k/2}^2 for even k.
+// The loop runs its full 15 trips with the surplus terms masked, so the reads
+// of all trips are issued together; every read stays inside the padded
+// vector (i >= -15, k - i <= k/2 + 15).
+template <typename T>
+__device__ __forceinline__ T autoconv_finish(const T* __restrict__ sA, T h, int k) {
+  const int K = k >> 4, row = k & 15;
+  const int i0 = 16 * (((K + 1) >> 1) - 1) + row + 1;
+  const int cnt = ((k + 1) >> 1) - i0;
+  T r = h;
+#pragma unroll
+  for (int u = 0; u < 15; ++u) {
+    const T x = sA[pad_addr(i0 + u)];
+    const T y = sA[pad_addr(k - i0 - u)];
+    r += u < cnt ? x * y : T(0);
+  }
+  T b = r + r;
+  if (!(k & 1)) {
+    const T x = sA[pad_addr(k >> 1)];
+    b += x * x;
+  }
+  return b;
 }
 
 // ---------------------------------------------- MFMA correlation (split) ---

@@ -75,12 +75,12 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
   }
   __syncthreads();
   float sl = 0.0f;
-  for (int i = tid; i < n; i += BD) { const float x = a[i]; sA[pad_addr(i)] = x; sl += x; }
+  for (int i = tid; i < n; i += BD) { const float x = a[i]; pad_put(sA, i, x); sl += x; }
   const float s = block_sum<float, NW>(sl, slots);
   if (!(s > 0.0f)) return;
   {
     const float inv = 1.0f / s;
-    for (int i = tid; i < n; i += BD) sA[pad_addr(i)] *= inv;
+    for (int i = tid; i < n; i += BD) pad_put(sA, i, sA[pad_addr(i)] * inv);
   }
   __syncthreads();
 
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
         g[u][q] = (j < NTg && i < n) ? sA[pad_addr(i)] : 0.0f;
       }
     }
-    for (int i = tid; i < n; i += BD) sH[pad_addr(i)] = sA[pad_addr(i)];
+    for (int i = tid; i < n; i += BD) pad_put(sH, i, sA[pad_addr(i)]);
     for (int k = tid; k < LC; k += BD) sL[k] = 0.0f;
     __syncthreads();
 
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
           if (i < n) {
             float gn = g[u][q] + tau * (1.0f - c[q]);
             gn = gn > 0.0f ? gn : 0.0f;
-            sH[pad_addr(i)] = 2.0f * gn - g[u][q];
+            pad_put(sH, i, 2.0f * gn - g[u][q]);
             g[u][q] = gn;
           }
         }
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int i = 16 * (16 * j + t) + 4 * kk + q;
-          if (i < n) sH[pad_addr(i)] = (1.0f - tt) * sA[pad_addr(i)] + tt * g[u][q];
+          if (i < n) pad_put(sH, i, (1.0f - tt) * sA[pad_addr(i)] + tt * g[u][q]);
         }
       }
       for (int k = tid; k < LC; k += BD) sL[k] = 0.0f;
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int i = 16 * (16 * j + t) + 4 * kk + q;
-        if (i < n) sA[pad_addr(i)] = (1.0f - bestT) * sA[pad_addr(i)] + bestT * g[u][q];
+        if (i < n) pad_put(sA, i, (1.0f - bestT) * sA[pad_addr(i)] + bestT * g[u][q]);
       }
     }
     for (int k = tid; k < LC; k += BD) sL[k] = 0.0f;

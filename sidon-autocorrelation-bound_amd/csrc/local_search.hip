@@ -62,7 +62,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
   for (int i = tid; i < LC; i += BD) sC[i] = T(0);
   for (int i = tid; i < LG; i += BD) sG[i] = T(0);
   __syncthreads();
-  for (int i = tid; i < n; i += BD) sA[pad_addr(i)] = a[i];
+  for (int i = tid; i < n; i += BD) pad_put(sA, i, a[i]);
 
   T m[QMAX], v[QMAX];
 #pragma unroll
@@ -81,11 +81,15 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     const T pexp = A.pStart * dexp(lnPr * frac);
     const T lr = A.lrStart * dexp(lnLr * frac);
 
-    conv_mfma<T>(sA, sA, sC, n, wave, NW);
+    autoconv_mfma<T>(sA, sC, n, wave, NW);
     __syncthreads();
 
     T loc = T(0);
-    for (int k = tid; k < nb; k += BD) loc = sC[k] > loc ? sC[k] : loc;
+    for (int k = tid; k < nb; k += BD) {
+      const T b = autoconv_finish(sA, sC[k], k);
+      sC[k] = b;
+      loc = b > loc ? b : loc;
+    }
     const T M = block_max<T, NW>(loc, slots);
     if (!(M > T(0))) break;
 
@@ -139,7 +143,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) {
       const int i = tid + BD * q;
-      if (i < n) sA[pad_addr(i)] = reset ? T(1) / nT : sG[i] * inv;
+      if (i < n) pad_put(sA, i, reset ? T(1) / nT : sG[i] * inv);
     }
     for (int i = tid; i < LC; i += BD) sC[i] = T(0);
     for (int i = tid; i < LG; i += BD) sG[i] = T(0);
@@ -207,13 +211,17 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   for (int i = tid; i < LG; i += BD) sG[i] = T(0);
   __syncthreads();
   T sl = T(0);
-  for (int i = tid; i < n; i += BD) { const T x = a[i]; sA[pad_addr(i)] = x; sl += x; }
+  for (int i = tid; i < n; i += BD) { const T x = a[i]; pad_put(sA, i, x); sl += x; }
   const T S = block_sum<T, NW>(sl, slots);  // barrier also publishes sA
   const int nb = 2 * n - 1;
-  conv_mfma<T>(sA, sA, sC, n, wave, NW);
+  autoconv_mfma<T>(sA, sC, n, wave, NW);
   __syncthreads();
   T loc = T(0);
-  for (int k = tid; k < nb; k += BD) loc = sC[k] > loc ? sC[k] : loc;
+  for (int k = tid; k < nb; k += BD) {
+    const T b = autoconv_finish(sA, sC[k], k);
+    sC[k] = b;
+    loc = b > loc ? b : loc;
+  }
   const T M = block_max<T, NW>(loc, slots + NW);
   const T pm1 = A.p - T(1);
   T rloc = T(0);
