@@ -185,9 +185,8 @@ __device__ __forceinline__ Share make_share(int U, int maxcost, int wave, int nw
 // s = 4*kk + q (kk = lane/16 is the operand's k row, q = 0..3 the MFMA), so
 // the four B-operand values a lane feeds are four consecutive heights of one
 // segment: a single 16-byte LDS read (segments are 20 elements apart, so the
-// 16 columns of a row hit disjoint banks).  Operand reads for step d+1 are
-// issued before the MFMAs of step d, and even / odd steps accumulate into
-// separate registers so consecutive MFMAs do not wait on each other.
+// 16 columns of a row hit disjoint banks).  Even / odd steps accumulate into
+// separate registers.
 
 __device__ __forceinline__ void load4(const float* p, float (&o)[4]) {
   const float4 v = *reinterpret_cast<const float4*>(p);
@@ -199,17 +198,6 @@ __device__ __forceinline__ void load4(const double* p, double (&o)[4]) {
   o[0] = v0.x; o[1] = v0.y; o[2] = v1.x; o[3] = v1.y;
 }
 
-// Two independent accumulations issued MFMA by MFMA in turn: a wave running
-// alone on its SIMD (the tail of a phase) still has two MFMAs in flight.
-template <typename A4, typename T>
-__device__ __forceinline__ void mfma4x2(const T (&a0)[4], const T (&b0)[4], A4& acc0,
-                                        const T (&a1)[4], const T (&b1)[4], A4& acc1) {
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    acc0 = mfma16x16x4(a0[q], b0[q], acc0);
-    acc1 = mfma16x16x4(a1[q], b1[q], acc1);
-  }
-}
 template <typename A4, typename T>
 __device__ __forceinline__ A4 mfma4(const T (&a)[4], const T (&b)[4], A4 acc) {
 #pragma unroll
@@ -217,69 +205,90 @@ __device__ __forceinline__ A4 mfma4(const T (&a)[4], const T (&b)[4], A4 acc) {
   return acc;
 }
 
+// Operand steps kept in flight by the tile loops (registers: 8 values/step).
+template <typename T> struct Pipe { static constexpr int kDepth = 4; };
+template <> struct Pipe<double> { static constexpr int kDepth = 2; };
+
+// Runs `cnt` block steps of one output tile: step i multiplies the operands
+// that ld(i, a, b) reads (four A and four B values per lane, one MFMA each);
+// even steps accumulate into one register set, odd steps into another.  The
+// operands of D steps are in flight at once: slot s is refilled for step
+// i + D right after the MFMAs of step i have issued (the scheduling barriers
+// keep the reads there), so each read has D - 1 steps of MFMA time to land.
+// Nothing is read for steps outside [0, cnt).
+template <typename T, int D, typename Ld>
+__device__ __forceinline__ typename Acc4<T>::type run_steps(int cnt, const Ld& ld) {
+  static_assert(D % 2 == 0, "even depth keeps the accumulator parity static");
+  using A4 = typename Acc4<T>::type;
+  A4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+  T x[D][4], y[D][4];
+  if (cnt < 2 * D) {  // short range (shares' edges, small n)
+    for (int i = 0; i < cnt; ++i) {
+      ld(i, x[0], y[0]);
+      if (i & 1) acc1 = mfma4(x[0], y[0], acc1);
+      else acc0 = mfma4(x[0], y[0], acc0);
+    }
+    return acc0 + acc1;
+  }
+#pragma unroll
+  for (int s = 0; s < D; ++s) {
+    ld(s, x[s], y[s]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  int i = 0;
+  for (; i + 2 * D <= cnt; i += D) {
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+      if (s & 1) acc1 = mfma4(x[s], y[s], acc1);
+      else acc0 = mfma4(x[s], y[s], acc0);
+      __builtin_amdgcn_sched_barrier(0);
+      ld(i + s + D, x[s], y[s]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  // D <= cnt - i < 2D steps remain; refill only what lies in range
+#pragma unroll
+  for (int s = 0; s < D; ++s) {
+    if (s & 1) acc1 = mfma4(x[s], y[s], acc1);
+    else acc0 = mfma4(x[s], y[s], acc0);
+    if (i + s + D < cnt) {
+      __builtin_amdgcn_sched_barrier(0);
+      ld(i + s + D, x[s], y[s]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  i += D;
+#pragma unroll
+  for (int s = 0; s < D; ++s) {
+    if (i + s < cnt) {
+      if (s & 1) acc1 = mfma4(x[s], y[s], acc1);
+      else acc0 = mfma4(x[s], y[s], acc0);
+    }
+  }
+  return acc0 + acc1;
+}
+
 // Toeplitz tile: the 16x16 output block D[row][t] = (sA (*) sB)[16(K0+t) + row]
 // accumulated over block offsets d in [da, db) (rows per lane: Acc4<T>).
 //   A operand (lane t, kk; MFMA q): H_d[t][s] = sA[16d + t - s]
+//                                   (four adjacent elements, pad_addr(x) - q)
 //   B operand:                      sB[16(K0 + t - d) + s]      (column = t)
-// Steps run in pairs (d -> acc0, d+1 -> acc1) with two operand sets used in
-// turn: the next pair's operands are read while the current pair multiplies.
-// Nothing is read outside [da, db).  LDS addresses are kept as integer
-// element indices so the reads stay ds_read.
+// LDS addresses are kept as integer element indices so the reads stay
+// ds_read.
 template <typename T>
 __device__ __forceinline__ typename Acc4<T>::type conv_tile(const T* __restrict__ sA,
                                                            const T* __restrict__ sB, int K0,
                                                            int da, int db) {
-  using A4 = typename Acc4<T>::type;
   constexpr int S = kSegStride;
   const int lane = threadIdx.x & (kWave - 1);
   const int t = lane & 15, kk = lane >> 4;
-  A4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
-  int pa = pad_addr(16 * da + t - 4 * kk);  // A operand q at pa - q
-  int pb = pad_addr(16 * (K0 + t - da)) + 4 * kk;
-  // operands of the pair of steps starting `off` steps after the current one
-  auto load_pair = [&](int off, T (&x0)[4], T (&y0)[4], T (&x1)[4], T (&y1)[4]) {
+  const int pa = pad_addr(16 * da + t - 4 * kk);
+  const int pb = pad_addr(16 * (K0 + t - da)) + 4 * kk;
+  return run_steps<T, Pipe<T>::kDepth>(db - da, [&](int i, T (&x)[4], T (&y)[4]) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) { x0[q] = sA[pa + off * S - q]; x1[q] = sA[pa + (off + 1) * S - q]; }
-    load4(sB + pb - off * S, y0);
-    load4(sB + pb - (off + 1) * S, y1);
-  };
-  const int cnt = db - da, pairs = cnt >> 1;
-  T a0[4], b0[4], a1[4], b1[4], a2[4], b2[4], a3[4], b3[4];
-  if (pairs > 0) {
-    load_pair(0, a0, b0, a1, b1);
-    int k = 0;
-    for (; k + 2 < pairs; k += 2) {
-      load_pair(2, a2, b2, a3, b3);
-      __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the MFMAs
-      mfma4x2(a0, b0, acc0, a1, b1, acc1);
-      __builtin_amdgcn_sched_barrier(0);
-      load_pair(4, a0, b0, a1, b1);
-      __builtin_amdgcn_sched_barrier(0);
-      mfma4x2(a2, b2, acc0, a3, b3, acc1);
-      __builtin_amdgcn_sched_barrier(0);
-      pa += 4 * S;
-      pb -= 4 * S;
-    }
-    if (k + 1 < pairs) {
-      load_pair(2, a2, b2, a3, b3);
-      __builtin_amdgcn_sched_barrier(0);
-      mfma4x2(a0, b0, acc0, a1, b1, acc1);
-      mfma4x2(a2, b2, acc0, a3, b3, acc1);
-      pa += 4 * S;
-      pb -= 4 * S;
-    } else {
-      mfma4x2(a0, b0, acc0, a1, b1, acc1);
-      pa += 2 * S;
-      pb -= 2 * S;
-    }
-  }
-  if (cnt & 1) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) a0[q] = sA[pa - q];
-    load4(sB + pb, b0);
-    acc0 = mfma4(a0, b0, acc0);
-  }
-  return acc0 + acc1;
+    for (int q = 0; q < 4; ++q) x[q] = sA[pa + i * S - q];
+    load4(sB + pb - i * S, y);
+  });
 }
 
 // Hankel tile: D[row][t] = sum over e in [ea, eb) of
@@ -290,56 +299,16 @@ template <typename T>
 __device__ __forceinline__ typename Acc4<T>::type corr_tile(const T* __restrict__ c,
                                                            const T* __restrict__ sA, int I0,
                                                            int ea, int eb) {
-  using A4 = typename Acc4<T>::type;
   constexpr int S = kSegStride;
   const int lane = threadIdx.x & (kWave - 1);
   const int t = lane & 15, kk = lane >> 4;
-  A4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
-  int pc = 16 * ea + t + 4 * kk;
-  int pb = pad_addr(16 * (ea - I0 - t)) + 4 * kk;
-  auto load_pair = [&](int off, T (&x0)[4], T (&y0)[4], T (&x1)[4], T (&y1)[4]) {
+  const int pc = 16 * ea + t + 4 * kk;
+  const int pb = pad_addr(16 * (ea - I0 - t)) + 4 * kk;
+  return run_steps<T, Pipe<T>::kDepth>(eb - ea, [&](int i, T (&x)[4], T (&y)[4]) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) { x0[q] = c[pc + 16 * off + q]; x1[q] = c[pc + 16 * (off + 1) + q]; }
-    load4(sA + pb + off * S, y0);
-    load4(sA + pb + (off + 1) * S, y1);
-  };
-  const int cnt = eb - ea, pairs = cnt >> 1;
-  T a0[4], b0[4], a1[4], b1[4], a2[4], b2[4], a3[4], b3[4];
-  if (pairs > 0) {
-    load_pair(0, a0, b0, a1, b1);
-    int k = 0;
-    for (; k + 2 < pairs; k += 2) {
-      load_pair(2, a2, b2, a3, b3);
-      __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the MFMAs
-      mfma4x2(a0, b0, acc0, a1, b1, acc1);
-      __builtin_amdgcn_sched_barrier(0);
-      load_pair(4, a0, b0, a1, b1);
-      __builtin_amdgcn_sched_barrier(0);
-      mfma4x2(a2, b2, acc0, a3, b3, acc1);
-      __builtin_amdgcn_sched_barrier(0);
-      pc += 64;
-      pb += 4 * S;
-    }
-    if (k + 1 < pairs) {
-      load_pair(2, a2, b2, a3, b3);
-      __builtin_amdgcn_sched_barrier(0);
-      mfma4x2(a0, b0, acc0, a1, b1, acc1);
-      mfma4x2(a2, b2, acc0, a3, b3, acc1);
-      pc += 64;
-      pb += 4 * S;
-    } else {
-      mfma4x2(a0, b0, acc0, a1, b1, acc1);
-      pc += 32;
-      pb += 2 * S;
-    }
-  }
-  if (cnt & 1) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) a0[q] = c[pc + q];
-    load4(sA + pb, b0);
-    acc0 = mfma4(a0, b0, acc0);
-  }
-  return acc0 + acc1;
+    for (int q = 0; q < 4; ++q) x[q] = c[pc + 16 * i + q];
+    load4(sA + pb + i * S, y);
+  });
 }
 
 // Stores (whole tile) or atomically adds (half of a shared tile) a lane's 4
@@ -415,67 +384,82 @@ __device__ __forceinline__ int autoconv_cost(int j, int N16) {
   return hi >= lo ? hi - lo + 1 : 0;
 }
 
-// Zone steps d in [da, db) of tile K0 = 16j (8j <= da): column t keeps its B
-// operand only while d < dp(K0 + t) = 8j + ceil(t/2).
-template <typename T, typename A4>
-__device__ __forceinline__ A4 autoconv_zone(const T* __restrict__ sA, int K0, int da, int db,
-                                            A4 acc) {
+// Toeplitz steps d in [da, db) of autoconvolution tile K0 = 16j.  In the
+// zone d >= 8j, column t keeps its B operand only while d < dp(K0 + t) =
+// 8j + ceil(t/2); a lane past that reads four zeros from the left padding
+// instead, so zone steps run in the same pipelined loop.
+template <typename T>
+__device__ __forceinline__ typename Acc4<T>::type autoconv_tile(const T* __restrict__ sA, int K0,
+                                                               int da, int db) {
+  constexpr int S = kSegStride;
   const int lane = threadIdx.x & (kWave - 1);
   const int t = lane & 15, kk = lane >> 4;
-  const int lim = (K0 >> 1) + ((t + 1) >> 1);  // dp of this lane's column
-  for (int d = da; d < db; ++d) {
-    T x[4], y[4];
-    const int pa = pad_addr(16 * d + t - 4 * kk);
+  const int pa = pad_addr(16 * da + t - 4 * kk);
+  const int pb = pad_addr(16 * (K0 + t - da)) + 4 * kk;
+  const int live = (K0 >> 1) + ((t + 1) >> 1) - da;  // steps i < live use this column
+  const int zero = pad_addr(-kPadSeg * kSeg);       // first padding segment: always zero
+  return run_steps<T, Pipe<T>::kDepth>(db - da, [&](int i, T (&x)[4], T (&y)[4]) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) x[q] = sA[pa - q];
-    load4(sA + pad_addr(16 * (K0 + t - d)) + 4 * kk, y);
-    if (d >= lim) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) y[q] = T(0);
-    }
-    acc = mfma4(x, y, acc);
-  }
-  return acc;
+    for (int q = 0; q < 4; ++q) x[q] = sA[pa + i * S - q];
+    load4(sA + (i < live ? pb - i * S : zero), y);
+  });
 }
 
-// out[k] += sum over the matrix-core share of the symmetric autoconvolution
-// (see above) for all k < 16 * 16 * conv_tiles(n); sA padded, `out` exactly
-// zero before the call.  Same split and determinism as conv_mfma.
+// ------------------------------------------ dynamic, deterministic split --
+// The waves of a workgroup take work items in a fixed order from an LDS
+// counter (zero when the phase starts), so waves that the matrix-core
+// arbiter serves faster simply take more items and all finish together.
+// One wave computes each item in a fixed order; a tile is at most two items
+// whose partial sums are added onto an exactly-zero output (0 + x + y ==
+// 0 + y + x), so the result has the same bits whichever wave takes which
+// item.  A wave asks for its next ticket when it starts an item, so the
+// counter's latency hides behind the item's matrix work.
+__device__ __forceinline__ int take_ticket(int* ctr) {
+  int it = 0;
+  if ((threadIdx.x & (kWave - 1)) == 0) it = atomicAdd(ctr, 1);
+  return it;  // lane 0 holds the ticket
+}
+__device__ __forceinline__ int ticket_value(int it) {
+  return __builtin_amdgcn_readfirstlane(it);  // every lane is active: lane 0's ticket
+}
+
+// Autoconvolution tiles in decreasing cost (ties: lower index first), so the
+// last items handed out are the smallest: order[r] = tile of rank r.  Called
+// by every thread; the caller's next barrier publishes the table.
+__device__ __forceinline__ void autoconv_order(int n, int* order, int tid, int nthreads) {
+  const int N16 = nseg16(n), NT = conv_tiles(n);
+  for (int j = tid; j < NT; j += nthreads) {
+    const int cj = autoconv_cost(j, N16);
+    int r = 0;
+    for (int i = 0; i < NT; ++i) {
+      const int ci = autoconv_cost(i, N16);
+      r += (ci > cj) || (ci == cj && i < j);
+    }
+    order[r] = j;
+  }
+}
+
+// out[k] += the matrix-core share of the symmetric autoconvolution (see
+// above) for all k < 16 * 16 * conv_tiles(n); one item per tile, in the
+// order of autoconv_order.  sA padded, `out` exactly zero before the call.
+// Every wave of the workgroup calls this; returns without a barrier.
 template <typename T>
-__device__ void autoconv_mfma(const T* __restrict__ sA, T* out, int n, int wave, int nwaves) {
+__device__ void autoconv_mfma(const T* __restrict__ sA, T* out, int n, int* ctr,
+                              const int* order) {
   const int N16 = nseg16(n);
   const int NT = conv_tiles(n);
-  int U = 0, maxc = 0;
-  for (int j = 0; j < NT; ++j) {
-    const int c = autoconv_cost(j, N16);
-    U += c;
-    maxc = c > maxc ? c : maxc;
-  }
-  if (U == 0) return;
-  Share sh = make_share(U, maxc, wave, nwaves);
-  if (!sh.active) return;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int t = lane & 15;
-  int j = 0, base = 0;
-  while (base + autoconv_cost(j, N16) <= sh.u0) { base += autoconv_cost(j, N16); ++j; }
-  while (base < sh.u1 && j < NT) {
+  const int t = threadIdx.x & 15;
+  int it = ticket_value(take_ticket(ctr));
+  while (it < NT) {
+    const int next = take_ticket(ctr);
+    const int j = order[it];
     int dlo, dhi;
     autoconv_drange(j, N16, dlo, dhi);
-    const int cost = dhi >= dlo ? dhi - dlo + 1 : 0;
-    if (cost > 0) {
-      const int ua = sh.u0 > base ? sh.u0 : base;
-      const int ub = sh.u1 < base + cost ? sh.u1 : base + cost;
-      const int K0 = 16 * j, z = 8 * j;
-      const int da = dlo + (ua - base), db = dlo + (ub - base);
-      const int dm = db < z ? db : z;
-      typename Acc4<T>::type acc = {0, 0, 0, 0};
-      if (da < dm) acc = conv_tile<T>(sA, sA, K0, da, dm);
-      const int dz = da > z ? da : z;
-      if (dz < db) acc = autoconv_zone<T>(sA, K0, dz, db, acc);
-      put4(out, K0 + t, acc, (ua == base) && (ub == base + cost));
+    if (dhi >= dlo) {
+      const int K0 = 16 * j;
+      put4(out, K0 + t, autoconv_tile<T>(sA, K0, dlo, dhi + 1), true);
     }
-    base += cost;
-    ++j;
+    it = ticket_value(next);
   }
 }
 
@@ -505,31 +489,29 @@ __device__ __forceinline__ T autoconv_finish(const T* __restrict__ sA, T h, int 
   return b;
 }
 
-// ---------------------------------------------- MFMA correlation (split) ---
+// ------------------------------------------------------ MFMA correlation --
 // out[i] += sum_j c[i + j] a[j] for i < 256 * corr_tiles(n); c in the normal
 // layout, zero for indices >= 2n-1 up to cbuf_len(n); a padded.  `out` must
-// be exactly zero before the call.
+// be exactly zero before the call.  The tiles are all equally long, so each
+// is cut into a head and a tail of a twelfth: items [0, NT) are the heads,
+// [NT, 2NT) the tails, which even out the finish.
 template <typename T>
 __device__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* out, int n,
-                          int wave, int nwaves) {
+                          int* ctr) {
   const int N16 = nseg16(n);
   const int NT = corr_tiles(n);
   const int cost = corr_cost(N16);
-  const int U = NT * cost;
-  Share sh = make_share(U, cost, wave, nwaves);
-  if (!sh.active) return;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int t = lane & 15;
-  int j = sh.u0 / cost;
-  int base = j * cost;
-  while (base < sh.u1 && j < NT) {
-    const int ua = sh.u0 > base ? sh.u0 : base;
-    const int ub = sh.u1 < base + cost ? sh.u1 : base + cost;
+  const int tl = cost >= 12 ? cost / 12 : 0;
+  const int t = threadIdx.x & 15;
+  int it = ticket_value(take_ticket(ctr));
+  while (it < 2 * NT) {
+    const int next = take_ticket(ctr);
+    const int j = it < NT ? it : it - NT;
     const int I0 = 16 * j;
-    const auto acc = corr_tile<T>(c, sA, I0, I0 + (ua - base), I0 + (ub - base));
-    put4(out, I0 + t, acc, (ua == base) && (ub == base + cost));
-    base += cost;
-    ++j;
+    const int ea = it < NT ? I0 : I0 + cost - tl;
+    const int eb = it < NT ? I0 + cost - tl : I0 + cost;
+    if (ea < eb) put4(out, I0 + t, corr_tile<T>(c, sA, I0, ea, eb), tl == 0);
+    it = ticket_value(next);
   }
 }
 

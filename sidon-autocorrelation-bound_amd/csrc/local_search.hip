@@ -38,7 +38,10 @@ struct LsArgs {
 // Bytes of dynamic LDS for one candidate of length n.
 template <typename T>
 __host__ __device__ inline long long ls_lds_bytes(int n, int block) {
-  const long long elems = (long long)pad_len(n) + cbuf_len(n) + gbuf_len(n) + 2 * (block / kWave);
+  // + reduction slots (2 per wave), two work counters and the autoconvolution
+  // tile order (ints, one per tile; T is at least as wide as int)
+  const long long elems = (long long)pad_len(n) + cbuf_len(n) + gbuf_len(n) + 2 * (block / kWave) +
+                          2 + conv_tiles(n);
   return elems * (long long)sizeof(T);
 }
 
@@ -55,9 +58,13 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
   T* sC = sA + pad_len(n);
   T* sG = sC + cbuf_len(n);
   T* slots = sG + gbuf_len(n);  // 2 * NW
-  const int tid = threadIdx.x, wave = wave_id();
+  int* ctr = reinterpret_cast<int*>(slots + 2 * NW);  // [0] conv items, [1] corr items
+  int* order = ctr + 2;                                // autoconvolution tile order
+  const int tid = threadIdx.x;
   const int LA = pad_len(n), LC = cbuf_len(n), LG = gbuf_len(n);
 
+  if (tid == 0) { ctr[0] = 0; ctr[1] = 0; }
+  autoconv_order(n, order, tid, BD);
   for (int i = tid; i < LA; i += BD) sA[i] = T(0);
   for (int i = tid; i < LC; i += BD) sC[i] = T(0);
   for (int i = tid; i < LG; i += BD) sG[i] = T(0);
@@ -81,9 +88,10 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     const T pexp = A.pStart * dexp(lnPr * frac);
     const T lr = A.lrStart * dexp(lnLr * frac);
 
-    autoconv_mfma<T>(sA, sC, n, wave, NW);
+    autoconv_mfma<T>(sA, sC, n, ctr, order);
     __syncthreads();
 
+    if (tid == 0) ctr[0] = 0;  // every wave is past the autoconvolution
     T loc = T(0);
     for (int k = tid; k < nb; k += BD) {
       const T b = autoconv_finish(sA, sC[k], k);
@@ -107,9 +115,10 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     b2t *= beta2;
     const T bc1 = T(1) / (T(1) - b1t), bc2 = T(1) / (T(1) - b2t);
 
-    corr_mfma<T>(sC, sA, sG, n, wave, NW);
+    corr_mfma<T>(sC, sA, sG, n, ctr + 1);
     __syncthreads();
 
+    if (tid == 0) ctr[1] = 0;  // every wave is past the correlation
     T sloc = T(0);
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) {
@@ -204,8 +213,12 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   T* sC = sA + pad_len(n);
   T* sG = sC + cbuf_len(n);
   T* slots = sG + gbuf_len(n);
-  const int tid = threadIdx.x, wave = wave_id();
+  int* ctr = reinterpret_cast<int*>(slots + 2 * NW);
+  int* order = ctr + 2;
+  const int tid = threadIdx.x;
   const int LA = pad_len(n), LC = cbuf_len(n), LG = gbuf_len(n);
+  if (tid == 0) { ctr[0] = 0; ctr[1] = 0; }
+  autoconv_order(n, order, tid, BD);
   for (int i = tid; i < LA; i += BD) sA[i] = T(0);
   for (int i = tid; i < LC; i += BD) sC[i] = T(0);
   for (int i = tid; i < LG; i += BD) sG[i] = T(0);
@@ -214,7 +227,7 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   for (int i = tid; i < n; i += BD) { const T x = a[i]; pad_put(sA, i, x); sl += x; }
   const T S = block_sum<T, NW>(sl, slots);  // barrier also publishes sA
   const int nb = 2 * n - 1;
-  autoconv_mfma<T>(sA, sC, n, wave, NW);
+  autoconv_mfma<T>(sA, sC, n, ctr, order);
   __syncthreads();
   T loc = T(0);
   for (int k = tid; k < nb; k += BD) {
@@ -235,7 +248,7 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   const T Rp = dexp(log(R) / A.p);
   if (A.J && tid == 0) A.J[cand] = T(2) * T(n) * M * Rp / (S * S);
   if (!A.grad) return;
-  corr_mfma<T>(sC, sA, sG, n, wave, NW);
+  corr_mfma<T>(sC, sA, sG, n, ctr + 1);
   __syncthreads();
   const T scale = T(4) * T(n) * Rp / (S * S);
   const T MS = M / S;
