@@ -423,41 +423,94 @@ __device__ __forceinline__ int ticket_value(int it) {
   return __builtin_amdgcn_readfirstlane(it);  // every lane is active: lane 0's ticket
 }
 
-// Autoconvolution tiles in decreasing cost (ties: lower index first), so the
-// last items handed out are the smallest: order[r] = tile of rank r.  Called
-// by every thread; the caller's next barrier publishes the table.
-__device__ __forceinline__ void autoconv_order(int n, int* order, int tid, int nthreads) {
-  const int N16 = nseg16(n), NT = conv_tiles(n);
+// Work items of a phase: (tile j, block steps [u0, u1) of the tile), packed
+// j << 22 | u0 << 11 | u1.  With `split`, tile j (cost c_j steps, rank r_j
+// in decreasing cost, ties by index) is cut into a head [0, c_j - s_j) and
+// a tail [c_j - s_j, c_j), s_j = c_j (1 + 24 r_j / (NT - 1)) / 50: 2 % of
+// the largest tile up to half of the smallest, none below 8 steps; without
+// it every tile is one item.  The list holds the heads (or whole tiles) in
+// decreasing size, then the tails in decreasing size (2 NT slots; empty
+// tails sort last and are skipped), so the items handed out last are the
+// smallest.  Equal-cost tiles (the correlation) need the split to end
+// together; the autoconvolution's tiles already range from 8 steps to N16/2
+// and run fastest whole.
+constexpr int kItemBits = 11;  // steps per tile < 2048, tiles < 1024
+__device__ __forceinline__ unsigned pack_item(int j, int u0, int u1) {
+  return ((unsigned)j << (2 * kItemBits)) | ((unsigned)u0 << kItemBits) | (unsigned)u1;
+}
+__device__ __forceinline__ void unpack_item(unsigned v, int& j, int& u0, int& u1) {
+  constexpr unsigned m = (1u << kItemBits) - 1;
+  j = (int)(v >> (2 * kItemBits));
+  u0 = (int)((v >> kItemBits) & m);
+  u1 = (int)(v & m);
+}
+
+// Builds the 2 NT items of a phase whose tile j costs cost(j) steps.  Every
+// thread of the workgroup calls this (it holds barriers); `scratch` is 3 NT
+// ints of LDS that nothing else uses meanwhile; ends with a barrier, so
+// `items` is published and `scratch` free on return.
+template <typename Cost>
+__device__ void build_items(int NT, const Cost& cost, bool split, unsigned* items, int* scratch,
+                            int tid, int nthreads) {
+  int* c = scratch;
+  int* h = scratch + NT;
+  int* tl = scratch + 2 * NT;
+  for (int j = tid; j < NT; j += nthreads) c[j] = cost(j);
+  __syncthreads();
   for (int j = tid; j < NT; j += nthreads) {
-    const int cj = autoconv_cost(j, N16);
+    const int cj = c[j];
     int r = 0;
-    for (int i = 0; i < NT; ++i) {
-      const int ci = autoconv_cost(i, N16);
-      r += (ci > cj) || (ci == cj && i < j);
+    for (int i = 0; i < NT; ++i) r += (c[i] > cj) || (c[i] == cj && i < j);
+    int sj = 0;
+    if (split && cj >= 8) {
+      sj = NT > 1 ? (int)((long long)cj * (NT - 1 + 24 * r) / (50LL * (NT - 1))) : cj / 50;
+      sj = sj < 1 ? 1 : (sj > cj - 1 ? cj - 1 : sj);
     }
-    order[r] = j;
+    h[j] = cj - sj;
+    tl[j] = sj;
   }
+  __syncthreads();
+  for (int j = tid; j < NT; j += nthreads) {
+    int rh = 0, rt = 0;
+    for (int i = 0; i < NT; ++i) {
+      rh += (h[i] > h[j]) || (h[i] == h[j] && i < j);
+      rt += (tl[i] > tl[j]) || (tl[i] == tl[j] && i < j);
+    }
+    items[rh] = pack_item(j, 0, h[j]);
+    items[NT + rt] = pack_item(j, h[j], c[j]);
+  }
+  __syncthreads();
+}
+
+// Work items of the symmetric autoconvolution (2 conv_tiles(n) entries).
+__device__ __forceinline__ void autoconv_items(int n, unsigned* items, int* scratch, int tid,
+                                               int nthreads) {
+  const int N16 = nseg16(n);
+  build_items(conv_tiles(n), [N16](int j) { return autoconv_cost(j, N16); }, false, items, scratch,
+              tid, nthreads);
 }
 
 // out[k] += the matrix-core share of the symmetric autoconvolution (see
-// above) for all k < 16 * 16 * conv_tiles(n); one item per tile, in the
-// order of autoconv_order.  sA padded, `out` exactly zero before the call.
-// Every wave of the workgroup calls this; returns without a barrier.
+// above) for all k < 16 * 16 * conv_tiles(n), item by item from the list of
+// autoconv_items.  sA padded, `out` exactly zero before the call.  Every
+// wave of the workgroup calls this; returns without a barrier.
 template <typename T>
 __device__ void autoconv_mfma(const T* __restrict__ sA, T* out, int n, int* ctr,
-                              const int* order) {
+                              const unsigned* items) {
   const int N16 = nseg16(n);
   const int NT = conv_tiles(n);
   const int t = threadIdx.x & 15;
   int it = ticket_value(take_ticket(ctr));
-  while (it < NT) {
+  while (it < 2 * NT) {
     const int next = take_ticket(ctr);
-    const int j = order[it];
-    int dlo, dhi;
-    autoconv_drange(j, N16, dlo, dhi);
-    if (dhi >= dlo) {
+    int j, u0, u1;
+    unpack_item(__builtin_amdgcn_readfirstlane(items[it]), j, u0, u1);  // wave-uniform
+    if (u0 < u1) {
+      int dlo, dhi;
+      autoconv_drange(j, N16, dlo, dhi);
       const int K0 = 16 * j;
-      put4(out, K0 + t, autoconv_tile<T>(sA, K0, dlo, dhi + 1), true);
+      put4(out, K0 + t, autoconv_tile<T>(sA, K0, dlo + u0, dlo + u1),
+           u0 == 0 && u1 == dhi + 1 - dlo);
     }
     it = ticket_value(next);
   }
@@ -490,27 +543,34 @@ __device__ __forceinline__ T autoconv_finish(const T* __restrict__ sA, T h, int 
 }
 
 // ------------------------------------------------------ MFMA correlation --
-// out[i] += sum_j c[i + j] a[j] for i < 256 * corr_tiles(n); c in the normal
-// layout, zero for indices >= 2n-1 up to cbuf_len(n); a padded.  `out` must
-// be exactly zero before the call.  The tiles are all equally long, so each
-// is cut into a head and a tail of a twelfth: items [0, NT) are the heads,
-// [NT, 2NT) the tails, which even out the finish.
+// Work items of the correlation (2 corr_tiles(n) entries): the tiles are all
+// equally long, so heads and tails ramp with the tile index.
+__device__ __forceinline__ void corr_items(int n, unsigned* items, int* scratch, int tid,
+                                           int nthreads) {
+  const int cost = corr_cost(nseg16(n));
+  build_items(corr_tiles(n), [cost](int) { return cost; }, true, items, scratch, tid, nthreads);
+}
+
+// out[i] += sum_j c[i + j] a[j] for i < 256 * corr_tiles(n), item by item
+// from the list of corr_items; c in the normal layout, zero for indices
+// >= 2n-1 up to cbuf_len(n); a padded.  `out` must be exactly zero before
+// the call.
 template <typename T>
 __device__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* out, int n,
-                          int* ctr) {
+                          int* ctr, const unsigned* items) {
   const int N16 = nseg16(n);
   const int NT = corr_tiles(n);
   const int cost = corr_cost(N16);
-  const int tl = cost >= 12 ? cost / 12 : 0;
   const int t = threadIdx.x & 15;
   int it = ticket_value(take_ticket(ctr));
   while (it < 2 * NT) {
     const int next = take_ticket(ctr);
-    const int j = it < NT ? it : it - NT;
-    const int I0 = 16 * j;
-    const int ea = it < NT ? I0 : I0 + cost - tl;
-    const int eb = it < NT ? I0 + cost - tl : I0 + cost;
-    if (ea < eb) put4(out, I0 + t, corr_tile<T>(c, sA, I0, ea, eb), tl == 0);
+    int j, u0, u1;
+    unpack_item(__builtin_amdgcn_readfirstlane(items[it]), j, u0, u1);  // wave-uniform
+    if (u0 < u1) {
+      const int I0 = 16 * j;
+      put4(out, I0 + t, corr_tile<T>(c, sA, I0, I0 + u0, I0 + u1), u0 == 0 && u1 == cost);
+    }
     it = ticket_value(next);
   }
 }

@@ -38,10 +38,10 @@ struct LsArgs {
 // Bytes of dynamic LDS for one candidate of length n.
 template <typename T>
 __host__ __device__ inline long long ls_lds_bytes(int n, int block) {
-  // + reduction slots (2 per wave), two work counters and the autoconvolution
-  // tile order (ints, one per tile; T is at least as wide as int)
+  // + reduction slots (2 per wave), two work counters and the work items of
+  // both products (32-bit each; T is at least as wide)
   const long long elems = (long long)pad_len(n) + cbuf_len(n) + gbuf_len(n) + 2 * (block / kWave) +
-                          2 + conv_tiles(n);
+                          2 + 2 * conv_tiles(n) + 2 * corr_tiles(n);
   return elems * (long long)sizeof(T);
 }
 
@@ -59,12 +59,14 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
   T* sG = sC + cbuf_len(n);
   T* slots = sG + gbuf_len(n);  // 2 * NW
   int* ctr = reinterpret_cast<int*>(slots + 2 * NW);  // [0] conv items, [1] corr items
-  int* order = ctr + 2;                                // autoconvolution tile order
+  unsigned* citems = reinterpret_cast<unsigned*>(ctr + 2);
+  unsigned* gitems = citems + 2 * conv_tiles(n);
   const int tid = threadIdx.x;
   const int LA = pad_len(n), LC = cbuf_len(n), LG = gbuf_len(n);
 
   if (tid == 0) { ctr[0] = 0; ctr[1] = 0; }
-  autoconv_order(n, order, tid, BD);
+  autoconv_items(n, citems, reinterpret_cast<int*>(sC), tid, BD);  // sC is scratch until zeroed
+  corr_items(n, gitems, reinterpret_cast<int*>(sC), tid, BD);
   for (int i = tid; i < LA; i += BD) sA[i] = T(0);
   for (int i = tid; i < LC; i += BD) sC[i] = T(0);
   for (int i = tid; i < LG; i += BD) sG[i] = T(0);
@@ -88,7 +90,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     const T pexp = A.pStart * dexp(lnPr * frac);
     const T lr = A.lrStart * dexp(lnLr * frac);
 
-    autoconv_mfma<T>(sA, sC, n, ctr, order);
+    autoconv_mfma<T>(sA, sC, n, ctr, citems);
     __syncthreads();
 
     if (tid == 0) ctr[0] = 0;  // every wave is past the autoconvolution
@@ -115,7 +117,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     b2t *= beta2;
     const T bc1 = T(1) / (T(1) - b1t), bc2 = T(1) / (T(1) - b2t);
 
-    corr_mfma<T>(sC, sA, sG, n, ctr + 1);
+    corr_mfma<T>(sC, sA, sG, n, ctr + 1, gitems);
     __syncthreads();
 
     if (tid == 0) ctr[1] = 0;  // every wave is past the correlation
@@ -214,11 +216,13 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   T* sG = sC + cbuf_len(n);
   T* slots = sG + gbuf_len(n);
   int* ctr = reinterpret_cast<int*>(slots + 2 * NW);
-  int* order = ctr + 2;
+  unsigned* citems = reinterpret_cast<unsigned*>(ctr + 2);
+  unsigned* gitems = citems + 2 * conv_tiles(n);
   const int tid = threadIdx.x;
   const int LA = pad_len(n), LC = cbuf_len(n), LG = gbuf_len(n);
   if (tid == 0) { ctr[0] = 0; ctr[1] = 0; }
-  autoconv_order(n, order, tid, BD);
+  autoconv_items(n, citems, reinterpret_cast<int*>(sC), tid, BD);
+  corr_items(n, gitems, reinterpret_cast<int*>(sC), tid, BD);
   for (int i = tid; i < LA; i += BD) sA[i] = T(0);
   for (int i = tid; i < LC; i += BD) sC[i] = T(0);
   for (int i = tid; i < LG; i += BD) sG[i] = T(0);
@@ -227,7 +231,7 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   for (int i = tid; i < n; i += BD) { const T x = a[i]; pad_put(sA, i, x); sl += x; }
   const T S = block_sum<T, NW>(sl, slots);  // barrier also publishes sA
   const int nb = 2 * n - 1;
-  autoconv_mfma<T>(sA, sC, n, ctr, order);
+  autoconv_mfma<T>(sA, sC, n, ctr, citems);
   __syncthreads();
   T loc = T(0);
   for (int k = tid; k < nb; k += BD) {
@@ -248,7 +252,7 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   const T Rp = dexp(log(R) / A.p);
   if (A.J && tid == 0) A.J[cand] = T(2) * T(n) * M * Rp / (S * S);
   if (!A.grad) return;
-  corr_mfma<T>(sC, sA, sG, n, ctr + 1);
+  corr_mfma<T>(sC, sA, sG, n, ctr + 1, gitems);
   __syncthreads();
   const T scale = T(4) * T(n) * Rp / (S * S);
   const T MS = M / S;
