@@ -61,6 +61,14 @@ __device__ __forceinline__ f64x4 mfma16x16x4(double a, double b, f64x4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Returns x unchanged but opaque to the optimiser, so values derived from it
+// inside a loop body are computed there rather than hoisted out of the loop
+// (where they would stay live, and spill, across the matrix-core loops).
+__device__ __forceinline__ int opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 // Wave index as a wave-uniform (scalar) value, so loops over a wave's share
 // of tiles compile to scalar control flow.
 __device__ __forceinline__ int wave_id() {
@@ -216,11 +224,9 @@ template <> struct Pipe<double> { static constexpr int kDepth = 2; };
 // i + D right after the MFMAs of step i have issued (the scheduling barriers
 // keep the reads there), so each read has D - 1 steps of MFMA time to land.
 // Nothing is read for steps outside [0, cnt).
-template <typename T, int D, typename Ld>
-__device__ __forceinline__ typename Acc4<T>::type run_steps(int cnt, const Ld& ld) {
+template <typename T, int D, typename Ld, typename A4>
+__device__ __forceinline__ void run_steps_into(int cnt, const Ld& ld, A4& acc0, A4& acc1) {
   static_assert(D % 2 == 0, "even depth keeps the accumulator parity static");
-  using A4 = typename Acc4<T>::type;
-  A4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
   T x[D][4], y[D][4];
   if (cnt < 2 * D) {  // short range (shares' edges, small n)
     for (int i = 0; i < cnt; ++i) {
@@ -228,7 +234,7 @@ __device__ __forceinline__ typename Acc4<T>::type run_steps(int cnt, const Ld& l
       if (i & 1) acc1 = mfma4(x[0], y[0], acc1);
       else acc0 = mfma4(x[0], y[0], acc0);
     }
-    return acc0 + acc1;
+    return;
   }
 #pragma unroll
   for (int s = 0; s < D; ++s) {
@@ -265,6 +271,11 @@ __device__ __forceinline__ typename Acc4<T>::type run_steps(int cnt, const Ld& l
       else acc0 = mfma4(x[s], y[s], acc0);
     }
   }
+}
+template <typename T, int D, typename Ld>
+__device__ __forceinline__ typename Acc4<T>::type run_steps(int cnt, const Ld& ld) {
+  typename Acc4<T>::type acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+  run_steps_into<T, D>(cnt, ld, acc0, acc1);
   return acc0 + acc1;
 }
 
@@ -309,6 +320,74 @@ __device__ __forceinline__ typename Acc4<T>::type corr_tile(const T* __restrict_
     for (int q = 0; q < 4; ++q) x[q] = c[pc + 16 * i + q];
     load4(sA + pb + i * S, y);
   });
+}
+
+// ------------------------------------------------ sparse weight vectors --
+// At high p most weights c_k = (b_k/M)^(p-1) are exact zeros.  The weight
+// pass records one bit per 16-element segment g of c (bit set: some c_k,
+// 16g <= k < 16g+16, is non-zero); Hankel step e reads c[16e .. 16e+30], so
+// it is needed only when segment e or e+1 is flagged.  Steps that are not
+// needed would add exact zeros, so skipping them leaves the sums unchanged
+// except for their grouping into runs.
+__host__ __device__ __forceinline__ int segmask_words(int n) {
+  return (cbuf_len(n) / kSeg + 31) / 32 + 1;
+}
+// Marks the segments of 64 consecutive weights c[k0 .. k0+63] (k0 a multiple
+// of 64, one per lane of the calling wave; lanes past the end inactive).
+__device__ __forceinline__ void mark_segments(unsigned* segmask, int k0, bool nonzero) {
+  const unsigned long long bal = __ballot(nonzero);
+  unsigned bits = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) bits |= ((bal >> (16 * q)) & 0xFFFFull) ? (1u << q) : 0u;
+  const int g = k0 >> 4;  // multiple of 4: the four bits share a word
+  if (bits && (threadIdx.x & (kWave - 1)) == __builtin_ctzll(__ballot(true)))
+    atomicOr(segmask + (g >> 5), bits << (g & 31));
+}
+// Needed-step bits of steps 32w .. 32w+31 (wave-uniform).
+__device__ __forceinline__ unsigned needed_word(const unsigned* segmask, int w) {
+  const unsigned s0 = __builtin_amdgcn_readfirstlane(segmask[w]);
+  const unsigned s1 = __builtin_amdgcn_readfirstlane(segmask[w + 1]);
+  return s0 | (s0 >> 1) | (s1 << 31);
+}
+// First step in [e, eb) whose needed bit equals `want` (eb if none).
+__device__ __forceinline__ int next_step(const unsigned* segmask, int e, int eb, bool want) {
+  while (e < eb) {
+    const int w = e >> 5;
+    unsigned bits = needed_word(segmask, w);
+    if (!want) bits = ~bits;
+    bits >>= (e & 31);
+    if (bits) {
+      const int f = e + __builtin_ctz(bits);
+      return f < eb ? f : eb;
+    }
+    e = (w + 1) << 5;
+  }
+  return eb;
+}
+
+// Hankel tile over the needed steps of [ea, eb) only (see corr_tile).
+template <typename T>
+__device__ __forceinline__ typename Acc4<T>::type corr_tile_sparse(const T* __restrict__ c,
+                                                                  const T* __restrict__ sA,
+                                                                  int I0, int ea, int eb,
+                                                                  const unsigned* segmask) {
+  constexpr int S = kSegStride;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int t = lane & 15, kk = lane >> 4;
+  typename Acc4<T>::type acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+  int e = next_step(segmask, ea, eb, true);
+  while (e < eb) {
+    const int e1 = next_step(segmask, e + 1, eb, false);
+    const int pc = 16 * e + t + 4 * kk;
+    const int pb = pad_addr(16 * (e - I0 - t)) + 4 * kk;
+    run_steps_into<T, Pipe<T>::kDepth>(e1 - e, [&](int i, T (&x)[4], T (&y)[4]) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) x[q] = c[pc + 16 * i + q];
+      load4(sA + pb + i * S, y);
+    }, acc0, acc1);
+    e = next_step(segmask, e1, eb, true);
+  }
+  return acc0 + acc1;
 }
 
 // Stores (whole tile) or atomically adds (half of a shared tile) a lane's 4
@@ -490,10 +569,11 @@ __device__ __forceinline__ void autoconv_items(int n, unsigned* items, int* scra
               tid, nthreads);
 }
 
-// out[k] += the matrix-core share of the symmetric autoconvolution (see
+// out[k] = the matrix-core share of the symmetric autoconvolution (see
 // above) for all k < 16 * 16 * conv_tiles(n), item by item from the list of
-// autoconv_items.  sA padded, `out` exactly zero before the call.  Every
-// wave of the workgroup calls this; returns without a barrier.
+// autoconv_items.  The autoconvolution's items are whole tiles, each stored
+// once (plain stores), so `out` needs no clearing between calls.  Every wave
+// of the workgroup calls this; returns without a barrier.
 template <typename T>
 __device__ void autoconv_mfma(const T* __restrict__ sA, T* out, int n, int* ctr,
                               const unsigned* items) {
@@ -505,12 +585,12 @@ __device__ void autoconv_mfma(const T* __restrict__ sA, T* out, int n, int* ctr,
     const int next = take_ticket(ctr);
     int j, u0, u1;
     unpack_item(__builtin_amdgcn_readfirstlane(items[it]), j, u0, u1);  // wave-uniform
-    if (u0 < u1) {
+    if (it < NT || u0 < u1) {  // heads always store, so every tile is written
       int dlo, dhi;
       autoconv_drange(j, N16, dlo, dhi);
+      const int cost = dhi >= dlo ? dhi + 1 - dlo : 0;
       const int K0 = 16 * j;
-      put4(out, K0 + t, autoconv_tile<T>(sA, K0, dlo + u0, dlo + u1),
-           u0 == 0 && u1 == dhi + 1 - dlo);
+      put4(out, K0 + t, autoconv_tile<T>(sA, K0, dlo + u0, dlo + u1), u0 == 0 && u1 == cost);
     }
     it = ticket_value(next);
   }
@@ -554,10 +634,11 @@ __device__ __forceinline__ void corr_items(int n, unsigned* items, int* scratch,
 // out[i] += sum_j c[i + j] a[j] for i < 256 * corr_tiles(n), item by item
 // from the list of corr_items; c in the normal layout, zero for indices
 // >= 2n-1 up to cbuf_len(n); a padded.  `out` must be exactly zero before
-// the call.
+// the call.  With `segmask` (the weight segments' non-zero bits) steps that
+// read only zero weights are skipped.
 template <typename T>
 __device__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* out, int n,
-                          int* ctr, const unsigned* items) {
+                          int* ctr, const unsigned* items, const unsigned* segmask) {
   const int N16 = nseg16(n);
   const int NT = corr_tiles(n);
   const int cost = corr_cost(N16);
@@ -569,7 +650,10 @@ __device__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* 
     unpack_item(__builtin_amdgcn_readfirstlane(items[it]), j, u0, u1);  // wave-uniform
     if (u0 < u1) {
       const int I0 = 16 * j;
-      put4(out, I0 + t, corr_tile<T>(c, sA, I0, I0 + u0, I0 + u1), u0 == 0 && u1 == cost);
+      put4(out, I0 + t,
+           segmask ? corr_tile_sparse<T>(c, sA, I0, I0 + u0, I0 + u1, segmask)
+                   : corr_tile<T>(c, sA, I0, I0 + u0, I0 + u1),
+           u0 == 0 && u1 == cost);
     }
     it = ticket_value(next);
   }
@@ -580,6 +664,21 @@ __device__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* 
 // (reference: src/kernels.cuh:35-44, :221-230); u^p = c (1 + d).
 __device__ __forceinline__ float lp_weight(float d, float pm1) {
   return d > -1.0f ? expf(pm1 * log1pf(d)) : 0.0f;
+}
+// Search-kernel form: the exponential as 2^(x log2 e) on the hardware
+// base-2 exponential (v_exp_f32), the fast exponential the reference's fp32
+// search uses at this point (src/kernels.cuh:43); log1p stays accurate, so
+// weights of near-maximal b_k keep their digits.  Relative error about
+// 1e-7 + 6e-8 |x|; weights below 2^-126 are flushed to zero (as a fast
+// exponential with denormals flushed does), which makes most weights exact
+// zeros at high p.
+__device__ __forceinline__ float lp_weight_fast(float d, float pm1) {
+  if (!(d > -1.0f)) return 0.0f;
+  const float y = pm1 * log1pf(d) * 1.44269504088896341f;
+  return y >= -126.0f ? __builtin_amdgcn_exp2f(y) : 0.0f;  // flushed like a fast exp: exact zeros
+}
+__device__ __forceinline__ double lp_weight_fast(double d, double pm1) {
+  return d > -1.0 ? exp(pm1 * log1p(d)) : 0.0;
 }
 __device__ __forceinline__ double lp_weight(double d, double pm1) {
   return d > -1.0 ? exp(pm1 * log1p(d)) : 0.0;

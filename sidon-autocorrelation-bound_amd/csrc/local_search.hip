@@ -38,10 +38,11 @@ struct LsArgs {
 // Bytes of dynamic LDS for one candidate of length n.
 template <typename T>
 __host__ __device__ inline long long ls_lds_bytes(int n, int block) {
-  // + reduction slots (2 per wave), two work counters and the work items of
-  // both products (32-bit each; T is at least as wide)
+  // + reduction slots (2 per wave), two work counters, the work items of
+  // both products and the weight segments' bits (32-bit each; T is at least
+  // as wide)
   const long long elems = (long long)pad_len(n) + cbuf_len(n) + gbuf_len(n) + 2 * (block / kWave) +
-                          2 + 2 * conv_tiles(n) + 2 * corr_tiles(n);
+                          2 + 2 * conv_tiles(n) + 2 * corr_tiles(n) + segmask_words(n);
   return elems * (long long)sizeof(T);
 }
 
@@ -61,8 +62,10 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
   int* ctr = reinterpret_cast<int*>(slots + 2 * NW);  // [0] conv items, [1] corr items
   unsigned* citems = reinterpret_cast<unsigned*>(ctr + 2);
   unsigned* gitems = citems + 2 * conv_tiles(n);
+  unsigned* segmask = gitems + 2 * corr_tiles(n);
   const int tid = threadIdx.x;
   const int LA = pad_len(n), LC = cbuf_len(n), LG = gbuf_len(n);
+  const int SW = segmask_words(n);
 
   if (tid == 0) { ctr[0] = 0; ctr[1] = 0; }
   autoconv_items(n, citems, reinterpret_cast<int*>(sC), tid, BD);  // sC is scratch until zeroed
@@ -94,6 +97,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     __syncthreads();
 
     if (tid == 0) ctr[0] = 0;  // every wave is past the autoconvolution
+    for (int w = tid; w < SW; w += BD) segmask[w] = 0u;  // the correlation is done with it
     T loc = T(0);
     for (int k = tid; k < nb; k += BD) {
       const T b = autoconv_finish(sA, sC[k], k);
@@ -107,24 +111,26 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     T rloc = T(0);
     for (int k = tid; k < nb; k += BD) {
       const T d = (sC[k] - M) / M;
-      const T c = lp_weight(d, pm1);
+      const T c = lp_weight_fast(d, pm1);
       rloc += c * (T(1) + d);
       sC[k] = c;
+      mark_segments(segmask, k - (tid & (kWave - 1)), c != T(0));
     }
-    const T R = block_sum<T, NW>(rloc, slots + NW);
+    const T R = block_sum<T, NW>(rloc, slots + NW);  // barrier also publishes segmask
 
     b1t *= beta1;
     b2t *= beta2;
     const T bc1 = T(1) / (T(1) - b1t), bc2 = T(1) / (T(1) - b2t);
 
-    corr_mfma<T>(sC, sA, sG, n, ctr + 1, gitems);
+    corr_mfma<T>(sC, sA, sG, n, ctr + 1, gitems, segmask);
+    const int ut = opaque(tid);  // element addresses below are built per step, not hoisted
     __syncthreads();
 
     if (tid == 0) ctr[1] = 0;  // every wave is past the correlation
     T sloc = T(0);
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) {
-      const int i = tid + BD * q;
+      const int i = ut + BD * q;
       if (i < n) {
         const T ai = sA[pad_addr(i)];
         const T gi = sG[i] / R - M;
@@ -153,10 +159,10 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     const T inv = T(1) / S;
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) {
-      const int i = tid + BD * q;
+      const int i = ut + BD * q;
       if (i < n) pad_put(sA, i, reset ? T(1) / nT : sG[i] * inv);
     }
-    for (int i = tid; i < LC; i += BD) sC[i] = T(0);
+    // sC needs no clearing: the autoconvolution stores every tile whole
     for (int i = tid; i < LG; i += BD) sG[i] = T(0);
     __syncthreads();
   }
@@ -218,9 +224,11 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   int* ctr = reinterpret_cast<int*>(slots + 2 * NW);
   unsigned* citems = reinterpret_cast<unsigned*>(ctr + 2);
   unsigned* gitems = citems + 2 * conv_tiles(n);
+  unsigned* segmask = gitems + 2 * corr_tiles(n);
   const int tid = threadIdx.x;
   const int LA = pad_len(n), LC = cbuf_len(n), LG = gbuf_len(n);
   if (tid == 0) { ctr[0] = 0; ctr[1] = 0; }
+  for (int w = tid; w < segmask_words(n); w += BD) segmask[w] = 0u;
   autoconv_items(n, citems, reinterpret_cast<int*>(sC), tid, BD);
   corr_items(n, gitems, reinterpret_cast<int*>(sC), tid, BD);
   for (int i = tid; i < LA; i += BD) sA[i] = T(0);
@@ -247,12 +255,13 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
     const T c = lp_weight(d, pm1);
     rloc += c * (T(1) + d);
     sC[k] = c;
+    mark_segments(segmask, k - (tid & (kWave - 1)), c != T(0));
   }
   const T R = block_sum<T, NW>(rloc, slots);
   const T Rp = dexp(log(R) / A.p);
   if (A.J && tid == 0) A.J[cand] = T(2) * T(n) * M * Rp / (S * S);
   if (!A.grad) return;
-  corr_mfma<T>(sC, sA, sG, n, ctr + 1, gitems);
+  corr_mfma<T>(sC, sA, sG, n, ctr + 1, gitems, segmask);
   __syncthreads();
   const T scale = T(4) * T(n) * Rp / (S * S);
   const T MS = M / S;
