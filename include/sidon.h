@@ -227,6 +227,13 @@ int sidon_generation_destroy(sidon_generation* g);
 int sidon_measure_fp32_ceilings(float* scratch, int scratch_len, double* valu_tflops,
                                 double* mfma_tflops, hipStream_t stream);
 
+/* Matrix block steps (four v_mfma_f32_16x16x4f32 per wave: 8192 FLOP per
+ * lane-group of 64, i.e. 4 * 2048 FLOP) that the local-search kernels have
+ * executed on the current device since the last call with reset != 0; the
+ * count excludes correlation steps skipped because their weights are all
+ * zero.  Synchronous.  For roofline accounting; no reference counterpart. */
+int sidon_ls_block_steps(unsigned long long* steps, int reset);
+
 /* ---- autotuner ---------------------------------------------------------- */
 
 /* Times the fused local search at length n for each workgroup size (4, 8 or

@@ -37,6 +37,7 @@ EXPORTS = (
     "sidon_generation_create", "sidon_generation_run", "sidon_generation_current",
     "sidon_generation_enqueue", "sidon_generation_destroy", "sidon_tune_local_search",
     "sidon_measure_fp32_ceilings",
+    "sidon_ls_block_steps",
 )
 
 
@@ -111,6 +112,7 @@ def _declare(L):
         "sidon_tune_local_search": ([_vp, _vp, _i, _i, _i, _i, C.POINTER(_i), C.POINTER(_f), _vp],
                                     _i),
         "sidon_measure_fp32_ceilings": ([_vp, _i, C.POINTER(_d), C.POINTER(_d), _vp], _i),
+        "sidon_ls_block_steps": ([C.POINTER(C.c_ulonglong), _i], _i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -334,6 +336,15 @@ def measure_fp32_ceilings(stream=None):
                                              C.byref(m), _stream(stream)),
            "sidon_measure_fp32_ceilings")
     return v.value, m.value
+
+
+def ls_block_steps(reset: bool = False) -> int:
+    """Matrix block steps (4 x 2048 FLOP each) the local-search kernels have
+    executed on the current device since the last reset (synchronous)."""
+    _need_gpu()
+    out = C.c_ulonglong()
+    _check(lib().sidon_ls_block_steps(C.byref(out), int(reset)), "sidon_ls_block_steps")
+    return out.value
 
 
 def tune_local_search(n: int, P: int = 64, iters: int = 20, repeats: int = 2, seed: int = 1,

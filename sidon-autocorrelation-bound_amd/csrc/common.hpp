@@ -214,6 +214,10 @@ __device__ __forceinline__ A4 mfma4(const T (&a)[4], const T (&b)[4], A4 acc) {
 }
 
 // Operand steps kept in flight by the tile loops (registers: 8 values/step).
+// Lane indices inside the tile code come from opaque(threadIdx.x), so the
+// lane-dependent address terms are rebuilt per phase instead of being
+// hoisted out of the step loop; that keeps a 16-wave workgroup (128
+// registers per lane) within its registers at this depth.
 template <typename T> struct Pipe { static constexpr int kDepth = 4; };
 template <> struct Pipe<double> { static constexpr int kDepth = 2; };
 
@@ -286,16 +290,16 @@ __device__ __forceinline__ typename Acc4<T>::type run_steps(int cnt, const Ld& l
 //   B operand:                      sB[16(K0 + t - d) + s]      (column = t)
 // LDS addresses are kept as integer element indices so the reads stay
 // ds_read.
-template <typename T>
+template <typename T, int D = Pipe<T>::kDepth>
 __device__ __forceinline__ typename Acc4<T>::type conv_tile(const T* __restrict__ sA,
                                                            const T* __restrict__ sB, int K0,
                                                            int da, int db) {
   constexpr int S = kSegStride;
-  const int lane = threadIdx.x & (kWave - 1);
+  const int lane = opaque((int)threadIdx.x) & (kWave - 1);
   const int t = lane & 15, kk = lane >> 4;
   const int pa = pad_addr(16 * da + t - 4 * kk);
   const int pb = pad_addr(16 * (K0 + t - da)) + 4 * kk;
-  return run_steps<T, Pipe<T>::kDepth>(db - da, [&](int i, T (&x)[4], T (&y)[4]) {
+  return run_steps<T, D>(db - da, [&](int i, T (&x)[4], T (&y)[4]) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) x[q] = sA[pa + i * S - q];
     load4(sB + pb - i * S, y);
@@ -306,16 +310,16 @@ __device__ __forceinline__ typename Acc4<T>::type conv_tile(const T* __restrict_
 // sum_s c[16e + row + s] * a[16(e - I0 - t) + s]  = G[16(I0 + t) + row].
 //   A operand: C_e[t][s] = c[16e + t + s]       (four consecutive elements)
 //   B operand: sA[16(e - I0 - t) + s]           (column = t)
-template <typename T>
+template <typename T, int D = Pipe<T>::kDepth>
 __device__ __forceinline__ typename Acc4<T>::type corr_tile(const T* __restrict__ c,
                                                            const T* __restrict__ sA, int I0,
                                                            int ea, int eb) {
   constexpr int S = kSegStride;
-  const int lane = threadIdx.x & (kWave - 1);
+  const int lane = opaque((int)threadIdx.x) & (kWave - 1);
   const int t = lane & 15, kk = lane >> 4;
   const int pc = 16 * ea + t + 4 * kk;
   const int pb = pad_addr(16 * (ea - I0 - t)) + 4 * kk;
-  return run_steps<T, Pipe<T>::kDepth>(eb - ea, [&](int i, T (&x)[4], T (&y)[4]) {
+  return run_steps<T, D>(eb - ea, [&](int i, T (&x)[4], T (&y)[4]) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) x[q] = c[pc + 16 * i + q];
     load4(sA + pb + i * S, y);
@@ -366,13 +370,14 @@ __device__ __forceinline__ int next_step(const unsigned* segmask, int e, int eb,
 }
 
 // Hankel tile over the needed steps of [ea, eb) only (see corr_tile).
-template <typename T>
+template <typename T, int D = Pipe<T>::kDepth>
 __device__ __forceinline__ typename Acc4<T>::type corr_tile_sparse(const T* __restrict__ c,
                                                                   const T* __restrict__ sA,
                                                                   int I0, int ea, int eb,
-                                                                  const unsigned* segmask) {
+                                                                  const unsigned* segmask,
+                                                                  int& steps) {
   constexpr int S = kSegStride;
-  const int lane = threadIdx.x & (kWave - 1);
+  const int lane = opaque((int)threadIdx.x) & (kWave - 1);
   const int t = lane & 15, kk = lane >> 4;
   typename Acc4<T>::type acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
   int e = next_step(segmask, ea, eb, true);
@@ -380,11 +385,12 @@ __device__ __forceinline__ typename Acc4<T>::type corr_tile_sparse(const T* __re
     const int e1 = next_step(segmask, e + 1, eb, false);
     const int pc = 16 * e + t + 4 * kk;
     const int pb = pad_addr(16 * (e - I0 - t)) + 4 * kk;
-    run_steps_into<T, Pipe<T>::kDepth>(e1 - e, [&](int i, T (&x)[4], T (&y)[4]) {
+    run_steps_into<T, D>(e1 - e, [&](int i, T (&x)[4], T (&y)[4]) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) x[q] = c[pc + 16 * i + q];
       load4(sA + pb + i * S, y);
     }, acc0, acc1);
+    steps += e1 - e;
     e = next_step(segmask, e1, eb, true);
   }
   return acc0 + acc1;
@@ -394,7 +400,7 @@ __device__ __forceinline__ typename Acc4<T>::type corr_tile_sparse(const T* __re
 // results of output column t: out[16 * col + row(kk, r)].
 template <typename T, typename A4>
 __device__ __forceinline__ void put4(T* out, int col, const A4& acc, bool whole) {
-  const int kk = (threadIdx.x & (kWave - 1)) >> 4;
+  const int kk = (opaque((int)threadIdx.x) & (kWave - 1)) >> 4;
   constexpr int RS = Acc4<T>::kRowStride;
   T* o = out + 16 * col + Acc4<T>::kRowMul * kk;
   if (whole) {
@@ -423,7 +429,7 @@ __device__ void conv_mfma(const T* __restrict__ sA, const T* __restrict__ sB, T*
   }
   Share sh = make_share(U, maxc, wave, nwaves);
   if (!sh.active) return;
-  const int lane = threadIdx.x & (kWave - 1);
+  const int lane = opaque((int)threadIdx.x) & (kWave - 1);
   const int t = lane & 15;
   int j = 0, base = 0;
   while (base + conv_cost(j, N16) <= sh.u0) { base += conv_cost(j, N16); ++j; }
@@ -467,17 +473,17 @@ __device__ __forceinline__ int autoconv_cost(int j, int N16) {
 // zone d >= 8j, column t keeps its B operand only while d < dp(K0 + t) =
 // 8j + ceil(t/2); a lane past that reads four zeros from the left padding
 // instead, so zone steps run in the same pipelined loop.
-template <typename T>
+template <typename T, int D = Pipe<T>::kDepth>
 __device__ __forceinline__ typename Acc4<T>::type autoconv_tile(const T* __restrict__ sA, int K0,
                                                                int da, int db) {
   constexpr int S = kSegStride;
-  const int lane = threadIdx.x & (kWave - 1);
+  const int lane = opaque((int)threadIdx.x) & (kWave - 1);
   const int t = lane & 15, kk = lane >> 4;
   const int pa = pad_addr(16 * da + t - 4 * kk);
   const int pb = pad_addr(16 * (K0 + t - da)) + 4 * kk;
   const int live = (K0 >> 1) + ((t + 1) >> 1) - da;  // steps i < live use this column
   const int zero = pad_addr(-kPadSeg * kSeg);       // first padding segment: always zero
-  return run_steps<T, Pipe<T>::kDepth>(db - da, [&](int i, T (&x)[4], T (&y)[4]) {
+  return run_steps<T, D>(db - da, [&](int i, T (&x)[4], T (&y)[4]) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) x[q] = sA[pa + i * S - q];
     load4(sA + (i < live ? pb - i * S : zero), y);
@@ -574,26 +580,29 @@ __device__ __forceinline__ void autoconv_items(int n, unsigned* items, int* scra
 // autoconv_items.  The autoconvolution's items are whole tiles, each stored
 // once (plain stores), so `out` needs no clearing between calls.  Every wave
 // of the workgroup calls this; returns without a barrier.
-template <typename T>
+template <typename T, int D = Pipe<T>::kDepth>
 __device__ void autoconv_mfma(const T* __restrict__ sA, T* out, int n, int* ctr,
-                              const unsigned* items) {
+                              const unsigned* items, unsigned* steps = nullptr) {
   const int N16 = nseg16(n);
   const int NT = conv_tiles(n);
-  const int t = threadIdx.x & 15;
+  const int t = opaque((int)threadIdx.x) & 15;
   int it = ticket_value(take_ticket(ctr));
+  int done = 0;  // block steps this wave ran
   while (it < 2 * NT) {
     const int next = take_ticket(ctr);
     int j, u0, u1;
     unpack_item(__builtin_amdgcn_readfirstlane(items[it]), j, u0, u1);  // wave-uniform
+    done += u1 - u0;
     if (it < NT || u0 < u1) {  // heads always store, so every tile is written
       int dlo, dhi;
       autoconv_drange(j, N16, dlo, dhi);
       const int cost = dhi >= dlo ? dhi + 1 - dlo : 0;
       const int K0 = 16 * j;
-      put4(out, K0 + t, autoconv_tile<T>(sA, K0, dlo + u0, dlo + u1), u0 == 0 && u1 == cost);
+      put4(out, K0 + t, autoconv_tile<T, D>(sA, K0, dlo + u0, dlo + u1), u0 == 0 && u1 == cost);
     }
     it = ticket_value(next);
   }
+  if (steps && done && (threadIdx.x & (kWave - 1)) == 0) atomicAdd(steps, (unsigned)done);
 }
 
 // b_k from the matrix-core share h = out[k] of autoconv_mfma: adds the
@@ -605,13 +614,18 @@ __device__ void autoconv_mfma(const T* __restrict__ sA, T* out, int n, int* ctr,
 template <typename T>
 __device__ __forceinline__ T autoconv_finish(const T* __restrict__ sA, T h, int k) {
   const int K = k >> 4, row = k & 15;
-  const int i0 = 16 * (((K + 1) >> 1) - 1) + row + 1;
+  const int dp = (K + 1) >> 1;
+  const int i0 = 16 * (dp - 1) + row + 1;
   const int cnt = ((k + 1) >> 1) - i0;
+  // a_{i0+u}: from pad_addr(i0), one segment boundary at most (skip the 4
+  // copy slots there); a_{k-i0-u} = a_{16(K-dp)+15-u}: one segment, descending
+  const int xs = pad_addr(i0), r1 = (i0 + kPadSeg * kSeg) & 15;
+  const int ys = pad_addr(16 * (K - dp) + 15);
   T r = h;
 #pragma unroll
   for (int u = 0; u < 15; ++u) {
-    const T x = sA[pad_addr(i0 + u)];
-    const T y = sA[pad_addr(k - i0 - u)];
+    const T x = sA[xs + u + (((r1 + u) >> 4) << 2)];
+    const T y = sA[ys - u];
     r += u < cnt ? x * y : T(0);
   }
   T b = r + r;
@@ -636,27 +650,34 @@ __device__ __forceinline__ void corr_items(int n, unsigned* items, int* scratch,
 // >= 2n-1 up to cbuf_len(n); a padded.  `out` must be exactly zero before
 // the call.  With `segmask` (the weight segments' non-zero bits) steps that
 // read only zero weights are skipped.
-template <typename T>
+template <typename T, int D = Pipe<T>::kDepth>
 __device__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* out, int n,
-                          int* ctr, const unsigned* items, const unsigned* segmask) {
+                          int* ctr, const unsigned* items, const unsigned* segmask,
+                          unsigned* steps = nullptr) {
   const int N16 = nseg16(n);
   const int NT = corr_tiles(n);
   const int cost = corr_cost(N16);
-  const int t = threadIdx.x & 15;
+  const int t = opaque((int)threadIdx.x) & 15;
   int it = ticket_value(take_ticket(ctr));
+  int done = 0;  // block steps this wave ran
   while (it < 2 * NT) {
     const int next = take_ticket(ctr);
     int j, u0, u1;
     unpack_item(__builtin_amdgcn_readfirstlane(items[it]), j, u0, u1);  // wave-uniform
     if (u0 < u1) {
       const int I0 = 16 * j;
-      put4(out, I0 + t,
-           segmask ? corr_tile_sparse<T>(c, sA, I0, I0 + u0, I0 + u1, segmask)
-                   : corr_tile<T>(c, sA, I0, I0 + u0, I0 + u1),
-           u0 == 0 && u1 == cost);
+      typename Acc4<T>::type acc;
+      if (segmask) {
+        acc = corr_tile_sparse<T, D>(c, sA, I0, I0 + u0, I0 + u1, segmask, done);
+      } else {
+        acc = corr_tile<T, D>(c, sA, I0, I0 + u0, I0 + u1);
+        done += u1 - u0;
+      }
+      put4(out, I0 + t, acc, u0 == 0 && u1 == cost);
     }
     it = ticket_value(next);
   }
+  if (steps && done && (threadIdx.x & (kWave - 1)) == 0) atomicAdd(steps, (unsigned)done);
 }
 
 // ------------------------------------------------------------ L^p weights --

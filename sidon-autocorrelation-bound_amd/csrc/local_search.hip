@@ -17,6 +17,12 @@
 
 namespace sidon {
 
+// Matrix block steps (4 MFMAs of 16x16x4 each) run by the local-search
+// kernels since the last sidon_ls_block_steps(reset) call: the work the
+// kernels actually executed, for roofline accounting (correlation steps that
+// would read only zero weights are skipped and not counted).
+__device__ unsigned long long g_block_steps = 0ull;
+
 template <typename T>
 struct LsArgs {
   T* pop;
@@ -38,17 +44,19 @@ struct LsArgs {
 // Bytes of dynamic LDS for one candidate of length n.
 template <typename T>
 __host__ __device__ inline long long ls_lds_bytes(int n, int block) {
-  // + reduction slots (2 per wave), two work counters, the work items of
-  // both products and the weight segments' bits (32-bit each; T is at least
-  // as wide)
+  // + reduction slots (2 per wave), three counters (work items of each
+  // product, matrix steps run), the work items of both products and the
+  // weight segments' bits (32-bit each; T is at least as wide)
   const long long elems = (long long)pad_len(n) + cbuf_len(n) + gbuf_len(n) + 2 * (block / kWave) +
-                          2 + 2 * conv_tiles(n) + 2 * corr_tiles(n) + segmask_words(n);
+                          3 + 2 * conv_tiles(n) + 2 * corr_tiles(n) + segmask_words(n);
   return elems * (long long)sizeof(T);
 }
 
 template <typename T, int BD, int QMAX>
 __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) {
   constexpr int NW = BD / kWave;
+  // operand steps in flight (fp64 operands take twice the registers)
+  constexpr int D = sizeof(T) == 8 ? 2 : 4;
   const int n = A.ns ? A.ns[cand] : A.n;
   if (A.skip && A.skip[cand]) return;
   if (n <= 0) return;
@@ -60,14 +68,15 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
   T* sG = sC + cbuf_len(n);
   T* slots = sG + gbuf_len(n);  // 2 * NW
   int* ctr = reinterpret_cast<int*>(slots + 2 * NW);  // [0] conv items, [1] corr items
-  unsigned* citems = reinterpret_cast<unsigned*>(ctr + 2);
+  unsigned* nsteps = reinterpret_cast<unsigned*>(ctr + 2);  // matrix block steps run
+  unsigned* citems = nsteps + 1;
   unsigned* gitems = citems + 2 * conv_tiles(n);
   unsigned* segmask = gitems + 2 * corr_tiles(n);
   const int tid = threadIdx.x;
   const int LA = pad_len(n), LC = cbuf_len(n), LG = gbuf_len(n);
   const int SW = segmask_words(n);
 
-  if (tid == 0) { ctr[0] = 0; ctr[1] = 0; }
+  if (tid == 0) { ctr[0] = 0; ctr[1] = 0; *nsteps = 0u; }
   autoconv_items(n, citems, reinterpret_cast<int*>(sC), tid, BD);  // sC is scratch until zeroed
   corr_items(n, gitems, reinterpret_cast<int*>(sC), tid, BD);
   for (int i = tid; i < LA; i += BD) sA[i] = T(0);
@@ -93,13 +102,14 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     const T pexp = A.pStart * dexp(lnPr * frac);
     const T lr = A.lrStart * dexp(lnLr * frac);
 
-    autoconv_mfma<T>(sA, sC, n, ctr, citems);
+    autoconv_mfma<T, D>(sA, sC, n, ctr, citems, nsteps);
     __syncthreads();
 
     if (tid == 0) ctr[0] = 0;  // every wave is past the autoconvolution
     for (int w = tid; w < SW; w += BD) segmask[w] = 0u;  // the correlation is done with it
+    const int st = opaque(tid);  // per-step copy: nothing derived from it is hoisted
     T loc = T(0);
-    for (int k = tid; k < nb; k += BD) {
+    for (int k = st; k < nb; k += BD) {
       const T b = autoconv_finish(sA, sC[k], k);
       sC[k] = b;
       loc = b > loc ? b : loc;
@@ -109,12 +119,12 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
 
     const T pm1 = pexp - T(1);
     T rloc = T(0);
-    for (int k = tid; k < nb; k += BD) {
+    for (int k = st; k < nb; k += BD) {
       const T d = (sC[k] - M) / M;
       const T c = lp_weight_fast(d, pm1);
       rloc += c * (T(1) + d);
       sC[k] = c;
-      mark_segments(segmask, k - (tid & (kWave - 1)), c != T(0));
+      mark_segments(segmask, k - (st & (kWave - 1)), c != T(0));
     }
     const T R = block_sum<T, NW>(rloc, slots + NW);  // barrier also publishes segmask
 
@@ -122,7 +132,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     b2t *= beta2;
     const T bc1 = T(1) / (T(1) - b1t), bc2 = T(1) / (T(1) - b2t);
 
-    corr_mfma<T>(sC, sA, sG, n, ctr + 1, gitems, segmask);
+    corr_mfma<T, D>(sC, sA, sG, n, ctr + 1, gitems, segmask, nsteps);
     const int ut = opaque(tid);  // element addresses below are built per step, not hoisted
     __syncthreads();
 
@@ -168,6 +178,8 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
   }
 
   for (int i = tid; i < n; i += BD) a[i] = sA[pad_addr(i)];
+  __syncthreads();  // every wave has added its matrix steps
+  if (tid == 0) atomicAdd(&g_block_steps, (unsigned long long)*nsteps);
   if (A.mom) {
     T* mo = A.mom + (size_t)cand * (size_t)A.stride;
 #pragma unroll
@@ -214,6 +226,7 @@ struct GradArgs {
 template <typename T, int BD>
 __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   constexpr int NW = BD / kWave;
+  constexpr int D = (BD >= 1024 || sizeof(T) == 8) ? 2 : 4;
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = A.n, cand = blockIdx.x;
   const T* a = A.a + (size_t)cand * n;
@@ -222,7 +235,7 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   T* sG = sC + cbuf_len(n);
   T* slots = sG + gbuf_len(n);
   int* ctr = reinterpret_cast<int*>(slots + 2 * NW);
-  unsigned* citems = reinterpret_cast<unsigned*>(ctr + 2);
+  unsigned* citems = reinterpret_cast<unsigned*>(ctr + 3);
   unsigned* gitems = citems + 2 * conv_tiles(n);
   unsigned* segmask = gitems + 2 * corr_tiles(n);
   const int tid = threadIdx.x;
@@ -239,7 +252,7 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   for (int i = tid; i < n; i += BD) { const T x = a[i]; pad_put(sA, i, x); sl += x; }
   const T S = block_sum<T, NW>(sl, slots);  // barrier also publishes sA
   const int nb = 2 * n - 1;
-  autoconv_mfma<T>(sA, sC, n, ctr, citems);
+  autoconv_mfma<T, D>(sA, sC, n, ctr, citems);
   __syncthreads();
   T loc = T(0);
   for (int k = tid; k < nb; k += BD) {
@@ -261,7 +274,7 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   const T Rp = dexp(log(R) / A.p);
   if (A.J && tid == 0) A.J[cand] = T(2) * T(n) * M * Rp / (S * S);
   if (!A.grad) return;
-  corr_mfma<T>(sC, sA, sG, n, ctr + 1, gitems, segmask);
+  corr_mfma<T, D>(sC, sA, sG, n, ctr + 1, gitems, segmask);
   __syncthreads();
   const T scale = T(4) * T(n) * Rp / (S * S);
   const T MS = M / S;
@@ -437,4 +450,16 @@ extern "C" long long sidon_ls_lds_bytes_impl(int kernel, int n, int block) {
     case 4: return ls_lds_bytes<float>(n, block);
     default: return -1;
   }
+}
+
+extern "C" int sidon_ls_block_steps(unsigned long long* out, int reset) {
+  if (!out) return SIDON_ERR_ARG;
+  int rc = SIDON_OK;
+  if (!device_state(&rc)) return rc;
+  SIDON_HIP_TRY(hipMemcpyFromSymbol(out, HIP_SYMBOL(g_block_steps), sizeof(*out)));
+  if (reset) {
+    const unsigned long long zero = 0ull;
+    SIDON_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_block_steps), &zero, sizeof(zero)));
+  }
+  return SIDON_OK;
 }

@@ -150,3 +150,54 @@ def test_ragged_uneven_lengths(S, W):
         finally:
             S.set_ls_block(int(n), 0)
         np.testing.assert_array_equal(got[p, :n], single.cpu().numpy()[0])
+
+
+def _dense_block_steps(n: int) -> int:
+    """Matrix block steps of one candidate-step when every weight c_k,
+    k < 2n-1, is non-zero: the symmetric autoconvolution's tiles (block
+    offsets max(0, 16j-N16+1) <= d <= min(16j+15, N16, 8j+7)) plus the
+    correlation steps e in [16j, 16j+N16+15) whose weight window
+    c[16e .. 16e+30] reaches below 2n-1, i.e. e <= (2n-2) // 16."""
+    N16 = (n + 15) // 16
+    NTc = (((2 * n - 1) + 15) // 16 + 15) // 16
+    conv = 0
+    for j in range(NTc):
+        lo, hi = max(0, 16 * j - N16 + 1), min(16 * j + 15, N16, 8 * j + 7)
+        conv += max(0, hi - lo + 1)
+    last = (2 * n - 2) // 16
+    corr = sum(max(0, min(16 * j + N16 + 14, last) - 16 * j + 1) for j in range((N16 + 15) // 16))
+    return conv + corr
+
+
+@pytest.mark.parametrize("n", [600, 2100])
+def test_block_step_count_dense(S, n):
+    """All weights non-zero (strictly positive heights, p = 4): the kernel
+    runs exactly the dense symmetric schedule."""
+    print(f"[ls] block-step count n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    t = dev(rng.uniform(0.5, 1.0, size=(3, n)))
+    S.ls_block_steps(reset=True)
+    S.local_search(t, 2, 4.0, 4.0, 0.01, 0.01)
+    torch.cuda.synchronize()
+    assert S.ls_block_steps(reset=True) == 3 * 2 * _dense_block_steps(n)
+
+
+def test_high_p_skips_zero_weights_and_tracks_oracle(S, O, W):
+    """At p = 1e4 most weights are exact zeros: fewer matrix steps run, and
+    the trajectory still tracks the fp64 oracle."""
+    print("[ls] high-p sparse weights", flush=True)
+    n = 2100
+    a = W.structured_prior(2, n, seed=77).astype(np.float64)
+    t = dev(a)
+    S.ls_block_steps(reset=True)
+    S.local_search(t, 4, 1e4, 1e4, 0.02, 0.02)
+    torch.cuda.synchronize()
+    steps = S.ls_block_steps(reset=True)
+    assert steps < 0.9 * 2 * 4 * _dense_block_steps(n)
+    got = t.cpu().numpy().astype(np.float64)
+    for p in range(2):
+        want, *_ = O.local_search(a[p].astype(np.float32).astype(np.float64), 4, 1e4, 1e4,
+                                  0.02, 0.02)
+        err = np.abs(got[p] - want) / np.abs(want).max()
+        assert np.quantile(err, 0.999) < 2e-3, err.max()
+        assert abs(O.F_hostorder(got[p]) - O.F_hostorder(want)) < 1e-4 * O.F_hostorder(want)
