@@ -575,36 +575,6 @@ __device__ __forceinline__ void autoconv_items(int n, unsigned* items, int* scra
               tid, nthreads);
 }
 
-// out[k] = the matrix-core share of the symmetric autoconvolution (see
-// above) for all k < 16 * 16 * conv_tiles(n), item by item from the list of
-// autoconv_items.  The autoconvolution's items are whole tiles, each stored
-// once (plain stores), so `out` needs no clearing between calls.  Every wave
-// of the workgroup calls this; returns without a barrier.
-template <typename T, int D = Pipe<T>::kDepth>
-__device__ void autoconv_mfma(const T* __restrict__ sA, T* out, int n, int* ctr,
-                              const unsigned* items, unsigned* steps = nullptr) {
-  const int N16 = nseg16(n);
-  const int NT = conv_tiles(n);
-  const int t = opaque((int)threadIdx.x) & 15;
-  int it = ticket_value(take_ticket(ctr));
-  int done = 0;  // block steps this wave ran
-  while (it < 2 * NT) {
-    const int next = take_ticket(ctr);
-    int j, u0, u1;
-    unpack_item(__builtin_amdgcn_readfirstlane(items[it]), j, u0, u1);  // wave-uniform
-    done += u1 - u0;
-    if (it < NT || u0 < u1) {  // heads always store, so every tile is written
-      int dlo, dhi;
-      autoconv_drange(j, N16, dlo, dhi);
-      const int cost = dhi >= dlo ? dhi + 1 - dlo : 0;
-      const int K0 = 16 * j;
-      put4(out, K0 + t, autoconv_tile<T, D>(sA, K0, dlo + u0, dlo + u1), u0 == 0 && u1 == cost);
-    }
-    it = ticket_value(next);
-  }
-  if (steps && done && (threadIdx.x & (kWave - 1)) == 0) atomicAdd(steps, (unsigned)done);
-}
-
 // b_k from the matrix-core share h = out[k] of autoconv_mfma: adds the
 // products 16(dp-1) + row < i < k/2 (dp = ceil(K/2), K = k/16, row = k%16;
 // at most 15 of them) in ascending i, doubles, and adds a_{k/2}^2 for even k.
@@ -635,6 +605,65 @@ __device__ __forceinline__ T autoconv_finish(const T* __restrict__ sA, T h, int 
   }
   return b;
 }
+
+// Atomic max of a non-negative value into an LDS slot holding one (IEEE bits
+// of non-negative floats order like unsigned integers).
+__device__ __forceinline__ void atomic_max_nonneg(float* p, float v) {
+  atomicMax(reinterpret_cast<unsigned*>(p), __float_as_uint(v));
+}
+__device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
+  atomicMax(reinterpret_cast<unsigned long long*>(p),
+            static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+
+// out[k] = the matrix-core share of the symmetric autoconvolution (see
+// above) for all k < 16 * 16 * conv_tiles(n), item by item from the list of
+// autoconv_items.  The autoconvolution's items are whole tiles, each stored
+// once (plain stores), so `out` needs no clearing between calls.  With
+// `bmax` (an LDS slot that holds 0 when the phase starts) each tile's wave
+// also finishes its outputs (autoconv_finish: out[k] = b_k) and raises
+// *bmax to the tile's largest b_k, so the finishing work overlaps the other
+// waves' matrix work.  Every wave of the workgroup calls this; returns
+// without a barrier.
+template <typename T, int D = Pipe<T>::kDepth>
+__device__ void autoconv_mfma(const T* __restrict__ sA, T* out, int n, int* ctr,
+                              const unsigned* items, unsigned* steps = nullptr,
+                              T* bmax = nullptr) {
+  const int N16 = nseg16(n);
+  const int NT = conv_tiles(n);
+  const int t = opaque((int)threadIdx.x) & 15;
+  int it = ticket_value(take_ticket(ctr));
+  int done = 0;  // block steps this wave ran
+  while (it < 2 * NT) {
+    const int next = take_ticket(ctr);
+    int j, u0, u1;
+    unpack_item(__builtin_amdgcn_readfirstlane(items[it]), j, u0, u1);  // wave-uniform
+    done += u1 - u0;
+    if (it < NT || u0 < u1) {  // heads always store, so every tile is written
+      int dlo, dhi;
+      autoconv_drange(j, N16, dlo, dhi);
+      const int cost = dhi >= dlo ? dhi + 1 - dlo : 0;
+      const int K0 = 16 * j;
+      auto acc = autoconv_tile<T, D>(sA, K0, dlo + u0, dlo + u1);
+      if (bmax) {  // whole tiles: this wave owns the outputs, so it finishes them
+        const int kk = (opaque((int)threadIdx.x) & (kWave - 1)) >> 4;
+        T m = T(0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int k = 16 * (K0 + t) + Acc4<T>::kRowMul * kk + Acc4<T>::kRowStride * r;
+          acc[r] = autoconv_finish(sA, acc[r], k);
+          m = acc[r] > m ? acc[r] : m;
+        }
+        m = wave_max(m);
+        if ((threadIdx.x & (kWave - 1)) == 0) atomic_max_nonneg(bmax, m);
+      }
+      put4(out, K0 + t, acc, u0 == 0 && u1 == cost);
+    }
+    it = ticket_value(next);
+  }
+  if (steps && done && (threadIdx.x & (kWave - 1)) == 0) atomicAdd(steps, (unsigned)done);
+}
+
 
 // ------------------------------------------------------ MFMA correlation --
 // Work items of the correlation (2 corr_tiles(n) entries): the tiles are all

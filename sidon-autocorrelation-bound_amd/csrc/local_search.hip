@@ -44,11 +44,12 @@ struct LsArgs {
 // Bytes of dynamic LDS for one candidate of length n.
 template <typename T>
 __host__ __device__ inline long long ls_lds_bytes(int n, int block) {
-  // + reduction slots (2 per wave), three counters (work items of each
-  // product, matrix steps run), the work items of both products and the
-  // weight segments' bits (32-bit each; T is at least as wide)
+  // + reduction slots (2 per wave), the running max of b, three counters
+  // (work items of each product, matrix steps run), the work items of both
+  // products and the weight segments' bits (32-bit each; T is at least as
+  // wide)
   const long long elems = (long long)pad_len(n) + cbuf_len(n) + gbuf_len(n) + 2 * (block / kWave) +
-                          3 + 2 * conv_tiles(n) + 2 * corr_tiles(n) + segmask_words(n);
+                          1 + 3 + 2 * conv_tiles(n) + 2 * corr_tiles(n) + segmask_words(n);
   return elems * (long long)sizeof(T);
 }
 
@@ -67,7 +68,8 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
   T* sC = sA + pad_len(n);
   T* sG = sC + cbuf_len(n);
   T* slots = sG + gbuf_len(n);  // 2 * NW
-  int* ctr = reinterpret_cast<int*>(slots + 2 * NW);  // [0] conv items, [1] corr items
+  T* bmax = slots + 2 * NW;     // max_k b_k of the step (raised by the autoconvolution)
+  int* ctr = reinterpret_cast<int*>(bmax + 1);  // [0] conv items, [1] corr items
   unsigned* nsteps = reinterpret_cast<unsigned*>(ctr + 2);  // matrix block steps run
   unsigned* citems = nsteps + 1;
   unsigned* gitems = citems + 2 * conv_tiles(n);
@@ -76,7 +78,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
   const int LA = pad_len(n), LC = cbuf_len(n), LG = gbuf_len(n);
   const int SW = segmask_words(n);
 
-  if (tid == 0) { ctr[0] = 0; ctr[1] = 0; *nsteps = 0u; }
+  if (tid == 0) { ctr[0] = 0; ctr[1] = 0; *nsteps = 0u; *bmax = T(0); }
   autoconv_items(n, citems, reinterpret_cast<int*>(sC), tid, BD);  // sC is scratch until zeroed
   corr_items(n, gitems, reinterpret_cast<int*>(sC), tid, BD);
   for (int i = tid; i < LA; i += BD) sA[i] = T(0);
@@ -102,27 +104,31 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     const T pexp = A.pStart * dexp(lnPr * frac);
     const T lr = A.lrStart * dexp(lnLr * frac);
 
-    autoconv_mfma<T, D>(sA, sC, n, ctr, citems, nsteps);
+    autoconv_mfma<T, D>(sA, sC, n, ctr, citems, nsteps, bmax);  // sC[k] = b_k, *bmax = M
     __syncthreads();
 
     if (tid == 0) ctr[0] = 0;  // every wave is past the autoconvolution
     for (int w = tid; w < SW; w += BD) segmask[w] = 0u;  // the correlation is done with it
     const int st = opaque(tid);  // per-step copy: nothing derived from it is hoisted
-    T loc = T(0);
-    for (int k = st; k < nb; k += BD) {
-      const T b = autoconv_finish(sA, sC[k], k);
-      sC[k] = b;
-      loc = b > loc ? b : loc;
-    }
-    const T M = block_max<T, NW>(loc, slots);
+    const T M = *bmax;
     if (!(M > T(0))) break;
 
     const T pm1 = pexp - T(1);
+    // fp32 weights flush to zero once (p-1) log2(b/M) < -126, i.e. b < M exp(-126 ln2 / (p-1));
+    // below 0.999 of that bound (a margin of >= 1.4e-3 in the exponent for p >= 2, far above
+    // its rounding error) the weight is certainly zero and its transcendentals are skipped
+    T cut = T(0);
+    if constexpr (sizeof(T) == 4)
+      if (pm1 >= T(1)) cut = M * T(0.999) * dexp(T(-87.33654475055310898657) / pm1);
     T rloc = T(0);
     for (int k = st; k < nb; k += BD) {
-      const T d = (sC[k] - M) / M;
-      const T c = lp_weight_fast(d, pm1);
-      rloc += c * (T(1) + d);
+      const T b = sC[k];
+      T c = T(0);
+      if (b >= cut) {
+        const T d = (b - M) / M;
+        c = lp_weight_fast(d, pm1);
+        rloc += c * (T(1) + d);
+      }
       sC[k] = c;
       mark_segments(segmask, k - (st & (kWave - 1)), c != T(0));
     }
@@ -136,7 +142,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     const int ut = opaque(tid);  // element addresses below are built per step, not hoisted
     __syncthreads();
 
-    if (tid == 0) ctr[1] = 0;  // every wave is past the correlation
+    if (tid == 0) { ctr[1] = 0; *bmax = T(0); }  // every wave is past the correlation (and M)
     T sloc = T(0);
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) {
@@ -234,13 +240,14 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   T* sC = sA + pad_len(n);
   T* sG = sC + cbuf_len(n);
   T* slots = sG + gbuf_len(n);
-  int* ctr = reinterpret_cast<int*>(slots + 2 * NW);
+  T* bmax = slots + 2 * NW;
+  int* ctr = reinterpret_cast<int*>(bmax + 1);
   unsigned* citems = reinterpret_cast<unsigned*>(ctr + 3);
   unsigned* gitems = citems + 2 * conv_tiles(n);
   unsigned* segmask = gitems + 2 * corr_tiles(n);
   const int tid = threadIdx.x;
   const int LA = pad_len(n), LC = cbuf_len(n), LG = gbuf_len(n);
-  if (tid == 0) { ctr[0] = 0; ctr[1] = 0; }
+  if (tid == 0) { ctr[0] = 0; ctr[1] = 0; *bmax = T(0); }
   for (int w = tid; w < segmask_words(n); w += BD) segmask[w] = 0u;
   autoconv_items(n, citems, reinterpret_cast<int*>(sC), tid, BD);
   corr_items(n, gitems, reinterpret_cast<int*>(sC), tid, BD);
@@ -252,15 +259,9 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   for (int i = tid; i < n; i += BD) { const T x = a[i]; pad_put(sA, i, x); sl += x; }
   const T S = block_sum<T, NW>(sl, slots);  // barrier also publishes sA
   const int nb = 2 * n - 1;
-  autoconv_mfma<T, D>(sA, sC, n, ctr, citems);
+  autoconv_mfma<T, D>(sA, sC, n, ctr, citems, nullptr, bmax);  // sC[k] = b_k, *bmax = M
   __syncthreads();
-  T loc = T(0);
-  for (int k = tid; k < nb; k += BD) {
-    const T b = autoconv_finish(sA, sC[k], k);
-    sC[k] = b;
-    loc = b > loc ? b : loc;
-  }
-  const T M = block_max<T, NW>(loc, slots + NW);
+  const T M = *bmax;
   const T pm1 = A.p - T(1);
   T rloc = T(0);
   for (int k = tid; k < nb; k += BD) {
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
     sC[k] = c;
     mark_segments(segmask, k - (tid & (kWave - 1)), c != T(0));
   }
-  const T R = block_sum<T, NW>(rloc, slots);
+  const T R = block_sum<T, NW>(rloc, slots + NW);
   const T Rp = dexp(log(R) / A.p);
   if (A.J && tid == 0) A.J[cand] = T(2) * T(n) * M * Rp / (S * S);
   if (!A.grad) return;
