@@ -15,7 +15,12 @@
 // tile owns those 256 heights for the whole solve, so the primal update,
 // the extrapolation and the write of gbar are all in the tile epilogue.  The
 // dual update is likewise fused into the Toeplitz-tile epilogue; Toeplitz
-// tiles (trapezoidal costs) are dealt to waves longest-first.
+// tiles (trapezoidal costs) are dealt to waves longest-first.  lambda is zero
+// on every slack constraint: the dual epilogue flags its non-zero 16-entry
+// segments (one ballot and one LDS atomicOr per tile) and the primal
+// correlation skips the Hankel steps that would read only zeros; two flag
+// arrays alternate between iterations, so one is cleared while the other is
+// in use.
 #include "common.hpp"
 #include "runtime.hpp"
 
@@ -27,7 +32,7 @@ constexpr int kMaxConvTiles = 80;   // n <= 10240
 template <int BD>
 __host__ __device__ inline long long fw_lds_bytes(int n) {
   const long long f = 2LL * pad_len(n) + cbuf_len(n) + 2 * (BD / kWave);
-  return f * 4 + 4LL * kMaxConvTiles;
+  return f * 4 + 4LL * kMaxConvTiles + 4LL * 2 * segmask_words(n);
 }
 
 struct FwArgs {
@@ -51,6 +56,8 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
   float* sL = sH + pad_len(n);     // lambda / autoconvolution
   float* slots = sL + cbuf_len(n);
   int* owner = reinterpret_cast<int*>(slots + 2 * NW);  // Toeplitz tile -> wave
+  const int SW = segmask_words(n);
+  unsigned* lmask = reinterpret_cast<unsigned*>(owner + kMaxConvTiles);  // 2 x SW: non-zero lambda
   const int tid = threadIdx.x, wave = wave_id();
   const int lane = tid & (kWave - 1), t = lane & 15, kk = lane >> 4;
   const int LA = pad_len(n), LC = cbuf_len(n);
@@ -107,9 +114,13 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
     }
     for (int i = tid; i < n; i += BD) pad_put(sH, i, sA[pad_addr(i)]);
     for (int k = tid; k < LC; k += BD) sL[k] = 0.0f;
+    for (int w = tid; w < 2 * SW; w += BD) lmask[w] = 0u;
     __syncthreads();
 
     for (int it = 0; it < A.cpIters; ++it) {
+      unsigned* mcur = lmask + (it & 1) * SW;
+      unsigned* mnext = lmask + ((it + 1) & 1) * SW;
+      for (int w = tid; w < SW; w += BD) mnext[w] = 0u;  // last read by iteration it-1
       // dual: lambda_k <- [lambda_k + sigma ((a * gbar)_k - M)]_+
       for (int j = 0; j < NTc; ++j) {
         if (owner[j] != wave) continue;
@@ -117,14 +128,21 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
         conv_drange(j, N16, dlo, dhi);
         const f32x4 acc = conv_tile<float>(sA, sH, 16 * j, dlo, dhi + 1);
         const int k0 = 16 * (16 * j + t) + 4 * kk;
+        bool nz = false;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int k = k0 + q;
           if (k < nb) {
-            const float l = sL[k] + sig * (acc[q] - M);
-            sL[k] = l > 0.0f ? l : 0.0f;
+            float l = sL[k] + sig * (acc[q] - M);
+            l = l > 0.0f ? l : 0.0f;
+            sL[k] = l;
+            nz = nz || l != 0.0f;
           }
         }
+        // segment 16j + t is non-zero if any of its lanes (t, t+16, t+32, t+48) saw one
+        const unsigned long long bal = __ballot(nz);
+        const unsigned seg = (unsigned)((bal | (bal >> 16) | (bal >> 32) | (bal >> 48)) & 0xFFFFull);
+        if (seg && lane == 0) atomicOr(mcur + (j >> 1), seg << (16 * (j & 1)));
       }
       __syncthreads();
       // primal: g_new <- [g + tau (1 - A' lambda)]_+, gbar <- 2 g_new - g
@@ -133,7 +151,8 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
         const int j = wave + NW * u;
         if (j >= NTg) continue;
         const int I0 = 16 * j;
-        const f32x4 c = corr_tile<float>(sL, sA, I0, I0, I0 + N16 + 15);
+        int steps = 0;
+        const f32x4 c = corr_tile_sparse<float>(sL, sA, I0, I0, I0 + N16 + 15, mcur, steps);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int i = 16 * (I0 + t) + 4 * kk + q;
