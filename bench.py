@@ -13,7 +13,10 @@ sample of candidates run back to back on the stream), the roofline of the
 local-search kernel against the 157.3 TFLOP/s dense FP32 matrix peak, and the
 CPU baseline: the reference's own fp64 CPU twin of the kernel (tools/probe.cpp
 --opt, built from source into oracle/_ref/) timed on this host, or the oracle
-port when that binary is absent.  Prints one JSON line.
+port when that binary is absent.  Then F-evaluations per second: the fp64
+evaluation kernel over 512 candidates at n = 512, 4096 and 8400, beside the
+reference's long-double OpenMP verifier (tools/probe.cpp --verify) on the same
+host.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -32,6 +35,7 @@ sys.path.insert(0, str(REPO))
 
 METRIC = "candidate-steps/s at n=8400, batched vs plain, F within 1e-12 of FP64"
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X dense FP32 (matrix and vector)
+PEAK_FP64 = 78.6  # TFLOP/s, MI355X dense FP64
 
 
 def parse():
@@ -114,6 +118,54 @@ def cpu_baseline(n: int, seconds: float, gpu_pop, gpu_F):
             "kind": "port",
             "sample": f"oracle fp64 local search, n={n}, {threads} candidates x {iters} "
                       f"iterations, {dt:.1f} s"}, fcheck
+
+
+def f_evals(S, W, torch, with_cpu: bool, seed: int):
+    """fp64 F-evaluations per second at n = 512, 4096, 8400: sidon_eval_f64 over
+    a batch of 512 structured-prior vectors (HIP events), beside the
+    reference's long-double verifier (tools/probe.cpp --verify, OpenMP, one
+    process per vector as the reference runs it) on this host."""
+    import numpy as np
+    out = []
+    probe = REPO / "oracle" / "_ref" / "probe"
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    for n in (512, 4096, 8400):
+        P = 512
+        pop = torch.as_tensor(W.structured_prior(P, n, seed=seed + n), device="cuda")
+        stream = torch.cuda.current_stream()
+        S.eval_f64(pop)
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 5
+        ev0.record(stream)
+        for _ in range(reps):
+            S.eval_f64(pop)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        gpu = P * reps / (ev0.elapsed_time(ev1) / 1e3)
+        # k_eval: n^2 fp64 FMAs (2 n^2 FLOP) per candidate on the vector ALUs
+        tf = 2.0 * n * n * gpu / 1e12
+        row = {"n": n, "gpu_evals_per_s": gpu, "gpu_batch": P, "gpu_fp64_tflops": tf,
+               "frac_fp64_peak": tf / PEAK_FP64}
+        if with_cpu and probe.exists():
+            sys.path.insert(0, str(REPO / "oracle"))
+            import oracle as O
+            import tempfile
+            a = pop[0].double().cpu().numpy()
+            with tempfile.TemporaryDirectory() as td:
+                path = os.path.join(td, f"vec{n}.txt")
+                O.write_vec(path, a, O.F_hostorder(a))
+                env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+                k, t0 = 0, time.perf_counter()
+                while k < 50 and (k < 3 or time.perf_counter() - t0 < 2.0):
+                    subprocess.run([str(probe), "--verify", path], capture_output=True, env=env,
+                                   check=True)
+                    k += 1
+                cpu = k / (time.perf_counter() - t0)
+            row.update({"cpu_evals_per_s": cpu, "cpu": f"tools/probe.cpp --verify (long double, "
+                        f"OpenMP {threads} threads), {k} runs", "gpu_over_cpu": gpu / cpu})
+        out.append(row)
+    return out
 
 
 def main() -> int:
@@ -237,6 +289,7 @@ def main() -> int:
                                    F[:8].cpu().numpy())
         out["cpu_baseline"] = cpu
         out["f_check"] = fcheck
+    out["f_evals"] = f_evals(S, W, torch, not args.no_cpu, args.seed)
     print(json.dumps(out))
     return 0
 
