@@ -122,7 +122,7 @@ def cpu_baseline(n: int, seconds: float, gpu_pop, gpu_F):
 
 def f_evals(S, W, torch, with_cpu: bool, seed: int):
     """fp64 F-evaluations per second at n = 512, 4096, 8400: sidon_eval_f64 over
-    a batch of 512 structured-prior vectors (HIP events), beside the
+    a batch of 512 * (8400 // n)^2 structured-prior vectors (HIP events), beside the
     reference's long-double verifier (tools/probe.cpp --verify, OpenMP, one
     process per vector as the reference runs it) on this host."""
     import numpy as np
@@ -130,8 +130,10 @@ def f_evals(S, W, torch, with_cpu: bool, seed: int):
     probe = REPO / "oracle" / "_ref" / "probe"
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     for n in (512, 4096, 8400):
-        P = 512
-        pop = torch.as_tensor(W.structured_prior(P, n, seed=seed + n), device="cuda")
+        # batch of about the same work at every n (512 distinct vectors, tiled)
+        P = 512 * max(1, (8400 // n) ** 2)
+        pop = torch.as_tensor(W.structured_prior(512, n, seed=seed + n), device="cuda")
+        pop = pop.repeat(P // 512, 1)
         stream = torch.cuda.current_stream()
         S.eval_f64(pop)
         torch.cuda.synchronize()

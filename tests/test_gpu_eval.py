@@ -42,7 +42,25 @@ def test_plain_config_vectors_bitwise(S, O, W):
     np.testing.assert_array_equal(na.cpu().numpy(), cnt)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 15, 16, 17, 31, 33, 257, 1000, 2100, 8400, 9000])
+@pytest.mark.parametrize("n,tol", [(1000, 0.3), (2048, 0.05), (2100, 0.3), (4096, 0.02),
+                                   (8400, 0.3), (8400, 1e-3)])
+def test_active_count_many_tiles(S, O, W, n, tol):
+    """Loose tolerances flag many tiles (several recompute rounds per
+    workgroup); workgroup sizes 64 / 128 / 256 all appear.  Counts are exact
+    against the oracle's ascending-i autoconvolution, F bit for bit."""
+    print(f"[eval] active count n={n} tol={tol}", flush=True)
+    a = W.structured_prior(2, n, seed=n).astype(np.float32).astype(np.float64)
+    F, na = S.eval_f64(dev(a), active_tol=tol)
+    F, na = F.cpu().numpy(), na.cpu().numpy()
+    for p, r in enumerate(a):
+        b = O.autoconv(r)
+        M = b.max()
+        assert F[p] == O.F_hostorder(r)
+        assert na[p] == int((b >= M * (1.0 - tol)).sum())
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 15, 16, 17, 31, 33, 257, 1000, 2047, 2048, 2100, 4095,
+                               4096, 8400, 9000])
 def test_ragged_sizes_bitwise(S, O, n):
     print(f"[eval] size n={n}", flush=True)
     rng = np.random.default_rng(n)
