@@ -12,9 +12,10 @@
 // boustrophedon rounds (lane l of L takes ranks l, 2L-1-l, 2L+l, ...), which
 // gives every lane the same number of trips to about one percent; a lane
 // walks its tiles in one flat loop, so a wave never waits at a tile boundary.
-// b is never stored: per-tile maxima are kept, and the active-peak pass
-// recomputes only the outputs of tiles whose maximum reaches the threshold,
-// one output per lane, summed in the same order.
+// The workgroup is as wide as still deals two tiles a lane (up to 1024
+// lanes), for four waves a SIMD at the LDS footprint of one candidate.  b is
+// never stored: per-tile maxima are kept, and the outputs that may reach the
+// active-peak threshold are listed as their tiles finish (see k_eval).
 //
 // For fp32 heights every product is exact in fp64 and each b_k accumulates in
 // ascending i, and S is summed sequentially, exactly as hostF
@@ -42,9 +43,10 @@ __host__ __device__ inline int eval_slots(int n) {
   const int y = n + 2 * EPAD;
   return y + (y >> 4) + 1;
 }
-// heights, tile maxima, 2 NW reduction slots, S / flag / counter
+// heights; NT + 16 doubles of tile maxima (and, in k_eval, candidate
+// values); 2 NW reduction slots; S, running maximum and counters
 __host__ __device__ inline long long eval_lds_elems(int n, int block) {
-  return (long long)eval_slots(n) + eval_tiles(n) + 2 * (block / kWave) + 4;
+  return (long long)eval_slots(n) + eval_tiles(n) + 16 + 2 * (block / kWave) + 4;
 }
 
 template <typename Tin>
@@ -128,23 +130,58 @@ __device__ __forceinline__ void eval_trip(const double* E, int J, int d, double 
     for (int t = 0; t < 16; ++t) acc[t] = fma(x[s], w[t - s + 15], acc[t]);
 }
 
+// Ascending sequential sum over i of term(x_i, y_i) with x_i = a_i and
+// y_i = a_{k-i} (or 1 when k < 0), i from i0 to i1 rounded up to a multiple
+// of 8: heights past either end read zero cells, and a zero term leaves the
+// sum unchanged.  The next 8 pairs load while the current 8 are summed, so
+// the chain waits on the adds, not on LDS.
+template <class Acc, class Add>
+__device__ __forceinline__ void seq_sum(const double* E, int k, int i0, int i1, Acc& s,
+                                        Add&& add) {
+  double x[8], y[8], nx[8], ny[8];
+  auto load = [&](int i, double* X, double* Y) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      X[j] = E[eslot(i + j)];
+      Y[j] = k < 0 ? 1.0 : E[eslot(k - i - j)];
+    }
+  };
+  load(i0, nx, ny);
+  for (int i = i0; i <= i1; i += 8) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      x[j] = nx[j];
+      y[j] = ny[j];
+    }
+    if (i + 8 <= i1) load(i + 8, nx, ny);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) add(s, x[j], y[j]);
+  }
+}
+
 // b_k alone, ascending i, one FMA per term: the value eval_trip reaches (the
 // zero products a tile adds before and after the range leave a sum unchanged).
 __device__ double output_b(const double* E, int n, int k) {
   const int ilo = k - (n - 1) > 0 ? k - (n - 1) : 0;
   const int ihi = k < n - 1 ? k : n - 1;
   double s = 0.0;
-#pragma unroll 8
-  for (int i = ilo; i <= ihi; ++i) s = fma(E[eslot(i)], E[eslot(k - i)], s);
+  seq_sum(E, k, ilo, ihi, s, [](double& acc, double x, double y) { acc = fma(x, y, acc); });
+  return s;
+}
+
+// S = a_0 + a_1 + ... in sequence, as hostF sums it.
+__device__ double height_sum(const double* E, int n) {
+  double s = 0.0;
+  seq_sum(E, -1, 0, n - 1, s, [](double& acc, double x, double) { acc += x; });
   return s;
 }
 
 // Writes the tiles q with keep(tmax[q]) to list (which may alias tmax) and
 // returns their number; every thread of the workgroup calls it.  *cnt is 0 on
 // entry.  NT <= kFlagRounds * nthreads.
-template <class Keep>
-__device__ int compact_tiles(const double* tmax, int NT, int* list, int* cnt, int tid,
-                             int nthreads, Keep&& keep) {
+template <class T, class Keep>
+__device__ int compact_tiles(const T* tmax, int NT, int* list, int* cnt, int tid, int nthreads,
+                             Keep&& keep) {
   unsigned flags = 0;
   for (int j = 0; j < kFlagRounds; ++j) {
     const int q = tid + j * nthreads;
@@ -157,6 +194,14 @@ __device__ int compact_tiles(const double* tmax, int NT, int* list, int* cnt, in
   return *cnt;
 }
 
+// The active count needs the b_k at or above thr = M (1 - tol), M known only
+// once every tile is done.  A tile keeps, as it finishes, the outputs at or
+// above max(its maximum, the running maximum) * (1 - tol) in a candidate
+// list: both maxima are at most M, so the list holds every output that
+// reaches thr, with its value as computed, and the count needs no second
+// pass.  When the list overflows (or 0 <= tol <= 1 does not hold), the
+// outputs of the tiles whose maximum reaches thr are summed again instead;
+// tile maxima are kept as floats rounded up for that test.
 template <typename Tin, int BD>
 __global__ __launch_bounds__(BD) void k_eval(const Tin* __restrict__ pop, long long stride,
                                              const int* __restrict__ ns, int n_uniform,
@@ -166,19 +211,24 @@ __global__ __launch_bounds__(BD) void k_eval(const Tin* __restrict__ pop, long l
   extern __shared__ __align__(16) unsigned char smem[];
   double* E = reinterpret_cast<double*>(smem);
   const int NT = eval_tiles(n);
-  double* tmax = E + eval_slots(n);
-  double* slots = tmax + NT;
-  double* sS = slots + 2 * NW;
-  int* cnt = reinterpret_cast<int*>(sS + 2);
+  const int nb = 2 * n - 1;
+  float* tmaxf = reinterpret_cast<float*>(E + eval_slots(n));  // NT floats
+  double* cand = E + eval_slots(n) + (NT + 1) / 2;
+  const int C = NT - (NT + 1) / 2 + 16;
+  double* slots = cand + C;
+  double* sS = slots + 2 * NW;  // S, running maximum, counters
+  double* mrun = sS + 1;
+  int* cnt = reinterpret_cast<int*>(sS + 2);  // [0] tile list, [1] candidates
   const int tid = threadIdx.x;
   load_heights(pop + (size_t)blockIdx.x * stride, n, E, tid, BD);
-  if (tid == 0) *cnt = 0;
-  __syncthreads();
-  if (tid == 0) {  // sequential S, hostF order
-    double S = 0.0;
-    for (int i = 0; i < n; ++i) S += E[eslot(i)];
-    sS[0] = S;
+  if (tid == 0) {
+    cnt[0] = 0;
+    cnt[1] = 0;
+    *mrun = 0.0;
   }
+  __syncthreads();
+  const bool listed = nActive && tol >= 0.0 && tol <= 1.0;
+  const double keep = 1.0 - tol;
   double loc = 0.0;
   double acc[16];
 #pragma unroll
@@ -189,24 +239,45 @@ __global__ __launch_bounds__(BD) void k_eval(const Tin* __restrict__ pop, long l
       [&](int tq) {
         double mx = 0.0;
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
-          mx = acc[t] > mx ? acc[t] : mx;
-          acc[t] = 0.0;
-        }
-        tmax[tq] = mx;
+        for (int t = 0; t < 16; ++t) mx = acc[t] > mx ? acc[t] : mx;
+        float mf = (float)mx;
+        if ((double)mf < mx) mf = nextafterf(mf, HUGE_VALF);
+        tmaxf[tq] = mf;
         loc = mx > loc ? mx : loc;
+        if (listed) {
+          const double old = __longlong_as_double((long long)atomicMax(
+              reinterpret_cast<unsigned long long*>(mrun),
+              static_cast<unsigned long long>(__double_as_longlong(mx))));
+          const double lb = (old > mx ? old : mx) * keep;
+#pragma unroll
+          for (int t = 0; t < 16; ++t)
+            if (16 * tq + t < nb && acc[t] >= lb) {
+              const int pos = atomicAdd(&cnt[1], 1);
+              if (pos < C) cand[pos] = acc[t];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 16; ++t) acc[t] = 0.0;
       });
+  // sequential S, hostF order, after the tiles: the older waves of a SIMD win
+  // the issue arbitration and finish their tiles well before the younger
+  // ones, so the sum hides in that wait
+  if (tid == 0) sS[0] = height_sum(E, n);
   const double M = block_max<double, NW>(loc, slots);
   if (nActive) {
-    const int nb = 2 * n - 1;
-    const double thr = M * (1.0 - tol);
-    int* list = reinterpret_cast<int*>(tmax);
-    const int nf = compact_tiles(tmax, NT, list, cnt, tid, BD,
-                                 [thr](double v) { return v >= thr; });
+    const double thr = M * keep;
+    const int nc = cnt[1];
     double c = 0.0;
-    for (int g = tid; g < 16 * nf; g += BD) {
-      const int k = 16 * list[g >> 4] + (g & 15);
-      if (k < nb && output_b(E, n, k) >= thr) c += 1.0;
+    if (listed && nc <= C) {
+      for (int j = tid; j < nc; j += BD) c += cand[j] >= thr ? 1.0 : 0.0;
+    } else {
+      int* list = reinterpret_cast<int*>(tmaxf);
+      const int nf = compact_tiles(tmaxf, NT, list, cnt, tid, BD,
+                                   [thr](float v) { return (double)v >= thr; });
+      for (int g = tid; g < 16 * nf; g += BD) {
+        const int k = 16 * list[g >> 4] + (g & 15);
+        if (k < nb && output_b(E, n, k) >= thr) c += 1.0;
+      }
     }
     const double cc = block_sum<double, NW>(c, slots + NW);
     if (tid == 0) nActive[blockIdx.x] = (int)(cc + 0.5);
@@ -277,7 +348,7 @@ __device__ Acc3 output_exact(const double* E, int n, int k) {
   const int ilo = k - (n - 1) > 0 ? k - (n - 1) : 0;
   const int ihi = k < n - 1 ? k : n - 1;
   Acc3 s{0.0, 0.0, 0.0, true};
-  for (int i = ilo; i <= ihi; ++i) acc3_add(s, E[eslot(i)] * E[eslot(k - i)]);
+  seq_sum(E, k, ilo, ihi, s, [](Acc3& acc, double x, double y) { acc3_add(acc, x * y); });
   return s;
 }
 #pragma clang fp contract(on)
@@ -290,7 +361,7 @@ __global__ __launch_bounds__(BD) void k_eval_exact(const float* __restrict__ pop
   double* E = reinterpret_cast<double*>(smem);
   const int NT = eval_tiles(n);
   double* tmax = E + eval_slots(n);  // rounded tile maxima
-  double* slots = tmax + NT;
+  double* slots = tmax + NT + 16;
   double* sS = slots + 2 * NW;
   int* cnt = reinterpret_cast<int*>(sS + 2);
   const int tid = threadIdx.x;
@@ -352,13 +423,17 @@ __global__ __launch_bounds__(BD) void k_eval_exact(const float* __restrict__ pop
   }
 }
 
-// Workgroup size: the largest of 256 / 128 / 64 lanes that still deals at
-// least two tiles to every lane (with one tile a lane, most lanes wait for
-// the longest tile); at small n the small workgroups keep the CU occupied
-// through their small LDS footprint.
-inline int eval_block(int n) {
+// Workgroup size: the largest of 1024 / 512 / 256 / 128 / 64 lanes (up to
+// maxBD) that still deals at least two tiles to every lane (with one tile a
+// lane, most lanes wait for the longest tile).  Large workgroups give more
+// waves per CU at the same LDS footprint (the fp64 FMAs need about four waves
+// a SIMD); at small n the small workgroups keep the CU occupied through their
+// small LDS footprint.
+inline int eval_block(int n, int maxBD) {
   const int NT = eval_tiles(n);
-  return NT >= 512 ? 256 : NT >= 256 ? 128 : 64;
+  int bd = 64;
+  while (bd < maxBD && NT >= 4 * bd) bd *= 2;
+  return bd;
 }
 
 template <int BD, class Kernel, class... Args>
@@ -389,7 +464,13 @@ int eval_common(const Tin* pop, long long stride, const int* ns, int n, int P, d
                 int* nActive, double tol, hipStream_t stream) {
   if (!pop || !F || n < 1 || P < 0 || stride < n) return SIDON_ERR_ARG;
   if (P == 0) return SIDON_OK;
-  switch (eval_block(n)) {
+  switch (eval_block(n, 1024)) {
+    case 1024:
+      return launch_eval<1024>("eval", k_eval<Tin, 1024>, n, P, stream, pop, stride, ns, n, F,
+                               nActive, tol);
+    case 512:
+      return launch_eval<512>("eval", k_eval<Tin, 512>, n, P, stream, pop, stride, ns, n, F,
+                              nActive, tol);
     case 256:
       return launch_eval<256>("eval", k_eval<Tin, 256>, n, P, stream, pop, stride, ns, n, F,
                               nActive, tol);
@@ -409,7 +490,7 @@ using namespace sidon;
 
 extern "C" long long sidon_eval_lds_bytes_impl(int n) {
   if (n < 1) return -1;
-  return eval_lds_elems(n, eval_block(n)) * (long long)sizeof(double);
+  return eval_lds_elems(n, eval_block(n, 1024)) * (long long)sizeof(double);
 }
 
 extern "C" int sidon_eval_f64(const float* pop, int n, int P, double* F, int* nActive,
@@ -433,7 +514,11 @@ extern "C" int sidon_eval_exact(const float* pop, int n, int P, double* F, doubl
                                 int* ambiguous, hipStream_t stream) {
   if (!pop || !F || n < 1 || P < 0) return SIDON_ERR_ARG;
   if (P == 0) return SIDON_OK;
-  switch (eval_block(n)) {
+  // the exact accumulators take about 220 VGPRs: at most two waves a SIMD
+  switch (eval_block(n, 512)) {
+    case 512:
+      return launch_eval<512>("eval_exact", k_eval_exact<512>, n, P, stream, pop, n, F, M, S,
+                              ambiguous);
     case 256:
       return launch_eval<256>("eval_exact", k_eval_exact<256>, n, P, stream, pop, n, F, M, S,
                               ambiguous);
