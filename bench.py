@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0,
                     help="workgroup size of the local search at this n (0: the library's rule)")
+    ap.add_argument("--f-evals-only", action="store_true",
+                    help="run only the F-evaluation leg and print its rows as one JSON line")
     ap.add_argument("--profile-copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates the HBM "
                          "byte counters of a profiler run)")
@@ -166,8 +168,29 @@ def f_evals(S, W, torch, with_cpu: bool, seed: int):
                 cpu = k / (time.perf_counter() - t0)
             row.update({"cpu_evals_per_s": cpu, "cpu": f"tools/probe.cpp --verify (long double, "
                         f"OpenMP {threads} threads), {k} runs", "gpu_over_cpu": gpu / cpu})
+        if with_cpu:
+            hf = hostf_evals(pop[:512].double().cpu().numpy(), threads)
+            row.update({"cpu_hostf_evals_per_s": hf, "cpu_hostf": f"hostF-order fp64 "
+                        f"(src/main.cu:94-105, oracle restatement), {threads} threads over "
+                        f"vectors", "gpu_over_cpu_hostf": gpu / hf})
         out.append(row)
     return out
+
+
+def hostf_evals(vecs, threads: int, seconds: float = 1.5) -> float:
+    """hostF-order fp64 F-evaluations per second on `threads` host threads, one
+    vector per task (the oracle's C restatement; ctypes releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, str(REPO / "oracle"))
+    import oracle as O
+    O.F_hostorder(vecs[0])  # load the library before timing
+    done, t0 = 0, time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        while done < 4 * threads or time.perf_counter() - t0 < seconds:
+            batch = [vecs[(done + j) % len(vecs)] for j in range(threads)]
+            list(ex.map(O.F_hostorder, batch))
+            done += threads
+    return done / (time.perf_counter() - t0)
 
 
 def main() -> int:
@@ -182,6 +205,12 @@ def main() -> int:
     S = importlib.import_module("sidon-autocorrelation-bound_amd")
     W = importlib.import_module("sidon-autocorrelation-bound_amd.workloads")
     S.init(0)
+    host = {"nproc": os.cpu_count(),
+            "threads": int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)}
+    if args.f_evals_only:
+        print(json.dumps({"f_evals": f_evals(S, W, torch, not args.no_cpu, args.seed),
+                          "host": host}))
+        return 0
     n, P, iters = args.n, args.P, args.iters
     sched = (4.0, 1e4, 0.085, 0.0015)
     if args.block:
@@ -292,6 +321,7 @@ def main() -> int:
         out["cpu_baseline"] = cpu
         out["f_check"] = fcheck
     out["f_evals"] = f_evals(S, W, torch, not args.no_cpu, args.seed)
+    out["host"] = host
     print(json.dumps(out))
     return 0
 
