@@ -140,11 +140,17 @@ int sidon_frank_wolfe(float* pop, const int* skip, int n, int P, int rounds, int
  * (src/main.cu:491-502). */
 int sidon_refine(const float* src, int n, int P, float* dst, hipStream_t stream);
 
-/* Offspring j = operator op[j] applied to parent p1[j]: op 2 multiplies each
- * height by exp(strength[j] * z), z ~ N(0,1) from Philox4x32-10 keyed by
- * (seed, gen, j, i); any other op clones.  Then clamp >= 0 and renormalise
- * sum = 1 (uniform 1/n if the sum is not positive).  Replaces kVariation<float>
- * op 2 and the clone branch (src/kernels.cuh:582-801, launch src/main.cu:737-742). */
+/* Offspring j = operator op[j] applied to parents p1[j] and p2[j] (p2 may be
+ * NULL: p2 = p1), the reference's fourteen operators (src/kernels.cuh:614-787):
+ * 0 blend, 1 segment crossover, 2 log-normal noise exp(strength[j] z),
+ * 3 smooth Fourier perturbation, 4 additive noise, 5 sparsify, 6 symmetrise /
+ * reflect, 7 shift, 8 block splice, 9 random restart, 10 spike, 11 structured
+ * seed, 12 gap surgery, 13 endpoint atom; any other op clones.  Each height is
+ * formed in fp64 and rounded once to fp32; normals z come from Philox4x32-10
+ * keyed by (seed, gen, j, i), the 64 operator parameters from Philox keyed by
+ * (seed, gen, j).  Then renormalise sum = 1 (uniform 1/n if the sum is not
+ * positive).  Replaces kVariation<float> (src/kernels.cuh:582-801, launch
+ * src/main.cu:737-742). */
 int sidon_variation(const float* pop, float* out, int n, int nOff, const int* p1,
                     const int* p2, const int* op, const float* strength,
                     unsigned long long seed, unsigned long long gen, hipStream_t stream);
@@ -155,6 +161,14 @@ int sidon_variation_ragged(const float* pop, const int* ns, float* out, int* ns_
                            int stride, int nOff, const int* p1, const int* p2, const int* op,
                            const float* strength, unsigned long long seed,
                            const unsigned long long* gen_dev, hipStream_t stream);
+
+/* Support signature of each row (P rows of n heights): the support is
+ * {i : a_i > mean * tol} with the fp32 mean summed in the reference kernel's
+ * order; sig[p] = XOR over the support of a 64-bit mix of i, nnz[p] = its
+ * size.  Replaces kSupportSig<float> (src/kernels.cuh:803-849, launch
+ * src/main.cu:180-184, tol 0.02), the key of the tabu and cell-best tables. */
+int sidon_support_sig(const float* pop, int n, int P, float tol, unsigned long long* sig,
+                      int* nnz, hipStream_t stream);
 
 /* Island selection plan on the device: rank each island (P/NI members) by
  * fit (ties by index); every migrate_every generations (gen > 0) the best of

@@ -78,6 +78,15 @@ def lib():
     L.so_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
     L.so_variation.argtypes = [_fp, C.c_int, C.c_int, C.c_float, C.c_uint64, C.c_uint64,
                                C.c_int, _fp]
+    L.so_variation2.argtypes = [_fp, C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_uint64,
+                                C.c_uint64, C.c_int, _fp]
+    L.so_op_params.argtypes = [C.c_uint64, C.c_uint64, C.c_int, _fp]
+    L.so_op_uses_mean.argtypes = [C.c_int]
+    L.so_op_uses_mean.restype = C.c_int
+    L.so_support_sig.argtypes = [_fp, C.c_int, C.c_float, C.POINTER(C.c_uint64),
+                                 C.POINTER(C.c_int)]
+    L.so_sig_mix.argtypes = [C.c_int]
+    L.so_sig_mix.restype = C.c_uint64
     L.so_normal.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int]
     L.so_normal.restype = C.c_float
     L.so_select_plan.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
@@ -257,6 +266,41 @@ def variation(A, op, sigma, seed, gen, slot):
     out = np.zeros_like(A)
     lib().so_variation(A, A.size, op, sigma, seed, gen, slot, out)
     return out
+
+
+def variation2(A, B, op, sigma, seed, gen, slot):
+    """All fourteen operators (B = None: B = A)."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    out = np.zeros_like(A)
+    if B is None:
+        lib().so_variation2(A, None, A.size, op, sigma, seed, gen, slot, out)
+    else:
+        B = np.ascontiguousarray(B, dtype=np.float32)
+        assert B.size == A.size
+        lib().so_variation2(A, B.ctypes.data, A.size, op, sigma, seed, gen, slot, out)
+    return out
+
+
+def op_params(seed, gen, slot):
+    prm = np.zeros(64, np.float32)
+    lib().so_op_params(seed, gen, slot, prm)
+    return prm
+
+
+def op_uses_mean(op) -> bool:
+    return bool(lib().so_op_uses_mean(op))
+
+
+def support_sig(a, tol=0.02):
+    """(signature, support size) as kSupportSig computes them."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    sig, nnz = C.c_uint64(), C.c_int()
+    lib().so_support_sig(a, a.size, tol, C.byref(sig), C.byref(nnz))
+    return sig.value, nnz.value
+
+
+def sig_mix(i) -> int:
+    return lib().so_sig_mix(i)
 
 
 def normal(seed, gen, slot, i) -> float:

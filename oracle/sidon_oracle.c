@@ -565,21 +565,159 @@ float so_normal(uint64_t seed, uint64_t gen, int slot, int i) {
   return (float)normal_d(seed, gen, slot, i);
 }
 
-/* kernels.cuh:629-636 (op 2), :784-786 (clone), :791-800 (renormalise).
- * The product is formed in fp64 and rounded once to fp32; the sum is the
- * correctly rounded fp64 sum of the fp32 heights. */
-void so_variation(const float* A, int n, int op, float sigma, uint64_t seed, uint64_t gen,
-                  int slot, float* out) {
-  for (int i = 0; i < n; ++i) {
-    if (op == 2) {
-      const double z = normal_d(seed, gen, slot, i);
-      float x = (float)((double)A[i] * exp((double)sigma * z));
-      out[i] = x > 0.0f ? x : 0.0f;
-    } else {
-      out[i] = A[i];
-    }
+/* The 64 per-offspring operator parameters in (0, 1] (the role of sPar,
+ * kernels.cuh:604-609): Philox4x32-10 counters {q, slot, gen, gen>>32 ^ tag},
+ * q = 0..15, four words each, u = ((x >> 8) + 1) 2^-24 (exact in fp32). */
+#define SO_STREAM_PARAMS 0x50524D53u
+void so_op_params(uint64_t seed, uint64_t gen, int slot, float prm[64]) {
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int q = 0; q < 16; ++q) {
+    const uint32_t ctr[4] = {(uint32_t)q, (uint32_t)slot, (uint32_t)gen,
+                             (uint32_t)(gen >> 32) ^ SO_STREAM_PARAMS};
+    uint32_t x[4];
+    so_philox4x32_10(ctr, key, x);
+    for (int w = 0; w < 4; ++w) prm[4 * q + w] = (float)((x[w] >> 8) + 1u) * (1.0f / 16777216.0f);
   }
+}
+
+/* One height of offspring operator `op` (kernels.cuh:614-787), in fp64 and
+ * rounded once to fp32 by the caller: A the first parent, B the second (the
+ * crossover ops 0, 1, 8), u the parameters, mean = sum(A) / n (correctly
+ * rounded sum), sg the strength.  The same expression, operation for
+ * operation, is the device's vary(). */
+static double vary_one(int op, int i, int n, const float* A, const float* B, const double* u,
+                       double mean, double sg, uint64_t seed, uint64_t gen, int slot) {
+  const double a = A[i];
+  const double twopi = 6.283185307179586476925286766559;
+  switch (op) {
+    case 0: {  /* blend crossover, lambda in [-0.3, 1.3] */
+      const double lam = -0.3 + 1.6 * u[0];
+      const double x = lam * a + (1.0 - lam) * (double)B[i];
+      return x > 0.0 ? x : 0.0;
+    }
+    case 1: {  /* two-point segment crossover */
+      int c1 = (int)(u[1] * n), c2 = (int)(u[2] * n);
+      if (c1 > c2) { const int t = c1; c1 = c2; c2 = t; }
+      return (i >= c1 && i < c2) ? (double)B[i] : a;
+    }
+    case 2: {  /* log-normal multiplicative noise */
+      const double x = a * exp(sg * normal_d(seed, gen, slot, i));
+      return x > 0.0 ? x : 0.0;
+    }
+    case 3: {  /* smooth low-frequency perturbation */
+      const int nf = 1 + (int)(u[3] * 5.0);
+      double m = 0.0;
+      for (int f = 0; f < nf; ++f) {
+        const double freq = 1.0 + u[4 + f] * 12.0;
+        m += cos(twopi * freq * (double)i / (double)n + twopi * u[12 + f]) / (double)nf;
+      }
+      const double x = a * (1.0 + sg * 3.0 * m);
+      return x > 0.0 ? x : 0.0;
+    }
+    case 4: {  /* additive Gaussian noise scaled by the mean height */
+      const double x = a + sg * mean * normal_d(seed, gen, slot, i);
+      return x > 0.0 ? x : 0.0;
+    }
+    case 5: {  /* sparsify below a fraction of the mean */
+      const double thr = mean * (0.05 + 0.9 * u[20]);
+      return a < thr ? 0.0 : a;
+    }
+    case 6: {  /* symmetrise, or hard reflect */
+      const double r = A[n - 1 - i];
+      return u[21] < 0.5 ? r : 0.5 * (a + r);
+    }
+    case 7: {  /* shift by up to n/8 either way, zero fill */
+      const int sh = (int)((u[22] - 0.5) * (double)n * 0.25);
+      const int k = i - sh;
+      return (k >= 0 && k < n) ? (double)A[k] : 0.0;
+    }
+    case 8: {  /* splice a random block of B into A */
+      const int len = 1 + (int)(u[23] * 0.5 * (double)n);
+      const int st0 = (int)(u[24] * (double)(n - len > 0 ? n - len : 1));
+      return (i >= st0 && i < st0 + len) ? (double)B[i] : a;
+    }
+    case 9: {  /* random restart: smooth positive random function */
+      const int nf = 2 + (int)(u[25] * 6.0);
+      double m = 0.0;
+      for (int f = 0; f < nf; ++f) {
+        const double freq = 1.0 + u[4 + (f % 8)] * 10.0;
+        m += cos(twopi * freq * (double)i / (double)n + twopi * u[12 + (f % 8)]);
+      }
+      const double v = 0.3 + 0.7 * u[26] + m / (double)nf + 0.4 * normal_d(seed, gen, slot, i);
+      return v > 0.0 ? v : 0.0;
+    }
+    case 10: {  /* spike of width 2w+1 at a random location */
+      const int loc = (int)(u[27] * (double)n);
+      const int wdt = 1 + (int)(u[28] * (double)n * 0.03);
+      const double amp = mean * (1.0 + 6.0 * u[29]);
+      return (i >= loc - wdt && i <= loc + wdt) ? a + amp : a;
+    }
+    case 11: {  /* structured seed: plateau, depression, hump, band, gap, spike, atom */
+      const int flip = u[30] < 0.5;
+      const double atom = 30.0 + 250.0 * u[31];
+      const double pl1 = 0.28 + 0.16 * u[32];
+      const double pl2 = pl1 + 0.10 + 0.14 * u[33];
+      const double hump = pl2 + 0.02 + 0.08 * u[34];
+      const double gap0 = 0.70 + 0.16 * u[35];
+      const double gap1 = gap0 + 0.05 + 0.15 * u[36];
+      const double tail = 3.0 + 14.0 * u[37];
+      const int nf = 3 + (int)(u[38] * 6.0);
+      const int d = flip ? n - 1 - i : i;
+      const double x = ((double)d + 0.5) / (double)n;
+      double rough = 0.0;
+      for (int f = 0; f < nf; ++f)
+        rough += cos(twopi * (2.0 + u[40 + f] * 40.0) * x + twopi * u[48 + f]);
+      rough /= (double)nf;
+      double v;
+      if (x < pl1) v = 4.2 * (1.0 + 0.45 * rough);
+      else if (x < pl2) v = 0.7 * (1.0 + 0.80 * rough);
+      else if (x < hump) v = 5.2 * (1.0 + 0.30 * rough);
+      else if (x < gap0) v = 2.0 * (1.0 + 0.70 * rough);
+      else if (x < gap1) v = 0.0;
+      else v = tail * (1.0 + 0.20 * rough);
+      if (d == 0) v += atom;
+      else if (d < 4) v += atom * 0.10 / (double)d;
+      return v > 0.0 ? v : 0.0;
+    }
+    case 12: {  /* carve out (or fill in) a hard support gap */
+      const int len = 1 + (int)((0.015 + 0.13 * u[30]) * (double)n);
+      const int st0 = (int)(u[31] * (double)(n > len ? n - len : 1));
+      if (i < st0 || i >= st0 + len) return a;
+      return u[32] < 0.7 ? 0.0 : a + mean * (0.2 + 1.8 * u[33]);
+    }
+    case 13: {  /* grow or shrink the endpoint atom */
+      const int w = 1 + (int)(u[31] * 0.012 * (double)n);
+      const int d = u[30] < 0.5 ? i : n - 1 - i;
+      if (d >= w) return a;
+      const double x = a + mean * (-20.0 + 260.0 * u[32]) / (double)(1 + d);
+      return x > 0.0 ? x : 0.0;
+    }
+    default:   /* clone */
+      return a;
+  }
+}
+
+int so_op_uses_mean(int op) { return op == 4 || op == 5 || op == 10 || op == 12 || op == 13; }
+
+/* kernels.cuh:582-801: offspring = operator `op` on parents A (and B), each
+ * height formed in fp64 and rounded once to fp32, then renormalised to sum 1
+ * by the correctly rounded fp64 sum of the fp32 heights (uniform 1/n if that
+ * sum is not positive).  B = NULL means B = A. */
+void so_variation2(const float* A, const float* B, int n, int op, float sigma, uint64_t seed,
+                   uint64_t gen, int slot, float* out) {
+  if (!B) B = A;
+  float prm[64];
+  double u[64];
+  so_op_params(seed, gen, slot, prm);
+  for (int k = 0; k < 64; ++k) u[k] = prm[k];
   double* tmp = (double*)malloc(sizeof(double) * (size_t)n);
+  double mean = 0.0;
+  if (so_op_uses_mean(op)) {
+    for (int i = 0; i < n; ++i) tmp[i] = A[i];
+    mean = so_fsum(tmp, n) / (double)n;
+  }
+  for (int i = 0; i < n; ++i)
+    out[i] = (float)vary_one(op, i, n, A, B, u, mean, (double)sigma, seed, gen, slot);
   for (int i = 0; i < n; ++i) tmp[i] = out[i];
   const double S = so_fsum(tmp, n); /* correctly rounded sum of the fp32 heights */
   free(tmp);
@@ -589,6 +727,63 @@ void so_variation(const float* A, int n, int op, float sigma, uint64_t seed, uin
   } else {
     for (int i = 0; i < n; ++i) out[i] = 1.0f / (float)n;
   }
+}
+
+void so_variation(const float* A, int n, int op, float sigma, uint64_t seed, uint64_t gen,
+                  int slot, float* out) {
+  so_variation2(A, NULL, n, op, sigma, seed, gen, slot, out);
+}
+
+/* -------------------------------------------------- support signature ---- */
+
+/* kernels.cuh:803-849 (kSupportSig), BD = 256 (kernels.cuh:30), tol = 0.02
+ * as runSig passes it (main.cu:180-184).  The fp32 mean is summed in the
+ * kernel's order: thread t of 256 adds a[t], a[t+256], ... in sequence;
+ * blockSum (kernels.cuh:80-90) reduces each 32-lane warp by the shuffle tree
+ * v += v[lane + 16], +8, +4, +2, +1 and adds the 8 warp sums in order from 0.
+ * The support is {i : a_i > mean * tol}; the signature is the XOR over the
+ * support of mix(i) (x = i * 0x9E3779B97F4A7C15, x ^= x >> 29,
+ * x *= 0xBF58476D1CE4E5B9, x ^= x >> 32): every thread starts its hash at the
+ * FNV offset basis, and 256 copies of it cancel in the XOR reduction.  nnz is
+ * the support size. */
+static float warp_tree_sum32(const float* v) {
+  float w[32];
+  for (int l = 0; l < 32; ++l) w[l] = v[l];
+  for (int off = 16; off > 0; off >>= 1)
+    for (int l = 0; l < 32; ++l)  /* ascending l: w[l + off] is not yet updated */
+      w[l] = w[l] + (l + off < 32 ? w[l + off] : w[l]);
+  return w[0];
+}
+
+uint64_t so_sig_mix(int i) {
+  uint64_t x = (uint64_t)i * 0x9E3779B97F4A7C15ULL;
+  x ^= x >> 29;
+  x *= 0xBF58476D1CE4E5B9ULL;
+  x ^= x >> 32;
+  return x;
+}
+
+void so_support_sig(const float* a, int n, float tol, uint64_t* sig, int* nnz) {
+  float part[256], warp[8];
+  for (int t = 0; t < 256; ++t) {
+    float s = 0.0f;
+    for (int i = t; i < n; i += 256) s += a[i];
+    part[t] = s;
+  }
+  for (int w = 0; w < 8; ++w) warp[w] = warp_tree_sum32(part + 32 * w);
+  float tot = 0.0f;
+  for (int w = 0; w < 8; ++w) tot += warp[w];
+  const float mean = tot / (float)n;
+  const float thr = mean * tol;
+  uint64_t h = 0;
+  int c = 0;
+  for (int i = 0; i < n; ++i)
+    if (a[i] > thr) {
+      ++c;
+      h ^= so_sig_mix(i);
+    }
+  *sig = h;
+  *nnz = c;
 }
 
 /* ---------------------------------------------------------- selection ---- */

@@ -116,6 +116,17 @@ void so_variation(const float* A, int n, int op, float sigma, uint64_t seed, uin
                   int slot, float* out);
 /* The standard normals the variation stream uses for element i of slot j. */
 float so_normal(uint64_t seed, uint64_t gen, int slot, int i);
+/* All fourteen operators (kernels.cuh:614-787) on parents A, B (B = NULL:
+ * B = A), each height in fp64 rounded once to fp32, then the renormalisation
+ * above; operator parameters from so_op_params. */
+void so_variation2(const float* A, const float* B, int n, int op, float sigma, uint64_t seed,
+                   uint64_t gen, int slot, float* out);
+void so_op_params(uint64_t seed, uint64_t gen, int slot, float prm[64]);
+int so_op_uses_mean(int op);
+/* kSupportSig (kernels.cuh:803-849): support {a_i > mean * tol} with the
+ * kernel's fp32 mean, 64-bit XOR signature and support size. */
+void so_support_sig(const float* a, int n, float tol, uint64_t* sig, int* nnz);
+uint64_t so_sig_mix(int i);
 
 /* Island selection plan (main.cu:676-735 with the configs' fixed operator):
  * per island rank by fp64 fit (ties by index), ring migration every

@@ -33,7 +33,7 @@ EXPORTS = (
     "sidon_eval_exact", "sidon_local_search", "sidon_local_search_f64",
     "sidon_local_search_ragged", "sidon_set_ls_block", "sidon_get_ls_block", "sidon_lp_grad",
     "sidon_lp_grad_f64", "sidon_frank_wolfe", "sidon_refine", "sidon_variation",
-    "sidon_variation_ragged", "sidon_select", "sidon_advance_gen", "sidon_order_by_length",
+    "sidon_variation_ragged", "sidon_support_sig", "sidon_select", "sidon_advance_gen", "sidon_order_by_length",
     "sidon_generation_create", "sidon_generation_run", "sidon_generation_current",
     "sidon_generation_enqueue", "sidon_generation_destroy", "sidon_tune_local_search",
     "sidon_measure_fp32_ceilings",
@@ -98,6 +98,7 @@ def _declare(L):
         "sidon_frank_wolfe": ([_vp, _vp, _i, _i, _i, _i, _i, _f, _vp], _i),
         "sidon_refine": ([_vp, _i, _i, _vp, _vp], _i),
         "sidon_variation": ([_vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _u64, _u64, _vp], _i),
+        "sidon_support_sig": ([_vp, _i, _i, C.c_float, _vp, _vp, _vp], _i),
         "sidon_variation_ragged": ([_vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _u64, _vp,
                                     _vp], _i),
         "sidon_select": ([_vp, _vp, _vp, _i, _i, _i, _i, _i, _f, _u64, _vp, _vp, _vp, _vp, _vp,
@@ -305,6 +306,17 @@ def variation_ragged(pop, ns, out, ns_out, p1, p2, op, strength, seed: int, gen_
                                         out.shape[0], _ptr(p1), _ptr(p2), _ptr(op),
                                         _ptr(strength), seed, _ptr(gen_dev), _stream(stream)),
            "sidon_variation_ragged")
+
+
+def support_sig(pop: torch.Tensor, tol: float = 0.02, stream=None):
+    """(sig uint64 as int64, nnz int32) of each row: kSupportSig."""
+    _need_gpu()
+    P, n = pop.shape
+    sig = torch.empty(P, dtype=torch.int64, device=pop.device)
+    nnz = torch.empty(P, dtype=torch.int32, device=pop.device)
+    _check(lib().sidon_support_sig(_ptr(pop), n, P, tol, _ptr(sig), _ptr(nnz), _stream(stream)),
+           "sidon_support_sig")
+    return sig, nnz
 
 
 def select(fit, pop, ns, NI: int, elite: int, migrate_every: int, sigma: float, seed: int,

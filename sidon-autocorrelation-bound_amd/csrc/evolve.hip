@@ -2,8 +2,9 @@
 // selection and the small device-side helpers that let a whole generation run
 // without the host (and so be captured as one hipGraph).
 //
-// Reference: src/kernels.cuh:534-547 (kRefine), :582-801 (kVariation, op 2 and
-// the clone branch), src/main.cu:650-735 (host selection / migration).
+// Reference: src/kernels.cuh:534-547 (kRefine), :582-801 (kVariation, its
+// fourteen operators and the clone branch), :803-849 (kSupportSig),
+// src/main.cu:650-735 (host selection / migration).
 // The reference draws mutation noise from curand Philox4_32_10 and selection
 // from std::mt19937_64 on the host; here both come from the build's own
 // Philox4x32-10 (Salmon et al., SC'11) keyed by (seed, generation, slot), so a
@@ -19,6 +20,7 @@ namespace {
 
 constexpr uint32_t kStreamVariation = 0x56415249u;
 constexpr uint32_t kStreamSelect = 0x53454C45u;
+constexpr uint32_t kStreamParams = 0x50524D53u;
 
 __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
@@ -66,50 +68,149 @@ __global__ void k_refine(const float* __restrict__ src, int n, float* __restrict
 }
 
 // --------------------------------------------------------- variation -----
-// One block per offspring.  op 2: O = A exp(sigma z) (product in fp64,
-// rounded once), clamp >= 0; other ops clone.  Then O /= S with S the
-// double-double (practically correctly rounded) sum of the fp32 heights.
-template <int BD>
-__global__ __launch_bounds__(BD) void k_variation(const float* __restrict__ pop,
-                                                  const int* __restrict__ ns, float* __restrict__ out,
-                                                  int* __restrict__ ns_out, long long stride, int n,
-                                                  int nOff, const int* __restrict__ p1,
-                                                  const int* __restrict__ op,
-                                                  const float* __restrict__ strength,
-                                                  uint64_t seed, uint64_t gen_host,
-                                                  const unsigned long long* gen_dev) {
-  constexpr int NW = BD / kWave;
-  __shared__ double s_hi[NW], s_lo[NW];
-  const int j = blockIdx.x;
-  if (j >= nOff) return;
-  const uint64_t gen = gen_dev ? *gen_dev : gen_host;
-  const int par = p1[j];
-  const int len = ns ? ns[par] : n;
-  const float* A = pop + (size_t)par * stride;
-  float* O = out + (size_t)j * stride;
-  const int o = op[j];
-  const double sg = (double)strength[j];
-  double hi = 0.0, lo = 0.0;
-  for (int i = threadIdx.x; i < len; i += BD) {
-    float x;
-    if (o == 2) {
-      const double z = normal_at(seed, gen, j, i);
-      x = (float)((double)A[i] * exp(sg * z));
-      x = x > 0.0f ? x : 0.0f;
-    } else {
-      x = A[i];
+// The 64 operator parameters of offspring `slot`, in (0, 1] (the role of the
+// reference's sPar): Philox4x32-10 counters {q, slot, gen, gen>>32 ^ tag},
+// q = 0..15, four words each, u = ((x >> 8) + 1) 2^-24, exact in fp32.
+__device__ __forceinline__ void op_params(uint64_t seed, uint64_t gen, int slot, int q,
+                                          float* prm) {
+  uint32_t c[4] = {(uint32_t)q, (uint32_t)slot, (uint32_t)gen,
+                   (uint32_t)(gen >> 32) ^ kStreamParams};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+#pragma unroll
+  for (int w = 0; w < 4; ++w) prm[4 * q + w] = (float)((c[w] >> 8) + 1u) * (1.0f / 16777216.0f);
+}
+
+__device__ __forceinline__ bool op_uses_mean(int op) {
+  return op == 4 || op == 5 || op == 10 || op == 12 || op == 13;
+}
+
+// One height of operator `op` (kernels.cuh:614-787) in fp64, rounded once to
+// fp32 by the caller: A the first parent, B the second (crossover ops 0, 1,
+// 8), u the parameters, mean = sum(A) / n, sg the strength.  No contraction:
+// every product and sum rounds on its own, so the host oracle reproduces the
+// value operation for operation (library cos / exp / log aside).
+#pragma clang fp contract(off)
+__device__ double vary(int op, int i, int n, const float* A, const float* B, const float* prm,
+                       double mean, double sg, uint64_t seed, uint64_t gen, int slot) {
+  const double a = A[i];
+  const double twopi = 6.283185307179586476925286766559;
+  auto u = [prm](int k) { return (double)prm[k]; };
+  switch (op) {
+    case 0: {  // blend crossover, lambda in [-0.3, 1.3]
+      const double lam = -0.3 + 1.6 * u(0);
+      const double x = lam * a + (1.0 - lam) * (double)B[i];
+      return x > 0.0 ? x : 0.0;
     }
-    O[i] = x;
-    double s, e;
-    two_sum(hi, (double)x, s, e);
-    hi = s;
-    lo += e;
+    case 1: {  // two-point segment crossover
+      int c1 = (int)(u(1) * n), c2 = (int)(u(2) * n);
+      if (c1 > c2) { const int t = c1; c1 = c2; c2 = t; }
+      return (i >= c1 && i < c2) ? (double)B[i] : a;
+    }
+    case 2: {  // log-normal multiplicative noise
+      const double x = a * exp(sg * normal_at(seed, gen, slot, i));
+      return x > 0.0 ? x : 0.0;
+    }
+    case 3: {  // smooth low-frequency perturbation
+      const int nf = 1 + (int)(u(3) * 5.0);
+      double m = 0.0;
+      for (int f = 0; f < nf; ++f) {
+        const double freq = 1.0 + u(4 + f) * 12.0;
+        m += cos(twopi * freq * (double)i / (double)n + twopi * u(12 + f)) / (double)nf;
+      }
+      const double x = a * (1.0 + sg * 3.0 * m);
+      return x > 0.0 ? x : 0.0;
+    }
+    case 4: {  // additive Gaussian noise scaled by the mean height
+      const double x = a + sg * mean * normal_at(seed, gen, slot, i);
+      return x > 0.0 ? x : 0.0;
+    }
+    case 5: {  // sparsify below a fraction of the mean
+      const double thr = mean * (0.05 + 0.9 * u(20));
+      return a < thr ? 0.0 : a;
+    }
+    case 6: {  // symmetrise, or hard reflect
+      const double r = A[n - 1 - i];
+      return u(21) < 0.5 ? r : 0.5 * (a + r);
+    }
+    case 7: {  // shift by up to n/8 either way, zero fill
+      const int sh = (int)((u(22) - 0.5) * (double)n * 0.25);
+      const int k = i - sh;
+      return (k >= 0 && k < n) ? (double)A[k] : 0.0;
+    }
+    case 8: {  // splice a random block of B into A
+      const int len = 1 + (int)(u(23) * 0.5 * (double)n);
+      const int st0 = (int)(u(24) * (double)(n - len > 0 ? n - len : 1));
+      return (i >= st0 && i < st0 + len) ? (double)B[i] : a;
+    }
+    case 9: {  // random restart: smooth positive random function
+      const int nf = 2 + (int)(u(25) * 6.0);
+      double m = 0.0;
+      for (int f = 0; f < nf; ++f) {
+        const double freq = 1.0 + u(4 + (f % 8)) * 10.0;
+        m += cos(twopi * freq * (double)i / (double)n + twopi * u(12 + (f % 8)));
+      }
+      const double v = 0.3 + 0.7 * u(26) + m / (double)nf + 0.4 * normal_at(seed, gen, slot, i);
+      return v > 0.0 ? v : 0.0;
+    }
+    case 10: {  // spike of width 2w+1 at a random location
+      const int loc = (int)(u(27) * (double)n);
+      const int wdt = 1 + (int)(u(28) * (double)n * 0.03);
+      const double amp = mean * (1.0 + 6.0 * u(29));
+      return (i >= loc - wdt && i <= loc + wdt) ? a + amp : a;
+    }
+    case 11: {  // structured seed: plateau, depression, hump, band, gap, spike, atom
+      const bool flip = u(30) < 0.5;
+      const double atom = 30.0 + 250.0 * u(31);
+      const double pl1 = 0.28 + 0.16 * u(32);
+      const double pl2 = pl1 + 0.10 + 0.14 * u(33);
+      const double hump = pl2 + 0.02 + 0.08 * u(34);
+      const double gap0 = 0.70 + 0.16 * u(35);
+      const double gap1 = gap0 + 0.05 + 0.15 * u(36);
+      const double tail = 3.0 + 14.0 * u(37);
+      const int nf = 3 + (int)(u(38) * 6.0);
+      const int d = flip ? n - 1 - i : i;
+      const double x = ((double)d + 0.5) / (double)n;
+      double rough = 0.0;
+      for (int f = 0; f < nf; ++f)
+        rough += cos(twopi * (2.0 + u(40 + f) * 40.0) * x + twopi * u(48 + f));
+      rough /= (double)nf;
+      double v;
+      if (x < pl1) v = 4.2 * (1.0 + 0.45 * rough);
+      else if (x < pl2) v = 0.7 * (1.0 + 0.80 * rough);
+      else if (x < hump) v = 5.2 * (1.0 + 0.30 * rough);
+      else if (x < gap0) v = 2.0 * (1.0 + 0.70 * rough);
+      else if (x < gap1) v = 0.0;
+      else v = tail * (1.0 + 0.20 * rough);
+      if (d == 0) v += atom;
+      else if (d < 4) v += atom * 0.10 / (double)d;
+      return v > 0.0 ? v : 0.0;
+    }
+    case 12: {  // carve out (or fill in) a hard support gap
+      const int len = 1 + (int)((0.015 + 0.13 * u(30)) * (double)n);
+      const int st0 = (int)(u(31) * (double)(n > len ? n - len : 1));
+      if (i < st0 || i >= st0 + len) return a;
+      return u(32) < 0.7 ? 0.0 : a + mean * (0.2 + 1.8 * u(33));
+    }
+    case 13: {  // grow or shrink the endpoint atom
+      const int w = 1 + (int)(u(31) * 0.012 * (double)n);
+      const int d = u(30) < 0.5 ? i : n - 1 - i;
+      if (d >= w) return a;
+      const double x = a + mean * (-20.0 + 260.0 * u(32)) / (double)(1 + d);
+      return x > 0.0 ? x : 0.0;
+    }
+    default:  // clone
+      return a;
   }
-  // deterministic double-double reduction: lanes by butterfly, waves in order
+}
+
+// Deterministic double-double block sum of per-thread (hi, lo): lanes by
+// butterfly (symmetric in self and partner, so every lane holds the same
+// pair), waves in order.  s_hi / s_lo are free again on return.
+template <int NW>
+__device__ double block_dd_sum(double hi, double lo, double* s_hi, double* s_lo) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     const double oh = __shfl_xor(hi, off, kWave), ol = __shfl_xor(lo, off, kWave);
-    // combine symmetric in (self, other) so every lane gets the same pair
     const double a = hi < oh ? hi : oh, b = hi < oh ? oh : hi;
     double s, e;
     two_sum(b, a, s, e);
@@ -126,7 +227,65 @@ __global__ __launch_bounds__(BD) void k_variation(const float* __restrict__ pop,
     H = s;
     L = (L + s_lo[q]) + e;
   }
-  const double S = H + L;
+  __syncthreads();
+  return H + L;
+}
+#pragma clang fp contract(on)
+
+// One block per offspring: O = operator op[j] on parents p1[j] (and p2[j]),
+// each height in fp64 rounded once to fp32, then O /= S with S the
+// double-double (practically correctly rounded) sum of the fp32 heights.  In
+// the uneven-length form the offspring keeps p1's length, and a second parent
+// of another length is replaced by the first.
+template <int BD>
+__global__ __launch_bounds__(BD) void k_variation(const float* __restrict__ pop,
+                                                  const int* __restrict__ ns, float* __restrict__ out,
+                                                  int* __restrict__ ns_out, long long stride, int n,
+                                                  int nOff, const int* __restrict__ p1,
+                                                  const int* __restrict__ p2,
+                                                  const int* __restrict__ op,
+                                                  const float* __restrict__ strength,
+                                                  uint64_t seed, uint64_t gen_host,
+                                                  const unsigned long long* gen_dev) {
+  constexpr int NW = BD / kWave;
+  __shared__ double s_hi[NW], s_lo[NW];
+  __shared__ float prm[64];
+  const int j = blockIdx.x;
+  if (j >= nOff) return;
+  const uint64_t gen = gen_dev ? *gen_dev : gen_host;
+  const int par = p1[j];
+  const int len = ns ? ns[par] : n;
+  const float* A = pop + (size_t)par * stride;
+  const int pb = p2 ? p2[j] : par;
+  const float* B = (ns && ns[pb] != len) ? A : pop + (size_t)pb * stride;
+  float* O = out + (size_t)j * stride;
+  const int o = op[j];
+  const double sg = (double)strength[j];
+  if (o >= 0 && o <= 13) {
+    if (threadIdx.x < 16) op_params(seed, gen, j, threadIdx.x, prm);
+    __syncthreads();
+  }
+  double mean = 0.0;
+  if (op_uses_mean(o)) {
+    double hi = 0.0, lo = 0.0;
+    for (int i = threadIdx.x; i < len; i += BD) {
+      double s, e;
+      two_sum(hi, (double)A[i], s, e);
+      hi = s;
+      lo += e;
+    }
+    mean = block_dd_sum<NW>(hi, lo, s_hi, s_lo) / (double)len;
+  }
+  double hi = 0.0, lo = 0.0;
+  for (int i = threadIdx.x; i < len; i += BD) {
+    const float x = (float)vary(o, i, len, A, B, prm, mean, sg, seed, gen, j);
+    O[i] = x;
+    double s, e;
+    two_sum(hi, (double)x, s, e);
+    hi = s;
+    lo += e;
+  }
+  const double S = block_dd_sum<NW>(hi, lo, s_hi, s_lo);
   if (S > 0.0) {
     const double inv = 1.0 / S;
     for (int i = threadIdx.x; i < len; i += BD) O[i] = (float)((double)O[i] * inv);
@@ -134,6 +293,64 @@ __global__ __launch_bounds__(BD) void k_variation(const float* __restrict__ pop,
     for (int i = threadIdx.x; i < len; i += BD) O[i] = 1.0f / (float)len;
   }
   if (ns_out && threadIdx.x == 0) ns_out[j] = len;
+}
+
+// ------------------------------------------------- support signature -----
+// kSupportSig (kernels.cuh:803-849) with its block of 256 threads and 32-lane
+// warps (kernels.cuh:30, :80-90): thread t sums a[t], a[t+256], ... in fp32;
+// each 32-lane half of a wave reduces by the shuffle tree v += v[lane + 16],
+// +8, +4, +2, +1 (the reference's warp), and thread 0 adds the 8 warp sums in
+// order, so the fp32 mean, the threshold mean * tol and every a_i > thr
+// decision are the reference's.  The signature is the XOR of mix(i) over the
+// support (the per-thread FNV start values cancel in pairs), nnz its size.
+__device__ __forceinline__ unsigned long long sig_mix(int i) {
+  unsigned long long x = (unsigned long long)i * 0x9E3779B97F4A7C15ull;
+  x ^= x >> 29;
+  x *= 0xBF58476D1CE4E5B9ull;
+  x ^= x >> 32;
+  return x;
+}
+
+__global__ __launch_bounds__(256) void k_support_sig(const float* __restrict__ pop, int n,
+                                                     float tol, unsigned long long* __restrict__ sig,
+                                                     int* __restrict__ nnz) {
+  __shared__ float s_warp[8];
+  __shared__ float s_thr;
+  __shared__ unsigned long long s_h[4];
+  __shared__ int s_c[4];
+  const float* a = pop + (size_t)blockIdx.x * n;
+  const int t = threadIdx.x;
+  float v = 0.0f;
+  for (int i = t; i < n; i += 256) v += a[i];
+#pragma unroll
+  for (int off = 16; off > 0; off >>= 1) v += __shfl_down(v, off, 32);
+  if ((t & 31) == 0) s_warp[t >> 5] = v;
+  __syncthreads();
+  if (t == 0) {
+    float tot = 0.0f;
+    for (int w = 0; w < 8; ++w) tot += s_warp[w];
+    s_thr = (tot / (float)n) * tol;
+  }
+  __syncthreads();
+  const float thr = s_thr;
+  unsigned long long h = 0;
+  int c = 0;
+  for (int i = t; i < n; i += 256)
+    if (a[i] > thr) {
+      ++c;
+      h ^= sig_mix(i);
+    }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    h ^= __shfl_xor(h, off, kWave);
+    c += __shfl_xor(c, off, kWave);
+  }
+  if ((t & (kWave - 1)) == 0) { s_h[t / kWave] = h; s_c[t / kWave] = c; }
+  __syncthreads();
+  if (t == 0) {
+    sig[blockIdx.x] = s_h[0] ^ s_h[1] ^ s_h[2] ^ s_h[3];
+    nnz[blockIdx.x] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+  }
 }
 
 // --------------------------------------------------------- selection -----
@@ -232,12 +449,11 @@ extern "C" int sidon_variation(const float* pop, float* out, int n, int nOff, co
                                const int* p2, const int* op, const float* strength,
                                unsigned long long seed, unsigned long long gen,
                                hipStream_t stream) {
-  (void)p2;  // second parent: used only by the crossover operators (not built)
   if (!pop || !out || !p1 || !op || !strength || n < 1 || nOff < 0) return SIDON_ERR_ARG;
   if (nOff == 0) return SIDON_OK;
   hipLaunchKernelGGL(k_variation<256>, dim3(nOff), dim3(256), 0, stream, pop, (const int*)nullptr,
-                     out, (int*)nullptr, (long long)n, n, nOff, p1, op, strength, (uint64_t)seed,
-                     (uint64_t)gen, (const unsigned long long*)nullptr);
+                     out, (int*)nullptr, (long long)n, n, nOff, p1, p2, op, strength,
+                     (uint64_t)seed, (uint64_t)gen, (const unsigned long long*)nullptr);
   SIDON_HIP_TRY(hipGetLastError());
   return SIDON_OK;
 }
@@ -247,13 +463,21 @@ extern "C" int sidon_variation_ragged(const float* pop, const int* ns, float* ou
                                       const int* op, const float* strength,
                                       unsigned long long seed,
                                       const unsigned long long* gen_dev, hipStream_t stream) {
-  (void)p2;
   if (!pop || !ns || !out || !p1 || !op || !strength || !gen_dev || stride < 1 || nOff < 0)
     return SIDON_ERR_ARG;
   if (nOff == 0) return SIDON_OK;
   hipLaunchKernelGGL(k_variation<256>, dim3(nOff), dim3(256), 0, stream, pop, ns, out, ns_out,
-                     (long long)stride, stride, nOff, p1, op, strength, (uint64_t)seed,
+                     (long long)stride, stride, nOff, p1, p2, op, strength, (uint64_t)seed,
                      (uint64_t)0, gen_dev);
+  SIDON_HIP_TRY(hipGetLastError());
+  return SIDON_OK;
+}
+
+extern "C" int sidon_support_sig(const float* pop, int n, int P, float tol,
+                                 unsigned long long* sig, int* nnz, hipStream_t stream) {
+  if (!pop || !sig || !nnz || n < 1 || P < 0) return SIDON_ERR_ARG;
+  if (P == 0) return SIDON_OK;
+  hipLaunchKernelGGL(k_support_sig, dim3(P), dim3(256), 0, stream, pop, n, tol, sig, nnz);
   SIDON_HIP_TRY(hipGetLastError());
   return SIDON_OK;
 }

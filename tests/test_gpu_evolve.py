@@ -33,6 +33,84 @@ def test_variation_matches_oracle(S, O, W):
         assert d.max() <= 1 and (d > 0).mean() < 1e-3
 
 
+# operators with a library exp / log / cos in them may differ by an ulp
+TRANSCENDENTAL = {2, 3, 4, 9, 11}
+
+
+@pytest.mark.parametrize("n", [1200, 8400])
+def test_all_operators_match_oracle(S, O, W, n):
+    """Each of the fourteen operators (and clone) on two parents, GPU vs the
+    oracle: bit for bit where only + - * / and comparisons are involved."""
+    print(f"[evolve] operators n={n}", flush=True)
+    ops = list(range(14)) + [20]
+    reps = 3
+    P = len(ops) * reps
+    pop = W.structured_prior(P, n, seed=8).astype(np.float32)
+    rng = np.random.default_rng(3)
+    p1 = rng.integers(0, P, size=P).astype(np.int32)
+    p2 = rng.integers(0, P, size=P).astype(np.int32)
+    op = np.repeat(np.array(ops, np.int32), reps)
+    st = rng.uniform(0.03, 0.38, size=P).astype(np.float32)
+    out = torch.zeros(P, n, dtype=torch.float32, device="cuda")
+    S.variation(dev(pop), out, dev(p1, torch.int32), dev(p2, torch.int32), dev(op, torch.int32),
+                dev(st), 20260905, 11)
+    got = out.cpu().numpy()
+    for j in range(P):
+        want = O.variation2(pop[p1[j]], pop[p2[j]], int(op[j]), float(st[j]), 20260905, 11, j)
+        if op[j] in TRANSCENDENTAL:
+            d = ulp_diff32(got[j], want)
+            assert d.max() <= 1 and (d > 0).mean() < 1e-3, (op[j], j)
+        else:
+            np.testing.assert_array_equal(got[j], want, err_msg=f"op {op[j]} slot {j}")
+
+
+def test_variation_ragged_crossover_lengths(S, O):
+    """Uneven lengths: the offspring keeps p1's length; a second parent of
+    another length is replaced by the first."""
+    print("[evolve] ragged crossover", flush=True)
+    rng = np.random.default_rng(9)
+    ns = np.array([600, 600, 900, 600], np.int32)
+    stride = 1000
+    pop = np.zeros((4, stride), np.float32)
+    for p, n in enumerate(ns):
+        pop[p, :n] = rng.uniform(0.5, 1.5, size=n)
+    p1 = np.array([0, 0, 2, 3, 1, 2], np.int32)
+    p2 = np.array([1, 2, 0, 1, 3, 2], np.int32)
+    op = np.array([0, 0, 1, 8, 8, 0], np.int32)
+    st = np.full(6, 0.1, np.float32)
+    out = torch.zeros(6, stride, dtype=torch.float32, device="cuda")
+    ns_out = torch.zeros(6, dtype=torch.int32, device="cuda")
+    gd = dev(np.array([5], np.uint64).view(np.int64), torch.int64)
+    S.variation_ragged(dev(pop), dev(ns, torch.int32), out, ns_out, dev(p1, torch.int32),
+                       dev(p2, torch.int32), dev(op, torch.int32), dev(st), 20260905, gd)
+    got, gl = out.cpu().numpy(), ns_out.cpu().numpy()
+    for j in range(6):
+        n = ns[p1[j]]
+        assert gl[j] == n
+        B = pop[p2[j], :n] if ns[p2[j]] == n else None
+        want = O.variation2(pop[p1[j], :n], B, int(op[j]), 0.1, 20260905, 5, j)
+        np.testing.assert_array_equal(got[j, :n], want, err_msg=f"slot {j}")
+
+
+@pytest.mark.parametrize("n", [1, 17, 256, 257, 1200, 8400])
+def test_support_sig_matches_oracle(S, O, W, n):
+    """kSupportSig's signature and support size, bit for bit (the fp32 mean in
+    the reference kernel's summation order decides the support)."""
+    print(f"[evolve] support signature n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    rows = [rng.uniform(size=n), W.structured_prior(1, n, seed=n)[0],
+            np.where(rng.uniform(size=n) < 0.4, 0.0, rng.lognormal(0, 2, size=n)),
+            np.zeros(n), np.ones(n)]
+    pop = np.stack(rows).astype(np.float32)
+    for tol in (0.02, 0.5):
+        sig, nnz = S.support_sig(dev(pop), tol)
+        sig = sig.cpu().numpy().view(np.uint64)
+        nnz = nnz.cpu().numpy()
+        for p in range(pop.shape[0]):
+            ws, wn = O.support_sig(pop[p], tol)
+            assert int(sig[p]) == ws and int(nnz[p]) == wn, (p, tol)
+
+
 def test_select_matches_oracle_and_migrates(S, O, W):
     print("[evolve] select", flush=True)
     P, NI, n = 512, 8, 64
