@@ -259,6 +259,59 @@ int sidon_ls_block_steps(unsigned long long* steps, int reset);
 int sidon_tune_local_search(float* scratch, const float* reference, int n, int P, int iters,
                             int repeats, int* best_block, float* best_ms, hipStream_t stream);
 
+/* ---- island search policy (host only) ---------------------------------- */
+
+/* The island search policy of the reference's driver (src/main.cu:461-472,
+ * :598-609, :650-769) as a host-side object: it decides, the driver moves the
+ * vectors.  No function here touches the device; all arrays are HOST arrays
+ * of length P.  Operators 0..13 are sidon_variation's, 20 = clone. */
+#define SIDON_POLICY_OPS 14
+typedef struct sidon_policy sidon_policy;
+
+/* P candidates in `islands` equal islands, `elite` clones per island;
+ * random draws from a std::mt19937_64 seeded with `seed`.  Operator weights
+ * start at 1 except the structured seed, gap and atom (2.5, 1.8, 1.8). */
+int sidon_policy_create(int P, int islands, int elite, unsigned long long seed,
+                        sidon_policy** out);
+void sidon_policy_destroy(sidon_policy* pol);
+/* Starts a ladder level: the level best and the generations-since-improvement
+ * count restart. */
+int sidon_policy_new_level(sidon_policy* pol);
+/* One generation's fp64 fitness and support signatures: tracks the best
+ * (*improved = 1 when it beats the incumbent by more than 1e-13, *best_slot
+ * = first minimum) and the cell tables (a signature seen again without
+ * improving its cell's best by 1e-12 gains a tabu hit). */
+int sidon_policy_observe(sidon_policy* pol, const double* fit, const unsigned long long* sig,
+                         int* best_slot, int* improved);
+/* The offspring plan of generation `gen` (counted from 0 within a level):
+ * islands ranked by (fit, slot); when gen > 0 and gen % 12 == 0 the best of
+ * island i replaces the worst of island i+1 (mig_src[k] -> mig_dst[k],
+ * k < *n_mig <= islands: the caller copies those vectors before the
+ * variation); the first `elite` slots of each island clone its ranked elites
+ * (op 20, strength 0); every other slot draws two binary tournaments in its
+ * island, an operator from the weights, a diversifying operator instead when
+ * the first parent's cell has more than 6 tabu hits and its best is worse
+ * than best*(1+2e-4), a restart (op 9) with probability 0.1 once stagnant
+ * (more than 17 generations without a level improvement), and strength
+ * (0.03 + 0.35 U)(0.5 + min(1, gen/40))(1 + 1.5 min(1, stagnant/25)).
+ * *reseed_slot (may be NULL) = the slot where the caller puts the incumbent
+ * back (gen % 10 == 9), else -1. */
+int sidon_policy_plan(sidon_policy* pol, int gen, int* p1, int* p2, int* op, float* strength,
+                      int* mig_src, int* mig_dst, int* n_mig, int* reseed_slot);
+/* Credit assignment (the caller runs it every 5th generation, gen > 0): an
+ * operator wins when its child's fitness beats its first parent's; weights
+ * move to 0.85 w + 0.15 (0.15 + 3 win rate) (rate 0.25 below 20 uses),
+ * clamped to [0.05, 4]; use and win counts decay by 0.7. */
+int sidon_policy_credit(sidon_policy* pol, const double* child_fit);
+/* Operator weights (SIDON_POLICY_OPS), incumbent and level best, cell count,
+ * generations since the level improved; any output may be NULL. */
+int sidon_policy_stats(const sidon_policy* pol, double* weights, double* best_f,
+                       double* level_best, int* cells, int* since_improved);
+/* Anneal tier of each candidate from the operator that made it: 0 cold
+ * (restart or structured seed, at most 18 % of P), 1 medium (structure
+ * changed: ops 5, 7, 8, 9, 10, 11, 12, 13), 2 warm (everything else). */
+int sidon_policy_tiers(const int* cur_op, int P, int* tier, int* n_cold, int* n_med);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,6 +38,9 @@ EXPORTS = (
     "sidon_generation_enqueue", "sidon_generation_destroy", "sidon_tune_local_search",
     "sidon_measure_fp32_ceilings",
     "sidon_ls_block_steps",
+    "sidon_policy_create", "sidon_policy_destroy", "sidon_policy_new_level",
+    "sidon_policy_observe", "sidon_policy_plan", "sidon_policy_credit", "sidon_policy_stats",
+    "sidon_policy_tiers",
 )
 
 
@@ -99,6 +102,14 @@ def _declare(L):
         "sidon_refine": ([_vp, _i, _i, _vp, _vp], _i),
         "sidon_variation": ([_vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _u64, _u64, _vp], _i),
         "sidon_support_sig": ([_vp, _i, _i, C.c_float, _vp, _vp, _vp], _i),
+        "sidon_policy_create": ([_i, _i, _i, _u64, _vp], _i),
+        "sidon_policy_destroy": ([_vp], None),
+        "sidon_policy_new_level": ([_vp], _i),
+        "sidon_policy_observe": ([_vp, _vp, _vp, _vp, _vp], _i),
+        "sidon_policy_plan": ([_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+        "sidon_policy_credit": ([_vp, _vp], _i),
+        "sidon_policy_stats": ([_vp, _vp, _vp, _vp, _vp, _vp], _i),
+        "sidon_policy_tiers": ([_vp, _i, _vp, _vp, _vp], _i),
         "sidon_variation_ragged": ([_vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _u64, _vp,
                                     _vp], _i),
         "sidon_select": ([_vp, _vp, _vp, _i, _i, _i, _i, _i, _f, _u64, _vp, _vp, _vp, _vp, _vp,
@@ -317,6 +328,79 @@ def support_sig(pop: torch.Tensor, tol: float = 0.02, stream=None):
     _check(lib().sidon_support_sig(_ptr(pop), n, P, tol, _ptr(sig), _ptr(nnz), _stream(stream)),
            "sidon_support_sig")
     return sig, nnz
+
+
+class Policy:
+    """The island search policy (sidon_policy_*): host-only, no device needed.
+    Arrays are numpy (host) arrays of length P."""
+
+    OPS = 14
+
+    def __init__(self, P: int, islands: int, elite: int, seed: int):
+        import numpy as np
+        self._np = np
+        self.P = P
+        h = C.c_void_p()
+        _check(lib().sidon_policy_create(P, islands, elite, seed, C.byref(h)),
+               "sidon_policy_create")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().sidon_policy_destroy(h)
+            self._h = None
+
+    def new_level(self) -> None:
+        _check(lib().sidon_policy_new_level(self._h), "sidon_policy_new_level")
+
+    def observe(self, fit, sig):
+        np = self._np
+        fit = np.ascontiguousarray(fit, np.float64)
+        sig = np.ascontiguousarray(sig, np.uint64)
+        bi, imp = C.c_int(), C.c_int()
+        _check(lib().sidon_policy_observe(self._h, fit.ctypes.data, sig.ctypes.data,
+                                          C.byref(bi), C.byref(imp)), "sidon_policy_observe")
+        return bi.value, bool(imp.value)
+
+    def plan(self, gen: int) -> dict:
+        np = self._np
+        P = self.P
+        p1, p2, op = (np.zeros(P, np.int32) for _ in range(3))
+        st = np.zeros(P, np.float32)
+        ms, md = np.zeros(P, np.int32), np.zeros(P, np.int32)
+        nm, rs = C.c_int(), C.c_int()
+        _check(lib().sidon_policy_plan(self._h, gen, p1.ctypes.data, p2.ctypes.data,
+                                       op.ctypes.data, st.ctypes.data, ms.ctypes.data,
+                                       md.ctypes.data, C.byref(nm), C.byref(rs)),
+               "sidon_policy_plan")
+        return dict(p1=p1, p2=p2, op=op, strength=st, mig_src=ms[:nm.value],
+                    mig_dst=md[:nm.value], reseed=rs.value)
+
+    def credit(self, child_fit) -> None:
+        np = self._np
+        cf = np.ascontiguousarray(child_fit, np.float64)
+        _check(lib().sidon_policy_credit(self._h, cf.ctypes.data), "sidon_policy_credit")
+
+    def stats(self) -> dict:
+        np = self._np
+        w = np.zeros(self.OPS)
+        bf, lb = C.c_double(), C.c_double()
+        cells, since = C.c_int(), C.c_int()
+        _check(lib().sidon_policy_stats(self._h, w.ctypes.data, C.byref(bf), C.byref(lb),
+                                        C.byref(cells), C.byref(since)), "sidon_policy_stats")
+        return dict(weights=w, best=bf.value, level_best=lb.value, cells=cells.value,
+                    since_improved=since.value)
+
+    @staticmethod
+    def tiers(cur_op):
+        import numpy as np
+        cur = np.ascontiguousarray(cur_op, np.int32)
+        tier = np.zeros(cur.size, np.int32)
+        nc, nm = C.c_int(), C.c_int()
+        _check(lib().sidon_policy_tiers(cur.ctypes.data, cur.size, tier.ctypes.data,
+                                        C.byref(nc), C.byref(nm)), "sidon_policy_tiers")
+        return tier, nc.value, nm.value
 
 
 def select(fit, pop, ns, NI: int, elite: int, migrate_every: int, sigma: float, seed: int,

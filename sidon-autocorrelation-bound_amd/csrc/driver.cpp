@@ -5,7 +5,11 @@
 //   --selftest               known-answer tests of src/main.cu:186-291 against the device
 //   --uneven                 island generations of uneven lengths as HIP graphs
 //   --ladder                 Frank-Wolfe (Chambolle-Pock) rounds + n -> 2n cell split
-//   --search                 ladder search: generations per level, split, best vector out
+//   --search                 the reference's search loop: three-tier local search, fp64
+//                            evaluation, support signatures, the island policy (tabu,
+//                            operator weights, migration), all fourteen operators,
+//                            n -> 2n split per level, best vector out
+//   --search-graph           ladder search on the device generation graph (operator 2 only)
 // Every mode prints one JSON line per result.  All device work goes through
 // the C ABI; host code here only seeds inputs, reads results and writes files.
 #include <hip/hip_runtime_api.h>
@@ -125,6 +129,8 @@ struct Args {
   std::string mode;
   int P = 512, islands = 8, iters = 200, gens = 3, elite = 2;
   int fw_rounds = 50, cp_iters = 2000, nT = 12;
+  int cold_iters = 1000, search_fw = 0, fw_iters = 500;
+  double pmax = 2e4;
   float sigma = 0.05f, t_max = 0.30f;
   double seconds = 60.0;
   unsigned long long seed = 20260905ULL;
@@ -399,8 +405,176 @@ int ladder(const Args& A) {
   return 0;
 }
 
-// ---------------------------------------------------------------- search --
+// --------------------------------------------------------- policy search --
+// The reference's search loop (src/main.cu:445-451 level budgets, :586-769 the
+// generation) over the C ABI: each generation runs the three anneal tiers of
+// the local search (cold / medium / warm by the operator that made each
+// candidate; every fourth island uses mirror descent), optional Frank-Wolfe
+// rounds, the fp64 evaluation and support signatures; the host policy then
+// plans the offspring (tabu, operator weights, tournaments, migration), the
+// variation applies it, the incumbent is re-seeded every 10 generations and
+// the operator weights are credited every 5.  Levels get 1.85^level shares of
+// the time budget (the reference keeps 18 % for an fp64 polish that is not
+// built here, so the levels get all of it); between levels every candidate is
+// split n -> 2n (F preserved exactly).
 int search(const Args& A) {
+  const std::vector<int> L = A.ladder.empty() ? std::vector<int>{256, 512, 1024, 2048} : A.ladder;
+  const int P = A.P, NI = A.islands, nmax = *std::max_element(L.begin(), L.end());
+  if (P % NI != 0) { std::fprintf(stderr, "P must be divisible by islands\n"); return 2; }
+  const int PI = P / NI;
+  std::mt19937_64 rng(A.seed);
+  hipStream_t s;
+  CHECK_HIP(hipStreamCreate(&s));  // blocking: orders with the null-stream copies below
+  float* pop = dalloc<float>((size_t)P * nmax);
+  float* nxt = dalloc<float>((size_t)P * nmax);
+  double* fit = dalloc<double>(P);
+  int* nact = dalloc<int>(P);
+  unsigned long long* sig = dalloc<unsigned long long>(P);
+  int* nnz = dalloc<int>(P);
+  int* dp1 = dalloc<int>(P); int* dp2 = dalloc<int>(P); int* dop = dalloc<int>(P);
+  float* dst = dalloc<float>(P);
+  int* skip = dalloc<int>(P);
+  int* mode = dalloc<int>(P);
+  std::vector<int> hmode(P);
+  for (int p = 0; p < P; ++p) hmode[p] = (p / PI) % 4 == 3 ? 1 : 0;  // mirror descent island
+  CHECK_HIP(hipMemcpy(mode, hmode.data(), P * sizeof(int), hipMemcpyHostToDevice));
+  {
+    std::vector<float> h((size_t)P * L[0]);
+    for (int p = 0; p < P; ++p) structured_prior(rng, L[0], &h[(size_t)p * L[0]]);
+    CHECK_HIP(hipMemcpy(pop, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  sidon_policy* pol = nullptr;
+  CHECK_SIDON(sidon_policy_create(P, NI, A.elite, A.seed, &pol));
+  std::vector<int> curOp(P, 11);  // fresh structured seeds: cold / medium tiers first
+  std::vector<double> hfit(P), cfit(P);
+  std::vector<unsigned long long> hsig(P);
+  std::vector<int> hnnz(P), hnact(P), p1(P), p2(P), op(P), tier(P), hskip(P), msrc(NI), mdst(NI);
+  std::vector<float> hst(P);
+  std::vector<double> lvlBudget(L.size());
+  double tot = 0.0;
+  for (size_t l = 0; l < L.size(); ++l) tot += (lvlBudget[l] = std::pow(1.85, (double)l));
+  for (auto& x : lvlBudget) x = x / tot * A.seconds;
+  const auto t0 = Clock::now();
+  double bestF = 1e30;
+  std::vector<double> bestA;
+  int bestN = 0;
+  long long lsSteps = 0, evals = 0;
+  for (size_t lvl = 0; lvl < L.size(); ++lvl) {
+    const int n = L[lvl];
+    CHECK_SIDON(sidon_policy_new_level(pol));
+    const auto tl = Clock::now();
+    int gen = 0;
+    while (gen < A.gens || seconds_since(tl) < lvlBudget[lvl]) {
+      // ---- three anneal tiers (main.cu:598-629) ----
+      int nCold = 0, nMed = 0;
+      CHECK_SIDON(sidon_policy_tiers(curOp.data(), P, tier.data(), &nCold, &nMed));
+      const float pTop = (float)A.pmax;
+      auto tier_run = [&](int t, int iters, float pa, float pb, float la, float lb) {
+        for (int p = 0; p < P; ++p) hskip[p] = tier[p] != t;
+        CHECK_HIP(hipMemcpy(skip, hskip.data(), P * sizeof(int), hipMemcpyHostToDevice));
+        CHECK_SIDON(sidon_local_search(pop, nullptr, nullptr, skip, mode, n, P, iters, pa, pb,
+                                       la, lb, 0.9f, 0.999f, 0, s));
+        for (int p = 0; p < P; ++p) lsSteps += (long long)(hskip[p] == 0) * iters;
+      };
+      if (nCold) tier_run(0, A.cold_iters, 6.0f, pTop, 0.085f, 0.0015f);
+      if (nMed) tier_run(1, std::max(1, A.cold_iters / 3), 40.0f, pTop, 0.040f, 0.0010f);
+      if (nCold + nMed < P) tier_run(2, A.iters, 400.0f, 3.0f * pTop, 0.012f, 0.0003f);
+      if (A.search_fw > 0) {
+        const int rc = sidon_frank_wolfe(pop, nullptr, n, P, A.search_fw, A.fw_iters, 12, 0.30f, s);
+        if (rc != SIDON_ERR_LDS) CHECK_SIDON(rc);
+      }
+      // ---- evaluate, signatures, policy (main.cu:640-735) ----
+      CHECK_SIDON(sidon_eval_f64(pop, n, P, fit, nact, 1e-4, s));
+      CHECK_SIDON(sidon_support_sig(pop, n, P, 0.02f, sig, nnz, s));
+      CHECK_HIP(hipMemcpyAsync(hfit.data(), fit, P * sizeof(double), hipMemcpyDeviceToHost, s));
+      CHECK_HIP(hipMemcpyAsync(hsig.data(), sig, P * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, s));
+      CHECK_HIP(hipMemcpyAsync(hnnz.data(), nnz, P * sizeof(int), hipMemcpyDeviceToHost, s));
+      CHECK_HIP(hipMemcpyAsync(hnact.data(), nact, P * sizeof(int), hipMemcpyDeviceToHost, s));
+      CHECK_HIP(hipStreamSynchronize(s));
+      evals += P;
+      int bi = 0, improved = 0;
+      CHECK_SIDON(sidon_policy_observe(pol, hfit.data(), hsig.data(), &bi, &improved));
+      if (improved) {
+        std::vector<float> row(n);
+        CHECK_HIP(hipMemcpy(row.data(), pop + (size_t)bi * n, n * sizeof(float),
+                            hipMemcpyDeviceToHost));
+        bestA.assign(row.begin(), row.end());
+        bestF = hfit[bi];
+        bestN = n;
+        save_vec(A.out + ".txt", bestA, bestF);
+      }
+      int nmig = 0, reseed = -1;
+      CHECK_SIDON(sidon_policy_plan(pol, gen, p1.data(), p2.data(), op.data(), hst.data(),
+                                    msrc.data(), mdst.data(), &nmig, &reseed));
+      for (int k = 0; k < nmig; ++k)  // ring migration: copy the vector before the variation
+        CHECK_HIP(hipMemcpyAsync(pop + (size_t)mdst[k] * n, pop + (size_t)msrc[k] * n,
+                                 n * sizeof(float), hipMemcpyDeviceToDevice, s));
+      CHECK_HIP(hipMemcpyAsync(dp1, p1.data(), P * sizeof(int), hipMemcpyHostToDevice, s));
+      CHECK_HIP(hipMemcpyAsync(dp2, p2.data(), P * sizeof(int), hipMemcpyHostToDevice, s));
+      CHECK_HIP(hipMemcpyAsync(dop, op.data(), P * sizeof(int), hipMemcpyHostToDevice, s));
+      CHECK_HIP(hipMemcpyAsync(dst, hst.data(), P * sizeof(float), hipMemcpyHostToDevice, s));
+      CHECK_SIDON(sidon_variation(pop, nxt, n, P, dp1, dp2, dop, dst, A.seed,
+                                  (unsigned long long)(gen + 1000 * (int)lvl), s));
+      std::swap(pop, nxt);
+      curOp = op;
+      if (reseed >= 0 && bestN == n) {  // keep the incumbent in the gene pool (main.cu:747-755)
+        std::vector<float> row(bestA.begin(), bestA.end());
+        CHECK_HIP(hipMemcpyAsync(pop + (size_t)reseed * n, row.data(), n * sizeof(float),
+                                 hipMemcpyHostToDevice, s));
+        CHECK_HIP(hipStreamSynchronize(s));
+        curOp[reseed] = 20;
+      }
+      if (gen > 0 && gen % 5 == 0) {  // credit assignment (main.cu:757-769)
+        CHECK_SIDON(sidon_eval_f64(pop, n, P, fit, nact, 1e-4, s));
+        CHECK_HIP(hipMemcpyAsync(cfit.data(), fit, P * sizeof(double), hipMemcpyDeviceToHost, s));
+        CHECK_HIP(hipStreamSynchronize(s));
+        evals += P;
+        CHECK_SIDON(sidon_policy_credit(pol, cfit.data()));
+      }
+      if (gen % 10 == 0 || improved) {  // progress record (main.cu:771-783)
+        double mean = 0.0, lb = 0.0;
+        for (double x : hfit) mean += x / P;
+        int cells = 0, since = 0;
+        CHECK_SIDON(sidon_policy_stats(pol, nullptr, nullptr, &lb, &cells, &since));
+        std::printf("{\"stat\": {\"t\": %.1f, \"n\": %d, \"gen\": %d, \"best\": %.15f, "
+                    "\"level_best\": %.15f, \"mean\": %.9f, \"evals\": %lld, "
+                    "\"ls_steps\": %lld, \"cells\": %d, \"peaks\": %d, \"nnz\": %d, "
+                    "\"cold\": %d, \"medium\": %d, \"since_improved\": %d%s}}\n",
+                    seconds_since(t0), n, gen, bestF, lb, mean, evals, lsSteps, cells,
+                    hnact[bi], hnnz[bi], nCold, nMed, since, improved ? ", \"improved\": true" : "");
+      }
+      ++gen;
+    }
+    double w[SIDON_POLICY_OPS];
+    int cells = 0;
+    CHECK_SIDON(sidon_policy_stats(pol, w, nullptr, nullptr, &cells, nullptr));
+    std::printf("{\"level\": %d, \"generations\": %d, \"seconds\": %.1f, \"best_F\": %.15f, "
+                "\"cells\": %d, \"op_weights\": [", n, gen, seconds_since(t0), bestF, cells);
+    for (int o = 0; o < SIDON_POLICY_OPS; ++o) std::printf("%s%.3f", o ? ", " : "", w[o]);
+    std::printf("]}\n");
+    if (lvl + 1 < L.size()) {  // split every candidate n -> 2n
+      if (L[lvl + 1] != 2 * n) { std::fprintf(stderr, "ladder must double\n"); return 2; }
+      CHECK_SIDON(sidon_refine(pop, n, P, nxt, s));
+      CHECK_HIP(hipStreamSynchronize(s));
+      std::swap(pop, nxt);
+      std::fill(curOp.begin(), curOp.end(), 20);  // refined parents: warm refinement
+    }
+  }
+  sidon_policy_destroy(pol);
+  const double hf = host_F(bestA);
+  save_vec(A.out + ".txt", bestA, std::max(bestF, hf));
+  save_certificate(A.out + ".json", bestA, hf);
+  std::printf("{\"best_F\": %.15f, \"host_recheck\": %.15f, \"n\": %zu, \"bound\": \"C <= %.9f\", "
+              "\"files\": [\"%s.txt\", \"%s.json\"], \"seconds\": %.1f, \"ls_steps\": %lld, "
+              "\"evals\": %lld}\n",
+              bestF, hf, bestA.size(), std::max(bestF, hf), A.out.c_str(), A.out.c_str(),
+              seconds_since(t0), lsSteps, evals);
+  return 0;
+}
+
+// ---------------------------------------------------------- graph search --
+int search_graph(const Args& A) {
   const std::vector<int> L = A.ladder.empty() ? std::vector<int>{256, 512, 1024, 2048} : A.ladder;
   const int P = A.P, nmax = L.back();
   std::mt19937_64 rng(A.seed);
@@ -497,9 +671,11 @@ void usage() {
       "      --P 512 --islands 8 --iters 200 --gens 3 --sigma 0.05 --lengths 600,...,8400\n"
       "  --ladder                   Frank-Wolfe + n -> 2n split ladder\n"
       "      --P 256 --levels 1050,2100,4200,8400 --fw-rounds 50 --cp-iters 2000\n"
-      "  --search                   ladder search with generation graphs\n"
-      "      --P 256 --levels 256,512,1024,2048 --islands 8 --iters 200 --gens 4\n"
+      "  --search                   the reference's search loop (policy, 14 operators)\n"
+      "      --P 256 --levels 256,512,1024,2048 --islands 8 --elite 2 --iters 200\n"
+      "      --cold-iters 1000 --pmax 20000 --search-fw 0 --fw-iters 500 --gens 4\n"
       "      --seconds 60 --out best   (writes best.txt and best.json)\n"
+      "  --search-graph             ladder search on the device generation graph (op 2)\n"
       "  --seed S  --no-tune\n");
 }
 
@@ -514,7 +690,9 @@ int main(int argc, char** argv) {
       if (i + 1 >= argc) { usage(); std::exit(2); }
       return argv[++i];
     };
-    if (s == "--selftest" || s == "--uneven" || s == "--ladder" || s == "--search") A.mode = s;
+    if (s == "--selftest" || s == "--uneven" || s == "--ladder" || s == "--search" ||
+        s == "--search-graph")
+      A.mode = s;
     else if (s == "--P") A.P = std::atoi(next());
     else if (s == "--islands") A.islands = std::atoi(next());
     else if (s == "--iters") A.iters = std::atoi(next());
@@ -526,6 +704,10 @@ int main(int argc, char** argv) {
     else if (s == "--fw-rounds") A.fw_rounds = std::atoi(next());
     else if (s == "--cp-iters") A.cp_iters = std::atoi(next());
     else if (s == "--seconds") A.seconds = std::atof(next());
+    else if (s == "--cold-iters") A.cold_iters = std::atoi(next());
+    else if (s == "--pmax") A.pmax = std::atof(next());
+    else if (s == "--search-fw") A.search_fw = std::atoi(next());
+    else if (s == "--fw-iters") A.fw_iters = std::atoi(next());
     else if (s == "--seed") A.seed = std::strtoull(next(), nullptr, 10);
     else if (s == "--out") A.out = next();
     else if (s == "--no-tune") A.tune = false;
@@ -536,6 +718,7 @@ int main(int argc, char** argv) {
   if (A.mode == "--uneven") return uneven(A);
   if (A.mode == "--ladder") return ladder(A);
   if (A.mode == "--search") return search(A);
+  if (A.mode == "--search-graph") return search_graph(A);
   usage();
   return 2;
 }

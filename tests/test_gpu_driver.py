@@ -1,0 +1,60 @@
+"""The C++ host driver (sidon_search, the counterpart of the reference's
+sidon.exe host loop): its device self-test, the policy search (three anneal
+tiers, support signatures, tabu / operator weights, all fourteen operators,
+n -> 2n splits) and the files it writes, checked with the oracle's verifier
+(tools/probe.cpp --verify semantics) and hostF."""
+from __future__ import annotations
+
+import json
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import REPO
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = REPO / "sidon-autocorrelation-bound_amd" / "sidon_search"
+
+
+def run(args, timeout=150):
+    print("[driver]", " ".join(args), flush=True)
+    out = subprocess.run([str(DRIVER)] + args, capture_output=True, text=True, timeout=timeout)
+    assert out.returncode == 0, out.stderr[-2000:]
+    return out.stdout
+
+
+def test_selftest():
+    out = run(["--selftest"])
+    assert "FAIL" not in out and "OK" in out
+
+
+def test_policy_search_improves_and_writes_verifiable_files(O, tmp_path):
+    base = tmp_path / "best"
+    out = run(["--search", "--P", "64", "--islands", "4", "--elite", "2", "--levels", "128,256",
+               "--iters", "60", "--cold-iters", "240", "--gens", "12", "--seconds", "4",
+               "--out", str(base), "--seed", "7"])
+    recs = [json.loads(line) for line in out.splitlines() if line.startswith("{")]
+    stats = [r["stat"] for r in recs if "stat" in r]
+    levels = [r for r in recs if "level" in r]
+    final = recs[-1]
+    assert [r["level"] for r in levels] == [128, 256]
+    assert all(r["generations"] >= 12 for r in levels)
+    assert len(levels[0]["op_weights"]) == 14 and levels[0]["cells"] > 1
+    assert stats[0]["cold"] > 0                       # fresh seeds take the cold tier
+    assert final["best_F"] < 1.56 and final["n"] in (128, 256)
+    # the saved vector verifies, and its stored F is the hostF of its weights
+    v = O.verify_file(str(base) + ".txt")
+    assert v["rc"] == 0 and v["n"] == final["n"]
+    a = np.array(json.loads((tmp_path / "best.json").read_text())["weights"])
+    assert a.size == final["n"] and (a >= 0).all()
+    assert O.F_hostorder(a) == pytest.approx(final["host_recheck"], rel=0, abs=0)
+    assert abs(final["best_F"] - final["host_recheck"]) <= 1e-12 * final["best_F"]
+
+
+def test_graph_search_runs(tmp_path):
+    out = run(["--search-graph", "--P", "64", "--islands", "4", "--levels", "128,256",
+               "--iters", "50", "--gens", "3", "--seconds", "1", "--out", str(tmp_path / "g")])
+    final = json.loads(out.strip().splitlines()[-1])
+    assert final["best_F"] < 1.7 and (tmp_path / "g.json").exists()
