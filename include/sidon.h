@@ -274,6 +274,9 @@ typedef struct sidon_policy sidon_policy;
 int sidon_policy_create(int P, int islands, int elite, unsigned long long seed,
                         sidon_policy** out);
 void sidon_policy_destroy(sidon_policy* pol);
+/* Adopts an incumbent found outside the population (a resumed vector, its F
+ * by hostF): the tabu test compares cells against it (main.cu:414-428). */
+int sidon_policy_adopt(sidon_policy* pol, double f);
 /* Starts a ladder level: the level best and the generations-since-improvement
  * count restart. */
 int sidon_policy_new_level(sidon_policy* pol);

@@ -38,7 +38,7 @@ EXPORTS = (
     "sidon_generation_enqueue", "sidon_generation_destroy", "sidon_tune_local_search",
     "sidon_measure_fp32_ceilings",
     "sidon_ls_block_steps",
-    "sidon_policy_create", "sidon_policy_destroy", "sidon_policy_new_level",
+    "sidon_policy_create", "sidon_policy_destroy", "sidon_policy_adopt", "sidon_policy_new_level",
     "sidon_policy_observe", "sidon_policy_plan", "sidon_policy_credit", "sidon_policy_stats",
     "sidon_policy_tiers",
 )
@@ -104,6 +104,7 @@ def _declare(L):
         "sidon_support_sig": ([_vp, _i, _i, C.c_float, _vp, _vp, _vp], _i),
         "sidon_policy_create": ([_i, _i, _i, _u64, _vp], _i),
         "sidon_policy_destroy": ([_vp], None),
+        "sidon_policy_adopt": ([_vp, _d], _i),
         "sidon_policy_new_level": ([_vp], _i),
         "sidon_policy_observe": ([_vp, _vp, _vp, _vp, _vp], _i),
         "sidon_policy_plan": ([_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
@@ -350,6 +351,9 @@ class Policy:
         if h is not None and h.value:
             lib().sidon_policy_destroy(h)
             self._h = None
+
+    def adopt(self, f: float) -> None:
+        _check(lib().sidon_policy_adopt(self._h, f), "sidon_policy_adopt")
 
     def new_level(self) -> None:
         _check(lib().sidon_policy_new_level(self._h), "sidon_policy_new_level")

@@ -106,6 +106,39 @@ void save_vec(const std::string& path, const std::vector<double>& a, double F) {
   std::fclose(f);
 }
 
+// saved-vector reader (src/main.cu:340-352): '#' comments, "n N", "F x", then
+// one height a line; false when unreadable, empty, or its count disagrees with n.
+bool load_vec(const std::string& path, std::vector<double>& a) {
+  FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) return false;
+  a.clear();
+  char line[512];
+  int n = 0;
+  while (std::fgets(line, sizeof line, f)) {
+    if (line[0] == '#' || line[0] == '\n' || line[0] == '\0') continue;
+    if (line[0] == 'n' && line[1] == ' ') { n = std::atoi(line + 2); continue; }
+    if (line[0] == 'F' && line[1] == ' ') continue;
+    a.push_back(std::strtod(line, nullptr));
+  }
+  std::fclose(f);
+  if (n && (int)a.size() != n) return false;
+  return !a.empty();
+}
+
+// A vector of m heights onto n cells by nearest-lower index, sum 1
+// (the reference's injectDouble, src/main.cu:555-562).
+void resample(const std::vector<double>& v, int n, float* dst) {
+  const int m = (int)v.size();
+  double S = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double x = v[std::min(m - 1, (int)((long long)i * m / n))];
+    dst[i] = (float)x;
+    S += x;
+  }
+  if (S > 0.0)
+    for (int i = 0; i < n; ++i) dst[i] = (float)(dst[i] / S);
+}
+
 // certificate JSON (the schema of certificates/*.json)
 void save_certificate(const std::string& path, const std::vector<double>& a, double F) {
   FILE* f = std::fopen(path.c_str(), "w");
@@ -137,6 +170,7 @@ struct Args {
   std::vector<int> ladder;
   std::vector<int> lengths{600, 1200, 2100, 4096, 8400};
   std::string out = "best";
+  std::string resume;
   bool tune = true;
 };
 
@@ -418,7 +452,22 @@ int ladder(const Args& A) {
 // built here, so the levels get all of it); between levels every candidate is
 // split n -> 2n (F preserved exactly).
 int search(const Args& A) {
-  const std::vector<int> L = A.ladder.empty() ? std::vector<int>{256, 512, 1024, 2048} : A.ladder;
+  std::vector<int> L = A.ladder.empty() ? std::vector<int>{256, 512, 1024, 2048} : A.ladder;
+  // --resume (main.cu:414-443): the saved vector becomes the incumbent at once,
+  // and levels coarser than it are dropped (a split preserves F exactly, so
+  // they cannot beat it)
+  std::vector<double> seedVec;
+  if (!A.resume.empty()) {
+    if (!load_vec(A.resume, seedVec)) {
+      std::fprintf(stderr, "could not read %s\n", A.resume.c_str());
+      return 2;
+    }
+    size_t drop = 0;
+    while (drop + 1 < L.size() && L[drop] < (int)seedVec.size()) ++drop;
+    L.erase(L.begin(), L.begin() + drop);
+    std::printf("{\"resumed\": \"%s\", \"n\": %zu, \"F\": %.15f, \"levels_dropped\": %zu}\n",
+                A.resume.c_str(), seedVec.size(), host_F(seedVec), drop);
+  }
   const int P = A.P, NI = A.islands, nmax = *std::max_element(L.begin(), L.end());
   if (P % NI != 0) { std::fprintf(stderr, "P must be divisible by islands\n"); return 2; }
   const int PI = P / NI;
@@ -458,10 +507,27 @@ int search(const Args& A) {
   double bestF = 1e30;
   std::vector<double> bestA;
   int bestN = 0;
+  if (!seedVec.empty()) {
+    bestA = seedVec;
+    bestF = host_F(seedVec);
+    bestN = (int)seedVec.size();
+    CHECK_SIDON(sidon_policy_adopt(pol, bestF));
+  }
   long long lsSteps = 0, evals = 0;
   for (size_t lvl = 0; lvl < L.size(); ++lvl) {
     const int n = L[lvl];
     CHECK_SIDON(sidon_policy_new_level(pol));
+    // the incumbent in slot 0 and the resumed vector in slot 1, resampled to n
+    // (main.cu:553-565), both cloned into the next generation's warm tier
+    std::vector<float> row(n);
+    const std::vector<double>* inject[2] = {&bestA, &seedVec};
+    for (int k = 0; k < 2 && k < P; ++k) {
+      if (inject[k]->empty()) continue;
+      resample(*inject[k], n, row.data());
+      CHECK_HIP(hipMemcpy(pop + (size_t)k * n, row.data(), n * sizeof(float),
+                          hipMemcpyHostToDevice));
+      curOp[k] = 20;
+    }
     const auto tl = Clock::now();
     int gen = 0;
     while (gen < A.gens || seconds_since(tl) < lvlBudget[lvl]) {
@@ -675,6 +741,7 @@ void usage() {
       "      --P 256 --levels 256,512,1024,2048 --islands 8 --elite 2 --iters 200\n"
       "      --cold-iters 1000 --pmax 20000 --search-fw 0 --fw-iters 500 --gens 4\n"
       "      --seconds 60 --out best   (writes best.txt and best.json)\n"
+      "      --resume FILE             (a saved vector: adopted as the incumbent)\n"
       "  --search-graph             ladder search on the device generation graph (op 2)\n"
       "  --seed S  --no-tune\n");
 }
@@ -710,6 +777,7 @@ int main(int argc, char** argv) {
     else if (s == "--fw-iters") A.fw_iters = std::atoi(next());
     else if (s == "--seed") A.seed = std::strtoull(next(), nullptr, 10);
     else if (s == "--out") A.out = next();
+    else if (s == "--resume") A.resume = next();
     else if (s == "--no-tune") A.tune = false;
     else { usage(); return 2; }
   }

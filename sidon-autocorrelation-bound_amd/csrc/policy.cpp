@@ -68,6 +68,12 @@ extern "C" int sidon_policy_create(int P, int islands, int elite, unsigned long 
 
 extern "C" void sidon_policy_destroy(sidon_policy* pol) { delete pol; }
 
+extern "C" int sidon_policy_adopt(sidon_policy* pol, double f) {
+  if (!pol || !(f > 0.0)) return SIDON_ERR_ARG;
+  pol->bestF = std::min(pol->bestF, f);
+  return SIDON_OK;
+}
+
 extern "C" int sidon_policy_new_level(sidon_policy* pol) {
   if (!pol) return SIDON_ERR_ARG;
   pol->lvlBest = 1e30;

@@ -49,7 +49,7 @@ def test_policy_search_improves_and_writes_verifiable_files(O, tmp_path):
     assert v["rc"] == 0 and v["n"] == final["n"]
     a = np.array(json.loads((tmp_path / "best.json").read_text())["weights"])
     assert a.size == final["n"] and (a >= 0).all()
-    assert O.F_hostorder(a) == pytest.approx(final["host_recheck"], rel=0, abs=0)
+    assert O.F_hostorder(a) == pytest.approx(final["host_recheck"], rel=0, abs=1e-14)  # %.15f
     assert abs(final["best_F"] - final["host_recheck"]) <= 1e-12 * final["best_F"]
 
 
@@ -58,3 +58,21 @@ def test_graph_search_runs(tmp_path):
                "--iters", "50", "--gens", "3", "--seconds", "1", "--out", str(tmp_path / "g")])
     final = json.loads(out.strip().splitlines()[-1])
     assert final["best_F"] < 1.7 and (tmp_path / "g.json").exists()
+
+
+def test_resume_adopts_the_vector_and_drops_coarser_levels(O, certs, tmp_path):
+    a = certs[512]["a"]
+    src = tmp_path / "seed.txt"
+    O.write_vec(str(src), a, O.F_hostorder(a))
+    base = tmp_path / "res"
+    out = run(["--search", "--P", "64", "--islands", "4", "--levels", "256,512,1024",
+               "--iters", "60", "--cold-iters", "240", "--gens", "3", "--seconds", "2",
+               "--resume", str(src), "--out", str(base)])
+    recs = [json.loads(line) for line in out.splitlines() if line.startswith("{")]
+    assert recs[0]["resumed"] == str(src) and recs[0]["levels_dropped"] == 1
+    assert recs[0]["F"] == pytest.approx(O.F_hostorder(a), rel=0, abs=1e-14)
+    assert [r["level"] for r in recs if "level" in r] == [512, 1024]
+    final = recs[-1]
+    assert final["host_recheck"] <= O.F_hostorder(a) + 1e-14   # never worse than it was handed
+    v = O.verify_file(str(base) + ".txt")
+    assert v["rc"] == 0 and v["n"] == final["n"]
