@@ -95,6 +95,23 @@ int sidon_local_search_f64(double* pop, double* mom, double* vel, const int* ski
                            const int* mode, int n, int P, int iters, double pStart,
                            double pEnd, double lrStart, double lrEnd, double beta1,
                            double beta2, int modeDefault, hipStream_t stream);
+
+/* fp64 local search of K candidates of any length n, spread over the whole
+ * device (the reference's fp64 polish, src/main.cu:798-846, whose fused
+ * kernel needs ~3n doubles of on-chip memory: n = 8400 does not fit one
+ * MI355X workgroup).  Same step as sidon_local_search_f64 (mode 0 projected
+ * Adam, 1 mirror descent; p, lr annealed geometrically over `iters`; moments
+ * start at zero; mom/vel, may be NULL, receive the final moments), with both
+ * products cut into work units for every CU and summed in a fixed order
+ * (deterministic).  `workspace` is a device buffer of at least
+ * sidon_polish_f64_workspace(n, K) bytes (about 3n^2/512 + 5n doubles per
+ * candidate: 30 MB for K = 8 at n = 8400); it holds no state between calls.
+ * pop is [K x n] row-major. */
+size_t sidon_polish_f64_workspace(int n, int K);
+int sidon_polish_f64(double* pop, double* mom, double* vel, int n, int K, int iters,
+                     double pStart, double pEnd, double lrStart, double lrEnd, double beta1,
+                     double beta2, int mode, void* workspace, size_t workspace_bytes,
+                     hipStream_t stream);
 /* Uneven lengths in one persistent launch: candidate p occupies
  * pop[p*stride .. p*stride + ns[p]) (ns on the device, ns[p] <= nmax <= stride);
  * `order` (may be NULL) lists the processing order, longest first gives the

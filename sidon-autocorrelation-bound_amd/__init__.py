@@ -31,7 +31,8 @@ EXPORTS = (
     "sidon_init", "sidon_last_error", "sidon_status_string", "sidon_lds_bytes",
     "sidon_lds_limit", "sidon_eval_f64", "sidon_eval_f64_ragged", "sidon_eval_f64_d",
     "sidon_eval_exact", "sidon_local_search", "sidon_local_search_f64",
-    "sidon_local_search_ragged", "sidon_set_ls_block", "sidon_get_ls_block", "sidon_lp_grad",
+    "sidon_local_search_ragged", "sidon_polish_f64_workspace", "sidon_polish_f64",
+    "sidon_set_ls_block", "sidon_get_ls_block", "sidon_lp_grad",
     "sidon_lp_grad_f64", "sidon_frank_wolfe", "sidon_refine", "sidon_variation",
     "sidon_variation_ragged", "sidon_support_sig", "sidon_select", "sidon_advance_gen", "sidon_order_by_length",
     "sidon_generation_create", "sidon_generation_run", "sidon_generation_current",
@@ -102,6 +103,9 @@ def _declare(L):
         "sidon_refine": ([_vp, _i, _i, _vp, _vp], _i),
         "sidon_variation": ([_vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _u64, _u64, _vp], _i),
         "sidon_support_sig": ([_vp, _i, _i, C.c_float, _vp, _vp, _vp], _i),
+        "sidon_polish_f64_workspace": ([_i, _i], C.c_size_t),
+        "sidon_polish_f64": ([_vp, _vp, _vp, _i, _i, _i, _d, _d, _d, _d, _d, _d, _i, _vp,
+                              C.c_size_t, _vp], _i),
         "sidon_policy_create": ([_i, _i, _i, _u64, _vp], _i),
         "sidon_policy_destroy": ([_vp], None),
         "sidon_policy_adopt": ([_vp, _d], _i),
@@ -318,6 +322,24 @@ def variation_ragged(pop, ns, out, ns_out, p1, p2, op, strength, seed: int, gen_
                                         out.shape[0], _ptr(p1), _ptr(p2), _ptr(op),
                                         _ptr(strength), seed, _ptr(gen_dev), _stream(stream)),
            "sidon_variation_ragged")
+
+
+def polish_f64(pop: torch.Tensor, iters: int, pStart: float, pEnd: float, lrStart: float,
+               lrEnd: float, beta1: float = 0.9, beta2: float = 0.999, mode: int = 0,
+               mom=None, vel=None, workspace=None, stream=None) -> None:
+    """fp64 local search of the rows of `pop` (float64, K x n) spread over the
+    whole device: sidon_polish_f64.  `workspace` (uint8 device tensor) is
+    allocated here when not given."""
+    _need_gpu()
+    if pop.dtype != torch.float64:
+        raise ValueError("polish_f64 needs float64 rows")
+    K, n = pop.shape
+    need = lib().sidon_polish_f64_workspace(n, K)
+    if workspace is None:
+        workspace = torch.empty(need, dtype=torch.uint8, device=pop.device)
+    _check(lib().sidon_polish_f64(_ptr(pop), _ptr(mom), _ptr(vel), n, K, iters, pStart, pEnd,
+                                  lrStart, lrEnd, beta1, beta2, mode, _ptr(workspace),
+                                  workspace.numel(), _stream(stream)), "sidon_polish_f64")
 
 
 def support_sig(pop: torch.Tensor, tol: float = 0.02, stream=None):

@@ -1,0 +1,423 @@
+// polish.hip -- fp64 local search of a few candidates at any length, spread
+// over the whole device: the fp64 polish of the best vectors.
+//
+// Reference: src/main.cu:798-846 (fp64 polish: kLocalSearch<double> over the
+// elite, passes of rising exponent) and src/kernels.cuh:177-289 (the step).
+// The fused fp64 local search keeps a, b/c and G in one workgroup's LDS and so
+// stops near n = 4600 (3n doubles do not fit 160 KB at n = 8400).  Here one
+// step is five kernels over global memory (L2-resident: ~3n doubles per
+// candidate), and both products are cut into work units that every CU can take:
+//
+//   unit (candidate, tile q of 16 outputs, chunk cc of 32 blocks of 16 i)
+//     = one lane, 32 trips of 16 x 16 fp64 FMAs, partial sums to `part`;
+//
+// a workgroup holds 256 consecutive tiles of one chunk, so every lane runs the
+// same 32 trips (products outside a tile's range read zero padding), the
+// chunk's 512 heights are broadcast and the other operand's window sits in
+// LDS with one unused slot per 16 (lanes one tile apart read 17 doubles apart).
+// The next kernel adds a tile's chunk partials in chunk order, so the sums are
+// deterministic.  Step: conv units -> conv reduce (b, M) -> weights (c, R
+// partials) -> corr units -> update (G, R, Adam or mirror step, clamp >= 0,
+// renormalise), the same arithmetic as the fused kernel's step in fp64.
+#include <cmath>
+
+#include "common.hpp"
+#include "runtime.hpp"
+
+namespace sidon {
+namespace {
+
+constexpr int PU = 256;     // tiles (lanes) per unit workgroup
+constexpr int CH = 32;      // 16-blocks per chunk
+constexpr int PADA = 576;   // zero heights each side of the padded copy of a
+constexpr int RB = 256;     // threads per reduce / weights workgroup
+constexpr int UB = 1024;    // threads of the per-candidate update workgroup
+
+struct Shape {
+  int n, NJ, NTc, NCc, NTg, NCg, LA, LB;
+};
+
+__host__ __device__ inline Shape shape_of(int n) {
+  Shape s;
+  s.n = n;
+  s.NJ = (n + 15) / 16;                  // 16-blocks of a
+  s.NTc = (2 * n - 1 + 15) / 16;         // output tiles of b
+  s.NCc = (s.NJ + CH - 1) / CH;          // chunks of the conv reduction
+  s.NTg = s.NJ;                          // output tiles of G
+  s.NCg = s.NCc;                         // chunks of the corr reduction
+  s.LA = n + 2 * PADA;                   // padded copy of a
+  s.LB = 16 * (s.NTg + CH * s.NCg) + 64; // b / c, zero beyond 2n - 1
+  return s;
+}
+
+// workspace layout (doubles unless noted), per candidate:
+//   apad[LA], bc[LB], part[NCc * NTc * 16], gpart[NCg * NTg * 16], mom[n], vel[n],
+//   rpart[ceil(LB / RB)], then K u64 max bits, K ints: step counter, stopped.
+struct Ws {
+  double *apad, *bc, *part, *gpart, *mom, *vel, *rpart;
+  unsigned long long* mbits;
+  int *step, *stopped;
+  long long sa, sb, sp, sg, sn, sr;  // per-candidate strides
+};
+
+__host__ __device__ inline int rblocks(const Shape& s) { return (s.LB + RB - 1) / RB; }
+
+__host__ __device__ inline size_t ws_bytes(int n, int K) {
+  const Shape s = shape_of(n);
+  const size_t per = (size_t)s.LA + s.LB + (size_t)s.NCc * s.NTc * 16 + (size_t)s.NCg * s.NTg * 16 +
+                     2 * (size_t)n + rblocks(s);
+  return per * K * sizeof(double) + K * (sizeof(unsigned long long) + 2 * sizeof(int)) + 256;
+}
+
+__host__ __device__ inline Ws ws_of(void* base, int n, int K) {
+  const Shape s = shape_of(n);
+  Ws w;
+  w.sa = s.LA;
+  w.sb = s.LB;
+  w.sp = (long long)s.NCc * s.NTc * 16;
+  w.sg = (long long)s.NCg * s.NTg * 16;
+  w.sn = n;
+  w.sr = rblocks(s);
+  double* d = reinterpret_cast<double*>(base);
+  w.apad = d; d += w.sa * K;
+  w.bc = d; d += w.sb * K;
+  w.part = d; d += w.sp * K;
+  w.gpart = d; d += w.sg * K;
+  w.mom = d; d += w.sn * K;
+  w.vel = d; d += w.sn * K;
+  w.rpart = d; d += w.sr * K;
+  w.mbits = reinterpret_cast<unsigned long long*>(d);
+  w.step = reinterpret_cast<int*>(w.mbits + K);
+  w.stopped = w.step + K;
+  return w;
+}
+
+struct Sched {
+  int iters;
+  double pStart, lnPr, lrStart, lnLr, beta1, beta2;
+};
+
+// p and lr of step `it`: the fused kernel's geometric anneal
+__device__ __forceinline__ void sched_at(const Sched& S, int it, double& p, double& lr) {
+  const double frac = S.iters > 1 ? (double)it / (double)(S.iters - 1) : 0.0;
+  p = S.pStart * exp(S.lnPr * frac);
+  lr = S.lrStart * exp(S.lnLr * frac);
+}
+
+// LDS slot of window element e (one unused slot after every 16)
+__device__ __forceinline__ int wslot(int e) { return e + (e >> 4); }
+
+// ---------------------------------------------------------------- setup --
+__global__ void k_polish_setup(const double* __restrict__ pop, int n, Ws w) {
+  const int cand = blockIdx.y;
+  double* ap = w.apad + w.sa * cand;
+  const int LA = n + 2 * PADA;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < LA; e += gridDim.x * blockDim.x) {
+    const int i = e - PADA;
+    ap[e] = (i >= 0 && i < n) ? pop[(size_t)cand * n + i] : 0.0;
+  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    w.mom[w.sn * cand + i] = 0.0;
+    w.vel[w.sn * cand + i] = 0.0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    w.mbits[cand] = 0ull;
+    w.step[cand] = 0;
+    w.stopped[cand] = 0;
+  }
+}
+
+// --------------------------------------------------------- conv units ----
+// part[cand][cc][q][t] = sum over i in chunk cc of a_i a_{16q+t-i}.  Tiles of
+// chunk cc whose range of i meets the chunk are [qlo, qhi]; workgroup
+// blockIdx.x takes 256 of them.
+__device__ __forceinline__ void conv_tile_range(int n, int q, int& J0, int& J1) {
+  const int k0 = 16 * q;
+  J0 = (k0 - (n - 1) > 0 ? k0 - (n - 1) : 0) >> 4;
+  J1 = (k0 + 15 < n - 1 ? k0 + 15 : n - 1) >> 4;
+}
+
+__device__ void conv_chunk_tiles(const Shape& s, int cc, int& qlo, int& qhi) {
+  // J1(q) and J0(q) do not decrease with q: binary searches
+  int lo = 0, hi = s.NTc;  // first q with J1(q) >= 32cc
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    int a, b;
+    conv_tile_range(s.n, mid, a, b);
+    if (b >= CH * cc) hi = mid; else lo = mid + 1;
+  }
+  qlo = lo;
+  lo = 0;
+  hi = s.NTc;  // first q with J0(q) > 32cc + 31
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    int a, b;
+    conv_tile_range(s.n, mid, a, b);
+    if (a > CH * cc + CH - 1) hi = mid; else lo = mid + 1;
+  }
+  qhi = lo - 1;
+}
+
+__global__ __launch_bounds__(PU) void k_polish_conv_units(Shape s, Ws w) {
+  __shared__ double sx[CH * 16 + CH];             // the chunk's 512 heights (padded)
+  __shared__ double sw[(16 * (PU + CH) + 32) * 17 / 16 + 16];  // a window (padded)
+  const int cand = blockIdx.z, cc = blockIdx.y;
+  int qlo, qhi;
+  conv_chunk_tiles(s, cc, qlo, qhi);
+  const int qa = qlo + PU * blockIdx.x;
+  if (qa > qhi) return;
+  const int qb = qa + PU - 1 < qhi ? qa + PU - 1 : qhi;
+  const double* ap = w.apad + w.sa * cand + PADA;  // ap[i] = a_i, zero for -PADA <= i < n + PADA
+  const int J0c = CH * cc;
+  for (int e = threadIdx.x; e < CH * 16; e += PU) sx[wslot(e)] = ap[16 * J0c + e];
+  // window: a_x for x in [16 (qa - J0c - 31) - 15, 16 (qb - J0c) + 15]
+  const int wlo = 16 * (qa - J0c - (CH - 1)) - 15;
+  const int wlen = 16 * (qb - qa + CH) + 15;
+  for (int e = threadIdx.x; e < wlen; e += PU) sw[wslot(e)] = ap[wlo + e];
+  __syncthreads();
+  const int q = qa + (int)threadIdx.x;
+  if (q > qb) return;
+  double acc[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) acc[t] = 0.0;
+  for (int jj = 0; jj < CH; ++jj) {
+    double x[16], v[31];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) x[u] = sx[wslot(16 * jj + u)];
+    const int base = 16 * (q - J0c - jj) - 15 - wlo;  // window offset of a_{16(q-J)-15}
+#pragma unroll
+    for (int m = 0; m < 31; ++m) v[m] = sw[wslot(base + m)];
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+#pragma unroll
+      for (int t = 0; t < 16; ++t) acc[t] = fma(x[u], v[t - u + 15], acc[t]);
+  }
+  double* o = w.part + w.sp * cand + ((size_t)cc * s.NTc + q) * 16;
+#pragma unroll
+  for (int t = 0; t < 16; ++t) o[t] = acc[t];
+}
+
+// ---------------------------------------------------------- conv reduce --
+// b_k = sum of the chunk partials of tile k/16 in chunk order; the maximum
+// rises by a 64-bit atomic max on the bits of the non-negative value.
+__global__ __launch_bounds__(RB) void k_polish_conv_reduce(Shape s, Ws w) {
+  const int cand = blockIdx.y;
+  if (w.stopped[cand]) return;
+  const int k = blockIdx.x * RB + threadIdx.x;
+  double b = 0.0;
+  if (k < 16 * s.NTc) {
+    const int q = k >> 4, t = k & 15;
+    int J0, J1;
+    conv_tile_range(s.n, q, J0, J1);
+    const double* p = w.part + w.sp * cand + t;
+    for (int cc = J0 / CH; cc <= J1 / CH; ++cc) b += p[((size_t)cc * s.NTc + q) * 16];
+    if (k >= 2 * s.n - 1) b = 0.0;
+  }
+  if (k < s.LB) w.bc[w.sb * cand + k] = b;
+  const double m = wave_max(b);
+  if ((threadIdx.x & (kWave - 1)) == 0)
+    atomicMax(w.mbits + cand, (unsigned long long)__double_as_longlong(m));
+}
+
+// -------------------------------------------------------------- weights --
+// c_k = exp((p-1) log1p((b_k - M) / M)) in place of b_k; R partial sums of
+// c (1 + d) per workgroup (added in workgroup order by the update).
+__global__ __launch_bounds__(RB) void k_polish_weights(Shape s, Ws w, Sched S) {
+  constexpr int NW = RB / kWave;
+  __shared__ double slots[NW];
+  const int cand = blockIdx.y;
+  if (w.stopped[cand]) return;
+  const double M = __longlong_as_double((long long)w.mbits[cand]);
+  if (!(M > 0.0)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) w.stopped[cand] = 1;  // the fused kernel's break
+    return;
+  }
+  double p, lr;
+  sched_at(S, w.step[cand], p, lr);
+  const double pm1 = p - 1.0;
+  const int k = blockIdx.x * RB + threadIdx.x;
+  double r = 0.0;
+  if (k < s.LB) {
+    double* bc = w.bc + w.sb * cand;
+    double c = 0.0;
+    if (k < 2 * s.n - 1) {
+      const double d = (bc[k] - M) / M;
+      c = lp_weight_fast(d, pm1);
+      r = c * (1.0 + d);
+    }
+    bc[k] = c;
+  }
+  const double R = block_sum<double, NW>(r, slots);
+  if (threadIdx.x == 0) w.rpart[w.sr * cand + blockIdx.x] = R;
+}
+
+// --------------------------------------------------------- corr units ----
+// gpart[cand][cc][q][t] = sum over j in chunk cc of c_{16q+t+j} a_j.
+__global__ __launch_bounds__(PU) void k_polish_corr_units(Shape s, Ws w) {
+  __shared__ double sx[CH * 16 + CH];
+  __shared__ double sw[(16 * (PU + CH) + 32) * 17 / 16 + 16];
+  const int cand = blockIdx.z, cc = blockIdx.y;
+  if (w.stopped[cand]) return;
+  const int qa = PU * blockIdx.x;
+  if (qa >= s.NTg) return;
+  const int qb = qa + PU - 1 < s.NTg - 1 ? qa + PU - 1 : s.NTg - 1;
+  const double* ap = w.apad + w.sa * cand + PADA;
+  const double* cp = w.bc + w.sb * cand;
+  const int J0c = CH * cc;
+  for (int e = threadIdx.x; e < CH * 16; e += PU) sx[wslot(e)] = ap[16 * J0c + e];
+  // window: c_x for x in [16 (qa + J0c), 16 (qb + J0c + 31) + 30]
+  const int wlo = 16 * (qa + J0c);
+  const int wlen = 16 * (qb - qa + CH) + 15;
+  for (int e = threadIdx.x; e < wlen; e += PU) sw[wslot(e)] = cp[wlo + e];
+  __syncthreads();
+  const int q = qa + (int)threadIdx.x;
+  if (q > qb) return;
+  double acc[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) acc[t] = 0.0;
+  for (int jj = 0; jj < CH; ++jj) {
+    double x[16], v[31];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) x[u] = sx[wslot(16 * jj + u)];
+    const int base = 16 * (q + J0c + jj) - wlo;  // window offset of c_{16(q+J)}
+#pragma unroll
+    for (int m = 0; m < 31; ++m) v[m] = sw[wslot(base + m)];
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+#pragma unroll
+      for (int t = 0; t < 16; ++t) acc[t] = fma(x[u], v[t + u], acc[t]);
+  }
+  double* o = w.gpart + w.sg * cand + ((size_t)cc * s.NTg + q) * 16;
+#pragma unroll
+  for (int t = 0; t < 16; ++t) o[t] = acc[t];
+}
+
+// --------------------------------------------------------------- update --
+// One workgroup per candidate: R, G_i (chunk order), g = G/R - M, Adam step
+// (lr/n) mhat / (sqrt(vhat) + 1e-30) and clamp >= 0 (mode 0) or mirror step
+// a exp(clamp(-lr g / M, +-4)) (mode 1), renormalise sum a = 1 (uniform if not
+// positive), M back to 0, step + 1.
+__global__ __launch_bounds__(UB) void k_polish_update(Shape s, Ws w, Sched S, int mode,
+                                                      double* __restrict__ pop) {
+  constexpr int NW = UB / kWave;
+  __shared__ double slots[2 * NW];
+  const int cand = blockIdx.x;
+  if (w.stopped[cand]) return;
+  const int n = s.n;
+  const double M = __longlong_as_double((long long)w.mbits[cand]);
+  double R = 0.0;
+  for (int b = 0; b < w.sr; ++b) R += w.rpart[w.sr * cand + b];
+  const int it = w.step[cand];
+  double p, lr;
+  sched_at(S, it, p, lr);
+  double b1t = 1.0, b2t = 1.0;
+  for (int k = 0; k <= it; ++k) { b1t *= S.beta1; b2t *= S.beta2; }
+  const double bc1 = 1.0 / (1.0 - b1t), bc2 = 1.0 / (1.0 - b2t);
+  double* ap = w.apad + w.sa * cand + PADA;
+  double* mo = w.mom + w.sn * cand;
+  double* ve = w.vel + w.sn * cand;
+  const double* gp = w.gpart + w.sg * cand;
+  double* out = pop + (size_t)cand * n;
+  double sloc = 0.0;
+  for (int i = threadIdx.x; i < n; i += UB) {
+    double G = 0.0;
+    for (int cc = 0; cc < s.NCg; ++cc) G += gp[((size_t)cc * s.NTg + (i >> 4)) * 16 + (i & 15)];
+    const double gi = G / R - M;
+    const double ai = ap[i];
+    double xx;
+    if (mode == 0) {
+      const double mm = S.beta1 * mo[i] + (1.0 - S.beta1) * gi;
+      const double vv = S.beta2 * ve[i] + (1.0 - S.beta2) * gi * gi;
+      mo[i] = mm;
+      ve[i] = vv;
+      const double step = (lr / (double)n) * (mm * bc1) / (sqrt(vv * bc2) + 1e-30);
+      xx = ai - step;
+      if (xx < 0.0) xx = 0.0;
+    } else {
+      double e = -lr * (gi / M);
+      if (e > 4.0) e = 4.0;
+      if (e < -4.0) e = -4.0;
+      xx = ai * exp(e);
+    }
+    out[i] = xx;
+    sloc += xx;
+  }
+  double Ssum = block_sum<double, NW>(sloc, slots);
+  const bool reset = !(Ssum > 0.0);
+  if (reset) Ssum = 1.0;
+  const double inv = 1.0 / Ssum;
+  __syncthreads();  // every thread has written its heights to `out`
+  for (int i = threadIdx.x; i < n; i += UB) {
+    const double x = reset ? 1.0 / (double)n : out[i] * inv;
+    out[i] = x;
+    ap[i] = x;
+  }
+  if (threadIdx.x == 0) {
+    w.mbits[cand] = 0ull;
+    w.step[cand] = it + 1;
+  }
+}
+
+__global__ void k_polish_moments(const Ws w, int n, int K, double* mom, double* vel) {
+  const long long tot = (long long)n * K;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    if (mom) mom[e] = w.mom[e];
+    if (vel) vel[e] = w.vel[e];
+  }
+}
+
+}  // namespace
+}  // namespace sidon
+
+using namespace sidon;
+
+extern "C" size_t sidon_polish_f64_workspace(int n, int K) {
+  if (n < 1 || K < 1) return 0;
+  return ws_bytes(n, K);
+}
+
+extern "C" int sidon_polish_f64(double* pop, double* mom, double* vel, int n, int K, int iters,
+                                double pStart, double pEnd, double lrStart, double lrEnd,
+                                double beta1, double beta2, int mode, void* workspace,
+                                size_t workspace_bytes, hipStream_t stream) {
+  if (!pop || n < 1 || K < 0 || iters < 0 || !workspace) return SIDON_ERR_ARG;
+  if (!(pStart >= 1.0) || !(pEnd >= 1.0) || !(lrStart > 0.0) || !(lrEnd > 0.0) ||
+      (mode != 0 && mode != 1))
+    return SIDON_ERR_ARG;
+  if (K == 0 || iters == 0) return SIDON_OK;
+  if (workspace_bytes < ws_bytes(n, K)) {
+    set_last_error_msg("polish: workspace smaller than sidon_polish_f64_workspace(n, K)");
+    return SIDON_ERR_ARG;
+  }
+  const Shape s = shape_of(n);
+  const Ws w = ws_of(workspace, n, K);
+  Sched S;
+  S.iters = iters;
+  S.pStart = pStart;
+  S.lnPr = log(pEnd / pStart);
+  S.lrStart = lrStart;
+  S.lnLr = log(lrEnd / lrStart);
+  S.beta1 = beta1;
+  S.beta2 = beta2;
+  const int setup_x = (s.LA + 255) / 256 < 64 ? (s.LA + 255) / 256 : 64;
+  hipLaunchKernelGGL(k_polish_setup, dim3(setup_x, K), dim3(256), 0, stream, pop, n, w);
+  SIDON_HIP_TRY(hipGetLastError());
+  // conv units: tiles meeting a chunk number at most NJ + 35
+  const dim3 gconv((s.NJ + 35 + PU - 1) / PU, s.NCc, K);
+  const dim3 gcorr((s.NTg + PU - 1) / PU, s.NCg, K);
+  const dim3 gred(rblocks(s), K);
+  for (int it = 0; it < iters; ++it) {
+    hipLaunchKernelGGL(k_polish_conv_units, gconv, dim3(PU), 0, stream, s, w);
+    hipLaunchKernelGGL(k_polish_conv_reduce, gred, dim3(RB), 0, stream, s, w);
+    hipLaunchKernelGGL(k_polish_weights, gred, dim3(RB), 0, stream, s, w, S);
+    hipLaunchKernelGGL(k_polish_corr_units, gcorr, dim3(PU), 0, stream, s, w);
+    hipLaunchKernelGGL(k_polish_update, dim3(K), dim3(UB), 0, stream, s, w, S, mode, pop);
+    SIDON_HIP_TRY(hipGetLastError());
+  }
+  if (mom || vel) {
+    hipLaunchKernelGGL(k_polish_moments, dim3(64), dim3(256), 0, stream, w, n, K, mom, vel);
+    SIDON_HIP_TRY(hipGetLastError());
+  }
+  return SIDON_OK;
+}
