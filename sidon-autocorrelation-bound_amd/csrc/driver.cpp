@@ -163,6 +163,7 @@ struct Args {
   int P = 512, islands = 8, iters = 200, gens = 3, elite = 2;
   int fw_rounds = 50, cp_iters = 2000, nT = 12;
   int cold_iters = 1000, search_fw = 0, fw_iters = 500;
+  int polish_top = 8, polish_iters = 1200;
   double pmax = 2e4;
   float sigma = 0.05f, t_max = 0.30f;
   double seconds = 60.0;
@@ -439,6 +440,67 @@ int ladder(const Args& A) {
   return 0;
 }
 
+// ---------------------------------------------------------- fp64 polish --
+// The reference's polish of the incumbent (src/main.cu:798-846): K candidates
+// (the incumbent and K-1 copies with bounded log-normal kicks, 0.003 (1 + k%8)),
+// six fp64 passes of rising exponent p 200 4^q -> 30x that and falling step
+// 0.01 0.45^q -> 2 % of that, the best re-evaluated in fp64 after each pass.
+// Here the passes run through sidon_polish_f64, which spreads the K fp64
+// searches over the whole device at any n.
+void polish(const Args& A, std::vector<double>& bestA, double& bestF) {
+  const int n = (int)bestA.size(), K = std::max(1, A.polish_top);
+  std::mt19937_64 r2(A.seed ^ 0xABCDEFull);
+  std::normal_distribution<double> N01(0.0, 1.0);
+  std::vector<double> h((size_t)n * K);
+  for (int k = 0; k < K; ++k) {
+    double S = 0.0;
+    for (int i = 0; i < n; ++i) {
+      double x = bestA[i];
+      if (k) x *= std::exp(0.003 * (1 + (k % 8)) * N01(r2));
+      h[(size_t)k * n + i] = std::max(0.0, x);
+      S += std::max(0.0, x);
+    }
+    for (int i = 0; i < n; ++i) h[(size_t)k * n + i] /= S;
+  }
+  double* pop = dalloc<double>(h.size());
+  double* F = dalloc<double>(K);
+  const size_t wsb = sidon_polish_f64_workspace(n, K);
+  void* ws = nullptr;
+  CHECK_HIP(hipMalloc(&ws, wsb));
+  CHECK_HIP(hipMemcpy(pop, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  hipStream_t s;
+  CHECK_HIP(hipStreamCreate(&s));
+  const int passes = 6, per = std::max(1, A.polish_iters / passes);
+  const auto t0 = Clock::now();
+  std::vector<double> f(K);
+  for (int q = 0; q < passes; ++q) {
+    const double pA = 200.0 * std::pow(4.0, q), pB = 30.0 * pA;
+    const double lA = 0.010 * std::pow(0.45, q), lB = 0.02 * lA;
+    CHECK_SIDON(sidon_polish_f64(pop, nullptr, nullptr, n, K, per, pA, pB, lA, lB, 0.9, 0.999, 0,
+                                 ws, wsb, s));
+    CHECK_SIDON(sidon_eval_f64_d(pop, n, K, F, nullptr, 1e-6, s));
+    CHECK_HIP(hipMemcpyAsync(f.data(), F, K * sizeof(double), hipMemcpyDeviceToHost, s));
+    CHECK_HIP(hipStreamSynchronize(s));
+    const int bk = (int)(std::min_element(f.begin(), f.end()) - f.begin());
+    if (f[bk] < bestF) {
+      bestF = f[bk];
+      CHECK_HIP(hipMemcpy(bestA.data(), pop + (size_t)bk * n, n * sizeof(double),
+                          hipMemcpyDeviceToHost));
+      save_vec(A.out + ".txt", bestA, bestF);
+    }
+    std::printf("{\"polish_pass\": %d, \"p\": [%.0f, %.0f], \"best_F\": %.15f}\n", q, pA, pB,
+                f[bk]);
+  }
+  const double sec = seconds_since(t0);
+  std::printf("{\"polish\": {\"n\": %d, \"candidates\": %d, \"iters\": %d, \"seconds\": %.2f, "
+              "\"candidate_steps_per_s\": %.1f, \"best_F\": %.15f}}\n",
+              n, K, per * passes, sec, (double)K * per * passes / sec, bestF);
+  CHECK_HIP(hipFree(ws));
+  CHECK_HIP(hipFree(pop));
+  CHECK_HIP(hipFree(F));
+  CHECK_HIP(hipStreamDestroy(s));
+}
+
 // --------------------------------------------------------- policy search --
 // The reference's search loop (src/main.cu:445-451 level budgets, :586-769 the
 // generation) over the C ABI: each generation runs the three anneal tiers of
@@ -628,6 +690,8 @@ int search(const Args& A) {
     }
   }
   sidon_policy_destroy(pol);
+  CHECK_HIP(hipFree(pop)); CHECK_HIP(hipFree(nxt));
+  if (A.polish_iters > 0 && !bestA.empty()) polish(A, bestA, bestF);
   const double hf = host_F(bestA);
   save_vec(A.out + ".txt", bestA, std::max(bestF, hf));
   save_certificate(A.out + ".json", bestA, hf);
@@ -742,6 +806,7 @@ void usage() {
       "      --cold-iters 1000 --pmax 20000 --search-fw 0 --fw-iters 500 --gens 4\n"
       "      --seconds 60 --out best   (writes best.txt and best.json)\n"
       "      --resume FILE             (a saved vector: adopted as the incumbent)\n"
+      "      --polish-top 8 --polish-iters 1200   (fp64 polish of the incumbent; 0 = off)\n"
       "  --search-graph             ladder search on the device generation graph (op 2)\n"
       "  --seed S  --no-tune\n");
 }
@@ -778,6 +843,8 @@ int main(int argc, char** argv) {
     else if (s == "--seed") A.seed = std::strtoull(next(), nullptr, 10);
     else if (s == "--out") A.out = next();
     else if (s == "--resume") A.resume = next();
+    else if (s == "--polish-top") A.polish_top = std::atoi(next());
+    else if (s == "--polish-iters") A.polish_iters = std::atoi(next());
     else if (s == "--no-tune") A.tune = false;
     else { usage(); return 2; }
   }

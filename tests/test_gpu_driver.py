@@ -43,6 +43,8 @@ def test_policy_search_improves_and_writes_verifiable_files(O, tmp_path):
     assert all(r["generations"] >= 12 for r in levels)
     assert len(levels[0]["op_weights"]) == 14 and levels[0]["cells"] > 1
     assert stats[0]["cold"] > 0                       # fresh seeds take the cold tier
+    polish = [r["polish"] for r in recs if "polish" in r]
+    assert polish and polish[0]["best_F"] <= levels[-1]["best_F"] + 1e-12   # fp64 polish
     assert final["best_F"] < 1.56 and final["n"] in (128, 256)
     # the saved vector verifies, and its stored F is the hostF of its weights
     v = O.verify_file(str(base) + ".txt")
