@@ -264,6 +264,26 @@ def main() -> int:
         plain_value = k * iters / (ev0.elapsed_time(ev1) / 1e3)
         same = bool(torch.equal(plain, pop[:k]))
 
+    # whole-device single candidate: the fp64 search with both products cut
+    # into work units for every CU (sidon_polish_f64), same start and schedule
+    single = None
+    if k > 0:
+        one = start[:1].double().contiguous()
+        ws = torch.empty(S.lib().sidon_polish_f64_workspace(n, 1), dtype=torch.uint8,
+                         device="cuda")
+        S.polish_f64(one.clone(), 2, *sched, workspace=ws)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        S.polish_f64(one, iters, *sched, workspace=ws)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        single = {"value": iters / (ev0.elapsed_time(ev1) / 1e3), "unit": "candidate-steps/s",
+                  "form": "one candidate over the whole device in fp64 (sidon_polish_f64: both "
+                          "products cut into (tile, chunk) work units, six kernels a step)",
+                  "dtype": "f64"}
+
     valu_ceiling, mfma_ceiling = S.measure_fp32_ceilings()
 
     F, _ = S.eval_f64(pop)
@@ -308,6 +328,7 @@ def main() -> int:
                   "form": "one candidate per launch (grid = 1 workgroup), launched back to "
                           "back on one stream", "bitwise_equal_to_batched": same},
         "batched_over_plain": value / plain_value if plain_value else None,
+        "single_candidate_whole_device": single,
         "engine_ceilings": {"valu_fp32_tflops": valu_ceiling, "mfma_fp32_tflops": mfma_ceiling,
                             "note": "register-only FMA kernels on every CU, this build's flags "
                                     "(packed FP32 off); the local search runs on the matrix "

@@ -5,7 +5,7 @@
 // elite, passes of rising exponent) and src/kernels.cuh:177-289 (the step).
 // The fused fp64 local search keeps a, b/c and G in one workgroup's LDS and so
 // stops near n = 4600 (3n doubles do not fit 160 KB at n = 8400).  Here one
-// step is five kernels over global memory (L2-resident: ~3n doubles per
+// step is six kernels over global memory (L2-resident: ~3n doubles per
 // candidate), and both products are cut into work units that every CU can take:
 //
 //   unit (candidate, tile q of 16 outputs, chunk cc of 32 blocks of 16 i)
@@ -17,8 +17,9 @@
 // LDS with one unused slot per 16 (lanes one tile apart read 17 doubles apart).
 // The next kernel adds a tile's chunk partials in chunk order, so the sums are
 // deterministic.  Step: conv units -> conv reduce (b, M) -> weights (c, R
-// partials) -> corr units -> update (G, R, Adam or mirror step, clamp >= 0,
-// renormalise), the same arithmetic as the fused kernel's step in fp64.
+// partials) -> corr units -> step (G, R, Adam or mirror step, clamp >= 0,
+// S partials) -> renormalise, the same arithmetic as the fused kernel's step
+// in fp64.
 #include <cmath>
 
 #include "common.hpp"
@@ -31,7 +32,6 @@ constexpr int PU = 256;     // tiles (lanes) per unit workgroup
 constexpr int CH = 32;      // 16-blocks per chunk
 constexpr int PADA = 576;   // zero heights each side of the padded copy of a
 constexpr int RB = 256;     // threads per reduce / weights workgroup
-constexpr int UB = 1024;    // threads of the per-candidate update workgroup
 
 struct Shape {
   int n, NJ, NTc, NCc, NTg, NCg, LA, LB;
@@ -52,20 +52,22 @@ __host__ __device__ inline Shape shape_of(int n) {
 
 // workspace layout (doubles unless noted), per candidate:
 //   apad[LA], bc[LB], part[NCc * NTc * 16], gpart[NCg * NTg * 16], mom[n], vel[n],
-//   rpart[ceil(LB / RB)], then K u64 max bits, K ints: step counter, stopped.
+//   rpart[ceil(LB / RB)], spart[ceil(n / RB)], bias[2] (beta1^t, beta2^t), then K u64 max
+//   bits, K ints: step counter, stopped.
 struct Ws {
-  double *apad, *bc, *part, *gpart, *mom, *vel, *rpart;
+  double *apad, *bc, *part, *gpart, *mom, *vel, *rpart, *spart, *bias;
   unsigned long long* mbits;
   int *step, *stopped;
-  long long sa, sb, sp, sg, sn, sr;  // per-candidate strides
+  long long sa, sb, sp, sg, sn, sr, ss;  // per-candidate strides
 };
 
 __host__ __device__ inline int rblocks(const Shape& s) { return (s.LB + RB - 1) / RB; }
+__host__ __device__ inline int sblocks(const Shape& s) { return (s.n + RB - 1) / RB; }
 
 __host__ __device__ inline size_t ws_bytes(int n, int K) {
   const Shape s = shape_of(n);
   const size_t per = (size_t)s.LA + s.LB + (size_t)s.NCc * s.NTc * 16 + (size_t)s.NCg * s.NTg * 16 +
-                     2 * (size_t)n + rblocks(s);
+                     2 * (size_t)n + rblocks(s) + sblocks(s) + 2;
   return per * K * sizeof(double) + K * (sizeof(unsigned long long) + 2 * sizeof(int)) + 256;
 }
 
@@ -78,6 +80,7 @@ __host__ __device__ inline Ws ws_of(void* base, int n, int K) {
   w.sg = (long long)s.NCg * s.NTg * 16;
   w.sn = n;
   w.sr = rblocks(s);
+  w.ss = sblocks(s);
   double* d = reinterpret_cast<double*>(base);
   w.apad = d; d += w.sa * K;
   w.bc = d; d += w.sb * K;
@@ -86,6 +89,8 @@ __host__ __device__ inline Ws ws_of(void* base, int n, int K) {
   w.mom = d; d += w.sn * K;
   w.vel = d; d += w.sn * K;
   w.rpart = d; d += w.sr * K;
+  w.spart = d; d += w.ss * K;
+  w.bias = d; d += 2 * K;
   w.mbits = reinterpret_cast<unsigned long long*>(d);
   w.step = reinterpret_cast<int*>(w.mbits + K);
   w.stopped = w.step + K;
@@ -124,6 +129,8 @@ __global__ void k_polish_setup(const double* __restrict__ pop, int n, Ws w) {
     w.mbits[cand] = 0ull;
     w.step[cand] = 0;
     w.stopped[cand] = 0;
+    w.bias[2 * cand] = 1.0;
+    w.bias[2 * cand + 1] = 1.0;
   }
 }
 
@@ -293,39 +300,37 @@ __global__ __launch_bounds__(PU) void k_polish_corr_units(Shape s, Ws w) {
 }
 
 // --------------------------------------------------------------- update --
-// One workgroup per candidate: R, G_i (chunk order), g = G/R - M, Adam step
-// (lr/n) mhat / (sqrt(vhat) + 1e-30) and clamp >= 0 (mode 0) or mirror step
-// a exp(clamp(-lr g / M, +-4)) (mode 1), renormalise sum a = 1 (uniform if not
-// positive), M back to 0, step + 1.
-__global__ __launch_bounds__(UB) void k_polish_update(Shape s, Ws w, Sched S, int mode,
-                                                      double* __restrict__ pop) {
-  constexpr int NW = UB / kWave;
-  __shared__ double slots[2 * NW];
-  const int cand = blockIdx.x;
+// Many workgroups per candidate.  First: R (rpart in order), G_i (chunks in
+// order), g = G/R - M, the Adam step (lr/n) mhat / (sqrt(vhat) + 1e-30) and
+// clamp >= 0 (mode 0) or the mirror step a exp(clamp(-lr g / M, +-4)) (mode
+// 1) into `out`, partial sums of the new heights per workgroup.  The bias
+// corrections use beta^t kept in the workspace, multiplied once a step as the
+// fused kernel does.
+__global__ __launch_bounds__(RB) void k_polish_step(Shape s, Ws w, Sched S, int mode,
+                                                    double* __restrict__ pop) {
+  constexpr int NW = RB / kWave;
+  __shared__ double slots[NW];
+  const int cand = blockIdx.y;
   if (w.stopped[cand]) return;
   const int n = s.n;
   const double M = __longlong_as_double((long long)w.mbits[cand]);
   double R = 0.0;
   for (int b = 0; b < w.sr; ++b) R += w.rpart[w.sr * cand + b];
-  const int it = w.step[cand];
   double p, lr;
-  sched_at(S, it, p, lr);
-  double b1t = 1.0, b2t = 1.0;
-  for (int k = 0; k <= it; ++k) { b1t *= S.beta1; b2t *= S.beta2; }
+  sched_at(S, w.step[cand], p, lr);
+  const double b1t = w.bias[2 * cand] * S.beta1, b2t = w.bias[2 * cand + 1] * S.beta2;
   const double bc1 = 1.0 / (1.0 - b1t), bc2 = 1.0 / (1.0 - b2t);
-  double* ap = w.apad + w.sa * cand + PADA;
-  double* mo = w.mom + w.sn * cand;
-  double* ve = w.vel + w.sn * cand;
-  const double* gp = w.gpart + w.sg * cand;
-  double* out = pop + (size_t)cand * n;
-  double sloc = 0.0;
-  for (int i = threadIdx.x; i < n; i += UB) {
+  const int i = blockIdx.x * RB + threadIdx.x;
+  double xx = 0.0;
+  if (i < n) {
+    const double* gp = w.gpart + w.sg * cand;
     double G = 0.0;
     for (int cc = 0; cc < s.NCg; ++cc) G += gp[((size_t)cc * s.NTg + (i >> 4)) * 16 + (i & 15)];
     const double gi = G / R - M;
-    const double ai = ap[i];
-    double xx;
+    const double ai = w.apad[w.sa * cand + PADA + i];
     if (mode == 0) {
+      double* mo = w.mom + w.sn * cand;
+      double* ve = w.vel + w.sn * cand;
       const double mm = S.beta1 * mo[i] + (1.0 - S.beta1) * gi;
       const double vv = S.beta2 * ve[i] + (1.0 - S.beta2) * gi * gi;
       mo[i] = mm;
@@ -339,22 +344,38 @@ __global__ __launch_bounds__(UB) void k_polish_update(Shape s, Ws w, Sched S, in
       if (e < -4.0) e = -4.0;
       xx = ai * exp(e);
     }
-    out[i] = xx;
-    sloc += xx;
+    pop[(size_t)cand * n + i] = xx;
   }
-  double Ssum = block_sum<double, NW>(sloc, slots);
+  const double part = block_sum<double, NW>(xx, slots);
+  if (threadIdx.x == 0) w.spart[w.ss * cand + blockIdx.x] = part;
+}
+
+// Second: S (spart in order), renormalise sum a = 1 (uniform 1/n if S is not
+// positive) into pop and the padded copy; the last workgroup's thread 0
+// resets M and advances the step and the bias products.
+__global__ __launch_bounds__(RB) void k_polish_renorm(Shape s, Ws w, Sched S,
+                                                      double* __restrict__ pop) {
+  const int cand = blockIdx.y;
+  if (w.stopped[cand]) return;
+  const int n = s.n;
+  double Ssum = 0.0;
+  for (int b = 0; b < w.ss; ++b) Ssum += w.spart[w.ss * cand + b];
   const bool reset = !(Ssum > 0.0);
-  if (reset) Ssum = 1.0;
-  const double inv = 1.0 / Ssum;
-  __syncthreads();  // every thread has written its heights to `out`
-  for (int i = threadIdx.x; i < n; i += UB) {
+  const double inv = reset ? 1.0 : 1.0 / Ssum;
+  const int i = blockIdx.x * RB + threadIdx.x;
+  if (i < n) {
+    double* out = pop + (size_t)cand * n;
     const double x = reset ? 1.0 / (double)n : out[i] * inv;
     out[i] = x;
-    ap[i] = x;
+    w.apad[w.sa * cand + PADA + i] = x;
   }
-  if (threadIdx.x == 0) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // every workgroup of this candidate has read M and the step in the
+    // previous kernel; this one reads neither
     w.mbits[cand] = 0ull;
-    w.step[cand] = it + 1;
+    w.step[cand] += 1;
+    w.bias[2 * cand] *= S.beta1;
+    w.bias[2 * cand + 1] *= S.beta2;
   }
 }
 
@@ -407,12 +428,14 @@ extern "C" int sidon_polish_f64(double* pop, double* mom, double* vel, int n, in
   const dim3 gconv((s.NJ + 35 + PU - 1) / PU, s.NCc, K);
   const dim3 gcorr((s.NTg + PU - 1) / PU, s.NCg, K);
   const dim3 gred(rblocks(s), K);
+  const dim3 gupd(sblocks(s), K);
   for (int it = 0; it < iters; ++it) {
     hipLaunchKernelGGL(k_polish_conv_units, gconv, dim3(PU), 0, stream, s, w);
     hipLaunchKernelGGL(k_polish_conv_reduce, gred, dim3(RB), 0, stream, s, w);
     hipLaunchKernelGGL(k_polish_weights, gred, dim3(RB), 0, stream, s, w, S);
     hipLaunchKernelGGL(k_polish_corr_units, gcorr, dim3(PU), 0, stream, s, w);
-    hipLaunchKernelGGL(k_polish_update, dim3(K), dim3(UB), 0, stream, s, w, S, mode, pop);
+    hipLaunchKernelGGL(k_polish_step, gupd, dim3(RB), 0, stream, s, w, S, mode, pop);
+    hipLaunchKernelGGL(k_polish_renorm, gupd, dim3(RB), 0, stream, s, w, S, pop);
     SIDON_HIP_TRY(hipGetLastError());
   }
   if (mom || vel) {
