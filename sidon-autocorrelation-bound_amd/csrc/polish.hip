@@ -4,7 +4,7 @@
 // Reference: src/main.cu:798-846 (fp64 polish: kLocalSearch<double> over the
 // elite, passes of rising exponent) and src/kernels.cuh:177-289 (the step).
 // The fused fp64 local search keeps a, b/c and G in one workgroup's LDS and so
-// stops near n = 4600 (3n doubles do not fit 160 KB at n = 8400).  Here one
+// stops at n = 4560 (3n doubles do not fit 160 KB at n = 8400).  Here one
 // step is six kernels over global memory (L2-resident: ~3n doubles per
 // candidate), and both products are cut into work units that every CU can take:
 //
