@@ -78,3 +78,28 @@ def test_resume_adopts_the_vector_and_drops_coarser_levels(O, certs, tmp_path):
     assert final["host_recheck"] <= O.F_hostorder(a) + 1e-14   # never worse than it was handed
     v = O.verify_file(str(base) + ".txt")
     assert v["rc"] == 0 and v["n"] == final["n"]
+
+
+def test_uneven_islands_graph():
+    """Config 3 in small: uneven lengths, generations replayed from a HIP graph."""
+    out = run(["--uneven", "--P", "64", "--islands", "4", "--iters", "40", "--gens", "2",
+               "--lengths", "128,256,600", "--no-tune"])
+    rec = json.loads(out.strip().splitlines()[-1])
+    assert rec["generations"] == 2 and rec["ms_per_generation"] > 0
+    assert rec["candidate_steps_per_s"] > 0 and 1.0 < rec["best_F"] < 2.5
+
+
+def test_ladder_splits_preserve_F_exactly():
+    """Config 5 in small: Frank-Wolfe rounds per level, every n -> 2n split keeps
+    the correctly rounded F bit for bit (and certifies the rounding)."""
+    out = run(["--ladder", "--P", "32", "--levels", "256,512,1024", "--fw-rounds", "1",
+               "--cp-iters", "200"])
+    recs = [json.loads(line) for line in out.splitlines() if line.startswith("{")]
+    splits = [r for r in recs if "split" in r]
+    levels = [r for r in recs if "level" in r]
+    assert [r["level"] for r in levels] == [256, 512, 1024]
+    assert len(splits) == 2
+    for r in splits:
+        assert r["F_bit_exact"] == r["of"] == 32 and r["ambiguous"] == 0
+    for r in levels:
+        assert r["mean_F_after"] <= r["mean_F_before"] + 1e-12
