@@ -52,10 +52,10 @@ __host__ __device__ inline Shape shape_of(int n) {
 
 // workspace layout (doubles unless noted), per candidate:
 //   apad[LA], bc[LB], part[NCc * NTc * 16], gpart[NCg * NTg * 16], mom[n], vel[n],
-//   rpart[ceil(LB / RB)], spart[ceil(n / RB)], bias[2] (beta1^t, beta2^t), then K u64 max
-//   bits, K ints: step counter, stopped.
+//   rpart[ceil(LB / RB)], mpart[ceil(LB / RB)], spart[ceil(n / RB)], bias[2] (beta1^t,
+//   beta2^t), then K u64 max bits, K ints: step counter, stopped.
 struct Ws {
-  double *apad, *bc, *part, *gpart, *mom, *vel, *rpart, *spart, *bias;
+  double *apad, *bc, *part, *gpart, *mom, *vel, *rpart, *mpart, *spart, *bias;
   unsigned long long* mbits;
   int *step, *stopped;
   long long sa, sb, sp, sg, sn, sr, ss;  // per-candidate strides
@@ -67,7 +67,7 @@ __host__ __device__ inline int sblocks(const Shape& s) { return (s.n + RB - 1) /
 __host__ __device__ inline size_t ws_bytes(int n, int K) {
   const Shape s = shape_of(n);
   const size_t per = (size_t)s.LA + s.LB + (size_t)s.NCc * s.NTc * 16 + (size_t)s.NCg * s.NTg * 16 +
-                     2 * (size_t)n + rblocks(s) + sblocks(s) + 2;
+                     2 * (size_t)n + 2 * rblocks(s) + sblocks(s) + 2;
   return per * K * sizeof(double) + K * (sizeof(unsigned long long) + 2 * sizeof(int)) + 256;
 }
 
@@ -89,6 +89,7 @@ __host__ __device__ inline Ws ws_of(void* base, int n, int K) {
   w.mom = d; d += w.sn * K;
   w.vel = d; d += w.sn * K;
   w.rpart = d; d += w.sr * K;
+  w.mpart = d; d += w.sr * K;
   w.spart = d; d += w.ss * K;
   w.bias = d; d += 2 * K;
   w.mbits = reinterpret_cast<unsigned long long*>(d);
@@ -111,6 +112,20 @@ __device__ __forceinline__ void sched_at(const Sched& S, int it, double& p, doub
 
 // LDS slot of window element e (one unused slot after every 16)
 __device__ __forceinline__ int wslot(int e) { return e + (e >> 4); }
+
+// sum of p[c * stride] for c = c0 .. c1 in ascending c, loaded eight at a time
+// so the loads are in flight together (absent terms add an exact zero)
+__device__ __forceinline__ double chunk_sum(const double* p, long long stride, int c0, int c1) {
+  double r = 0.0;
+  for (int c = c0; c <= c1; c += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = c + u <= c1 ? p[(long long)(c + u) * stride] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) r += v[u];
+  }
+  return r;
+}
 
 // ---------------------------------------------------------------- setup --
 __global__ void k_polish_setup(const double* __restrict__ pop, int n, Ws w) {
@@ -205,9 +220,11 @@ __global__ __launch_bounds__(PU) void k_polish_conv_units(Shape s, Ws w) {
 }
 
 // ---------------------------------------------------------- conv reduce --
-// b_k = sum of the chunk partials of tile k/16 in chunk order; the maximum
-// rises by a 64-bit atomic max on the bits of the non-negative value.
+// b_k = sum of the chunk partials of tile k/16 in chunk order; each
+// workgroup's maximum goes to mpart (the weights take the max of those).
 __global__ __launch_bounds__(RB) void k_polish_conv_reduce(Shape s, Ws w) {
+  constexpr int NW = RB / kWave;
+  __shared__ double slots[NW];
   const int cand = blockIdx.y;
   if (w.stopped[cand]) return;
   const int k = blockIdx.x * RB + threadIdx.x;
@@ -216,14 +233,13 @@ __global__ __launch_bounds__(RB) void k_polish_conv_reduce(Shape s, Ws w) {
     const int q = k >> 4, t = k & 15;
     int J0, J1;
     conv_tile_range(s.n, q, J0, J1);
-    const double* p = w.part + w.sp * cand + t;
-    for (int cc = J0 / CH; cc <= J1 / CH; ++cc) b += p[((size_t)cc * s.NTc + q) * 16];
+    b = chunk_sum(w.part + w.sp * cand + (size_t)q * 16 + t, (long long)s.NTc * 16, J0 / CH,
+                  J1 / CH);
     if (k >= 2 * s.n - 1) b = 0.0;
   }
   if (k < s.LB) w.bc[w.sb * cand + k] = b;
-  const double m = wave_max(b);
-  if ((threadIdx.x & (kWave - 1)) == 0)
-    atomicMax(w.mbits + cand, (unsigned long long)__double_as_longlong(m));
+  const double m = block_max<double, NW>(b, slots);
+  if (threadIdx.x == 0) w.mpart[w.sr * cand + blockIdx.x] = m;
 }
 
 // -------------------------------------------------------------- weights --
@@ -232,9 +248,18 @@ __global__ __launch_bounds__(RB) void k_polish_conv_reduce(Shape s, Ws w) {
 __global__ __launch_bounds__(RB) void k_polish_weights(Shape s, Ws w, Sched S) {
   constexpr int NW = RB / kWave;
   __shared__ double slots[NW];
+  __shared__ double mslots[NW];
   const int cand = blockIdx.y;
   if (w.stopped[cand]) return;
-  const double M = __longlong_as_double((long long)w.mbits[cand]);
+  // M = the max of the reduce workgroups' maxima (every workgroup forms it)
+  double mloc = 0.0;
+  for (int b = threadIdx.x; b < w.sr; b += RB) {
+    const double v = w.mpart[w.sr * cand + b];
+    mloc = v > mloc ? v : mloc;
+  }
+  const double M = block_max<double, NW>(mloc, mslots);
+  if (blockIdx.x == 0 && threadIdx.x == 0)  // for the step kernel
+    w.mbits[cand] = (unsigned long long)__double_as_longlong(M);
   if (!(M > 0.0)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) w.stopped[cand] = 1;  // the fused kernel's break
     return;
@@ -323,9 +348,8 @@ __global__ __launch_bounds__(RB) void k_polish_step(Shape s, Ws w, Sched S, int 
   const int i = blockIdx.x * RB + threadIdx.x;
   double xx = 0.0;
   if (i < n) {
-    const double* gp = w.gpart + w.sg * cand;
-    double G = 0.0;
-    for (int cc = 0; cc < s.NCg; ++cc) G += gp[((size_t)cc * s.NTg + (i >> 4)) * 16 + (i & 15)];
+    const double G = chunk_sum(w.gpart + w.sg * cand + (size_t)(i >> 4) * 16 + (i & 15),
+                               (long long)s.NTg * 16, 0, s.NCg - 1);
     const double gi = G / R - M;
     const double ai = w.apad[w.sa * cand + PADA + i];
     if (mode == 0) {
