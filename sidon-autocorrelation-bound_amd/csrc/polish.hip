@@ -8,12 +8,12 @@
 // step is six kernels over global memory (L2-resident: ~3n doubles per
 // candidate), and both products are cut into work units that every CU can take:
 //
-//   unit (candidate, tile q of 16 outputs, chunk cc of 32 blocks of 16 i)
-//     = one lane, 32 trips of 16 x 16 fp64 FMAs, partial sums to `part`;
+//   unit (candidate, tile q of 16 outputs, chunk cc of 16 blocks of 16 i)
+//     = one lane, 16 trips of 16 x 16 fp64 FMAs, partial sums to `part`;
 //
 // a workgroup holds 256 consecutive tiles of one chunk, so every lane runs the
-// same 32 trips (products outside a tile's range read zero padding), the
-// chunk's 512 heights are broadcast and the other operand's window sits in
+// same 16 trips (products outside a tile's range read zero padding), the
+// chunk's 256 heights are broadcast and the other operand's window sits in
 // LDS with one unused slot per 16 (lanes one tile apart read 17 doubles apart).
 // The next kernel adds a tile's chunk partials in chunk order, so the sums are
 // deterministic.  Step: conv units -> conv reduce (b, M) -> weights (c, R
@@ -29,7 +29,7 @@ namespace sidon {
 namespace {
 
 constexpr int PU = 256;     // tiles (lanes) per unit workgroup
-constexpr int CH = 32;      // 16-blocks per chunk
+constexpr int CH = 16;      // 16-blocks per chunk
 constexpr int PADA = 576;   // zero heights each side of the padded copy of a
 constexpr int RB = 256;     // threads per reduce / weights workgroup
 
@@ -161,7 +161,7 @@ __device__ __forceinline__ void conv_tile_range(int n, int q, int& J0, int& J1) 
 
 __device__ void conv_chunk_tiles(const Shape& s, int cc, int& qlo, int& qhi) {
   // J1(q) and J0(q) do not decrease with q: binary searches
-  int lo = 0, hi = s.NTc;  // first q with J1(q) >= 32cc
+  int lo = 0, hi = s.NTc;  // first q with J1(q) >= CH cc
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
     int a, b;
@@ -170,7 +170,7 @@ __device__ void conv_chunk_tiles(const Shape& s, int cc, int& qlo, int& qhi) {
   }
   qlo = lo;
   lo = 0;
-  hi = s.NTc;  // first q with J0(q) > 32cc + 31
+  hi = s.NTc;  // first q with J0(q) > CH cc + CH - 1
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
     int a, b;
@@ -181,7 +181,7 @@ __device__ void conv_chunk_tiles(const Shape& s, int cc, int& qlo, int& qhi) {
 }
 
 __global__ __launch_bounds__(PU) void k_polish_conv_units(Shape s, Ws w) {
-  __shared__ double sx[CH * 16 + CH];             // the chunk's 512 heights (padded)
+  __shared__ double sx[CH * 16 + CH];             // the chunk's 16 CH heights (padded)
   __shared__ double sw[(16 * (PU + CH) + 32) * 17 / 16 + 16];  // a window (padded)
   const int cand = blockIdx.z, cc = blockIdx.y;
   int qlo, qhi;
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(PU) void k_polish_conv_units(Shape s, Ws w) {
   const double* ap = w.apad + w.sa * cand + PADA;  // ap[i] = a_i, zero for -PADA <= i < n + PADA
   const int J0c = CH * cc;
   for (int e = threadIdx.x; e < CH * 16; e += PU) sx[wslot(e)] = ap[16 * J0c + e];
-  // window: a_x for x in [16 (qa - J0c - 31) - 15, 16 (qb - J0c) + 15]
+  // window: a_x for x in [16 (qa - J0c - CH + 1) - 15, 16 (qb - J0c) + 15]
   const int wlo = 16 * (qa - J0c - (CH - 1)) - 15;
   const int wlen = 16 * (qb - qa + CH) + 15;
   for (int e = threadIdx.x; e < wlen; e += PU) sw[wslot(e)] = ap[wlo + e];
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(PU) void k_polish_corr_units(Shape s, Ws w) {
   const double* cp = w.bc + w.sb * cand;
   const int J0c = CH * cc;
   for (int e = threadIdx.x; e < CH * 16; e += PU) sx[wslot(e)] = ap[16 * J0c + e];
-  // window: c_x for x in [16 (qa + J0c), 16 (qb + J0c + 31) + 30]
+  // window: c_x for x in [16 (qa + J0c), 16 (qb + J0c + CH - 1) + 30]
   const int wlo = 16 * (qa + J0c);
   const int wlen = 16 * (qb - qa + CH) + 15;
   for (int e = threadIdx.x; e < wlen; e += PU) sw[wslot(e)] = cp[wlo + e];
