@@ -1,17 +1,23 @@
 // driver.cpp -- C++ host driver over the C ABI (include/sidon.h).
 //
 // The counterpart of the reference's sidon.exe host side (src/main.cu:40-876)
-// for the paths this library implements.  Modes:
+// for the paths this library implements.  It takes the reference's command
+// line (src/main.cu:41-82, :357-381: the same flags, meanings and defaults;
+// unknown options exit 1) and runs, without a mode flag, the reference's
+// search.  Modes:
+//   (none) / --search        the reference's search loop: three-tier local search,
+//                            Frank-Wolfe rounds, fp64 evaluation, support signatures,
+//                            the island policy (tabu, operator weights, migration), all
+//                            fourteen operators, n -> 2n split per level, fp64 polish,
+//                            best vector and certificate out
 //   --selftest               known-answer tests of src/main.cu:186-291 against the device
+//   --bench                  kernel micro-benchmark (src/main.cu:294-320)
 //   --uneven                 island generations of uneven lengths as HIP graphs
-//   --ladder                 Frank-Wolfe (Chambolle-Pock) rounds + n -> 2n cell split
-//   --search                 the reference's search loop: three-tier local search, fp64
-//                            evaluation, support signatures, the island policy (tabu,
-//                            operator weights, migration), all fourteen operators,
-//                            n -> 2n split per level, best vector out
+//   --fw-ladder              Frank-Wolfe (Chambolle-Pock) rounds + n -> 2n cell split
 //   --search-graph           ladder search on the device generation graph (operator 2 only)
-// Every mode prints one JSON line per result.  All device work goes through
-// the C ABI; host code here only seeds inputs, reads results and writes files.
+// Results are JSON lines; the search also prints the reference's STAT
+// progress records.  All device work goes through the C ABI; host code here
+// only seeds inputs, reads results and writes files.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -158,36 +164,49 @@ void save_certificate(const std::string& path, const std::vector<double>& a, dou
   std::fclose(f);
 }
 
+// Options.  The defaults are the reference's (src/main.cu:41-62), which the
+// search uses; the other modes replace some of them (mode_defaults) unless
+// they are given on the command line.
 struct Args {
-  std::string mode;
-  int P = 512, islands = 8, iters = 200, gens = 3, elite = 2;
-  int fw_rounds = 50, cp_iters = 2000, nT = 12;
-  int cold_iters = 1000, search_fw = 0, fw_iters = 500;
-  int polish_top = 8, polish_iters = 1200;
-  double pmax = 2e4;
+  std::string mode = "--search";
+  std::vector<int> ladder{64, 128, 256, 512, 1024, 2048};  // --ladder
+  int P = 256, islands = 8, elite = 2;                     // --pop --islands --elite
+  int iters = 4000, cold_iters = 20000;                    // --ls-iters --cold-iters
+  double pmax = 2e4;                                       // --pmax
+  int fw_rounds = 6, fw_iters = 500;                       // --fw-rounds --fw-iters (0 rounds = off)
+  int gens = 0;                                            // --gens: cap per level, 0 = time only
+  double seconds = 600.0;                                  // --seconds
+  unsigned long long seed = 20260905ULL;                   // --seed
+  std::string out = "best";                                // --out
+  std::string resume;                                      // --resume
+  int polish_iters = 40000, polish_top = 8;                // --polish-iters --polish-top
+  int verbose = 1;                                         // --quiet sets 0
+  // this driver's own modes
+  int nT = 12;
   float sigma = 0.05f, t_max = 0.30f;
-  double seconds = 60.0;
-  unsigned long long seed = 20260905ULL;
-  std::vector<int> ladder;
   std::vector<int> lengths{600, 1200, 2100, 4096, 8400};
-  std::string out = "best";
-  std::string resume;
   bool tune = true;
 };
 
-std::vector<int> parse_list(const char* s) {
-  std::vector<int> v;
+// Parses "64,128,..." into positive integers; false on anything else.
+bool parse_list(const char* s, std::vector<int>& v) {
+  v.clear();
   std::string cur;
   for (const char* p = s;; ++p) {
     if (*p == ',' || *p == '\0') {
-      if (!cur.empty()) v.push_back(std::atoi(cur.c_str()));
+      if (!cur.empty()) {
+        char* end = nullptr;
+        const long x = std::strtol(cur.c_str(), &end, 10);
+        if (*end || x <= 0 || x > (1L << 24)) return false;
+        v.push_back((int)x);
+      }
       cur.clear();
       if (!*p) break;
     } else {
       cur.push_back(*p);
     }
   }
-  return v;
+  return !v.empty();
 }
 
 // ------------------------------------------------------------ self-test --
@@ -371,8 +390,8 @@ int uneven(const Args& A) {
 }
 
 // ---------------------------------------------- Frank-Wolfe + split ladder --
-int ladder(const Args& A) {
-  const std::vector<int> L = A.ladder.empty() ? std::vector<int>{1050, 2100, 4200, 8400} : A.ladder;
+int fw_ladder(const Args& A) {
+  const std::vector<int>& L = A.ladder;
   const int P = A.P;
   std::mt19937_64 rng(A.seed);
   int n = L[0];
@@ -393,7 +412,7 @@ int ladder(const Args& A) {
     CHECK_HIP(hipEventCreate(&e1));
     CHECK_SIDON(sidon_eval_f64(pop, n, P, F, nullptr, 1e-9, s));
     CHECK_HIP(hipEventRecord(e0, s));
-    const int rc = sidon_frank_wolfe(pop, nullptr, n, P, A.fw_rounds, A.cp_iters, A.nT, A.t_max, s);
+    const int rc = sidon_frank_wolfe(pop, nullptr, n, P, A.fw_rounds, A.fw_iters, A.nT, A.t_max, s);
     CHECK_HIP(hipEventRecord(e1, s));
     if (rc == SIDON_ERR_LDS) {
       std::printf("{\"level\": %d, \"frank_wolfe\": \"does not fit in LDS\"}\n", n);
@@ -412,12 +431,12 @@ int ladder(const Args& A) {
     for (int p = 0; p < P; ++p) { m0 += f0[p] / P; m1 += f1[p] / P; b1 = std::min(b1, f1[p]); }
     // dense count: 4 n^2 FLOP per CP iteration; the dual correlation skips steps whose
     // multipliers are all zero, so this equivalent rate can exceed the matrix peak
-    const double cp_flop = 4.0 * n * (double)n * A.cp_iters * A.fw_rounds * P;
+    const double cp_flop = 4.0 * n * (double)n * A.fw_iters * A.fw_rounds * P;
     std::printf("{\"level\": %d, \"fw_rounds\": %d, \"cp_iters\": %d, \"ms\": %.1f, "
                 "\"cp_iterations_per_s\": %.1f, \"cp_dense_equivalent_tflops\": %.2f, "
                 "\"mean_F_before\": %.9f, "
                 "\"mean_F_after\": %.9f, \"best_F\": %.12f}\n",
-                n, A.fw_rounds, A.cp_iters, ms, (double)A.cp_iters * A.fw_rounds * P / (ms / 1e3),
+                n, A.fw_rounds, A.fw_iters, ms, (double)A.fw_iters * A.fw_rounds * P / (ms / 1e3),
                 cp_flop / (ms / 1e3) / 1e12, m0, m1, b1);
     if (lvl + 1 < L.size()) {
       if (L[lvl + 1] != 2 * n) { std::fprintf(stderr, "ladder must double\n"); return 2; }
@@ -510,11 +529,11 @@ void polish(const Args& A, std::vector<double>& bestA, double& bestF) {
 // plans the offspring (tabu, operator weights, tournaments, migration), the
 // variation applies it, the incumbent is re-seeded every 10 generations and
 // the operator weights are credited every 5.  Levels get 1.85^level shares of
-// the time budget (the reference keeps 18 % for an fp64 polish that is not
-// built here, so the levels get all of it); between levels every candidate is
-// split n -> 2n (F preserved exactly).
+// 82 % of the time budget (the rest is the polish's); a level ends when its
+// share is spent or after --gens generations (0 = no cap); between levels
+// every candidate is split n -> 2n (F preserved exactly).
 int search(const Args& A) {
-  std::vector<int> L = A.ladder.empty() ? std::vector<int>{256, 512, 1024, 2048} : A.ladder;
+  std::vector<int> L = A.ladder;
   // --resume (main.cu:414-443): the saved vector becomes the incumbent at once,
   // and levels coarser than it are dropped (a split preserves F exactly, so
   // they cannot beat it)
@@ -564,7 +583,7 @@ int search(const Args& A) {
   std::vector<double> lvlBudget(L.size());
   double tot = 0.0;
   for (size_t l = 0; l < L.size(); ++l) tot += (lvlBudget[l] = std::pow(1.85, (double)l));
-  for (auto& x : lvlBudget) x = x / tot * A.seconds;
+  for (auto& x : lvlBudget) x = x / tot * A.seconds * 0.82;
   const auto t0 = Clock::now();
   double bestF = 1e30;
   std::vector<double> bestA;
@@ -575,7 +594,8 @@ int search(const Args& A) {
     bestN = (int)seedVec.size();
     CHECK_SIDON(sidon_policy_adopt(pol, bestF));
   }
-  long long lsSteps = 0, evals = 0;
+  long long lsSteps = 0, evals = 0, fwRuns = 0;
+  double flop = 0.0;  // the reference's effort count (main.cu:624-642): 4 n^2 a local-search step
   for (size_t lvl = 0; lvl < L.size(); ++lvl) {
     const int n = L[lvl];
     CHECK_SIDON(sidon_policy_new_level(pol));
@@ -592,7 +612,9 @@ int search(const Args& A) {
     }
     const auto tl = Clock::now();
     int gen = 0;
-    while (gen < A.gens || seconds_since(tl) < lvlBudget[lvl]) {
+    const double n2 = (double)n * n;
+    bool fwOk = true;
+    while (seconds_since(tl) < lvlBudget[lvl] && (A.gens == 0 || gen < A.gens)) {
       // ---- three anneal tiers (main.cu:598-629) ----
       int nCold = 0, nMed = 0;
       CHECK_SIDON(sidon_policy_tiers(curOp.data(), P, tier.data(), &nCold, &nMed));
@@ -603,13 +625,21 @@ int search(const Args& A) {
         CHECK_SIDON(sidon_local_search(pop, nullptr, nullptr, skip, mode, n, P, iters, pa, pb,
                                        la, lb, 0.9f, 0.999f, 0, s));
         for (int p = 0; p < P; ++p) lsSteps += (long long)(hskip[p] == 0) * iters;
+        flop += 4.0 * n2 * (double)iters * (double)(t == 0 ? nCold : t == 1 ? nMed : P - nCold - nMed);
       };
       if (nCold) tier_run(0, A.cold_iters, 6.0f, pTop, 0.085f, 0.0015f);
       if (nMed) tier_run(1, std::max(1, A.cold_iters / 3), 40.0f, pTop, 0.040f, 0.0010f);
       if (nCold + nMed < P) tier_run(2, A.iters, 400.0f, 3.0f * pTop, 0.012f, 0.0003f);
-      if (A.search_fw > 0) {
-        const int rc = sidon_frank_wolfe(pop, nullptr, n, P, A.search_fw, A.fw_iters, 12, 0.30f, s);
-        if (rc != SIDON_ERR_LDS) CHECK_SIDON(rc);
+      if (A.fw_rounds > 0 && fwOk) {  // skipped for good where it does not fit (main.cu:633-639)
+        const int rc = sidon_frank_wolfe(pop, nullptr, n, P, A.fw_rounds, A.fw_iters, A.nT,
+                                         A.t_max, s);
+        if (rc == SIDON_ERR_LDS) {
+          fwOk = false;
+        } else {
+          CHECK_SIDON(rc);
+          fwRuns += (long long)A.fw_rounds * P;
+          flop += (double)P * A.fw_rounds * (4.0 * n2 * A.fw_iters + 26.0 * n2);
+        }
       }
       // ---- evaluate, signatures, policy (main.cu:640-735) ----
       CHECK_SIDON(sidon_eval_f64(pop, n, P, fit, nact, 1e-4, s));
@@ -621,6 +651,7 @@ int search(const Args& A) {
       CHECK_HIP(hipMemcpyAsync(hnact.data(), nact, P * sizeof(int), hipMemcpyDeviceToHost, s));
       CHECK_HIP(hipStreamSynchronize(s));
       evals += P;
+      flop += 2.0 * n2 * P;
       int bi = 0, improved = 0;
       CHECK_SIDON(sidon_policy_observe(pol, hfit.data(), hsig.data(), &bi, &improved));
       if (improved) {
@@ -658,19 +689,25 @@ int search(const Args& A) {
         CHECK_HIP(hipMemcpyAsync(cfit.data(), fit, P * sizeof(double), hipMemcpyDeviceToHost, s));
         CHECK_HIP(hipStreamSynchronize(s));
         evals += P;
+        flop += 2.0 * n2 * P;
         CHECK_SIDON(sidon_policy_credit(pol, cfit.data()));
       }
-      if (gen % 10 == 0 || improved) {  // progress record (main.cu:771-783)
+      if (A.verbose && (gen % 10 == 0 || improved)) {  // progress records (main.cu:771-783)
         double mean = 0.0, lb = 0.0;
         for (double x : hfit) mean += x / P;
         int cells = 0, since = 0;
         CHECK_SIDON(sidon_policy_stats(pol, nullptr, nullptr, &lb, &cells, &since));
         std::printf("{\"stat\": {\"t\": %.1f, \"n\": %d, \"gen\": %d, \"best\": %.15f, "
                     "\"level_best\": %.15f, \"mean\": %.9f, \"evals\": %lld, "
-                    "\"ls_steps\": %lld, \"cells\": %d, \"peaks\": %d, \"nnz\": %d, "
-                    "\"cold\": %d, \"medium\": %d, \"since_improved\": %d%s}}\n",
-                    seconds_since(t0), n, gen, bestF, lb, mean, evals, lsSteps, cells,
+                    "\"ls_steps\": %lld, \"fw\": %lld, \"cells\": %d, \"peaks\": %d, "
+                    "\"nnz\": %d, \"cold\": %d, \"medium\": %d, \"since_improved\": %d%s}}\n",
+                    seconds_since(t0), n, gen, bestF, lb, mean, evals, lsSteps, fwRuns, cells,
                     hnact[bi], hnnz[bi], nCold, nMed, since, improved ? ", \"improved\": true" : "");
+        // the reference's key=value record, for monitors written against it
+        std::printf("STAT t=%.1f n=%d gen=%d best=%.15f lvl=%.15f mean=%.9f evals=%lld "
+                    "lsiters=%lld fw=%lld cells=%d pflop=%.6f peaks=%d nnz=%d\n",
+                    seconds_since(t0), n, gen, bestF, lb, mean, evals, lsSteps, fwRuns, cells,
+                    flop / 1e15, hnact[bi], hnnz[bi]);
       }
       ++gen;
     }
@@ -705,7 +742,7 @@ int search(const Args& A) {
 
 // ---------------------------------------------------------- graph search --
 int search_graph(const Args& A) {
-  const std::vector<int> L = A.ladder.empty() ? std::vector<int>{256, 512, 1024, 2048} : A.ladder;
+  const std::vector<int>& L = A.ladder;
   const int P = A.P, nmax = L.back();
   std::mt19937_64 rng(A.seed);
   hipStream_t s;
@@ -743,11 +780,10 @@ int search_graph(const Args& A) {
     const double budget = A.seconds * (lvl + 1) / (L.size() * (L.size() + 1) / 2.0);
     const auto tl = Clock::now();
     int gens = 0;
-    while (gens < A.gens || seconds_since(tl) < budget) {
+    while (seconds_since(tl) < budget && (A.gens == 0 || gens < A.gens)) {
       CHECK_SIDON(sidon_generation_run(g, 1, s));
       ++gens;
-      if (gens >= A.gens && seconds_since(tl) >= budget) break;
-      if (gens % 4 == 0) CHECK_HIP(hipStreamSynchronize(s));
+      CHECK_HIP(hipStreamSynchronize(s));  // the clock sees the work
     }
     CHECK_HIP(hipStreamSynchronize(s));
     int parity = 0;
@@ -793,22 +829,135 @@ int search_graph(const Args& A) {
   return 0;
 }
 
+// ------------------------------------------------------------ benchmark --
+// The reference's kernel micro-benchmark (src/main.cu:294-320): per length, a
+// population of max(32, min(512, 2^22 / n)) candidates with heights U(0.2, 1),
+// 10 untimed local-search steps, then 100 timed ones (p 8 -> 200, step
+// 0.05 -> 0.005) and one fp64 evaluation, timed with events.  TFLOP/s on the
+// reference's count (4 n^2 FLOP a candidate-step) and on the matrix work the
+// kernel executed; n = 8400 follows the reference's lengths.
+int bench_kernels() {
+  hipStream_t s;
+  CHECK_HIP(hipStreamCreate(&s));
+  hipEvent_t e0, e1, e2;
+  CHECK_HIP(hipEventCreate(&e0));
+  CHECK_HIP(hipEventCreate(&e1));
+  CHECK_HIP(hipEventCreate(&e2));
+  for (int n : {128, 256, 512, 1024, 2048, 4096, 8400}) {
+    const int P = std::max(32, std::min(512, (4 << 20) / n));
+    std::vector<float> h((size_t)n * P);
+    std::mt19937 rng(1);
+    std::uniform_real_distribution<float> U(0.2f, 1.0f);
+    for (auto& x : h) x = U(rng);
+    float* pop = dalloc<float>(h.size());
+    double* F = dalloc<double>(P);
+    CHECK_HIP(hipMemcpy(pop, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    const int it = 100;
+    CHECK_SIDON(sidon_local_search(pop, nullptr, nullptr, nullptr, nullptr, n, P, 10, 8.0f, 8.0f,
+                                   0.01f, 0.01f, 0.9f, 0.999f, 0, s));
+    CHECK_HIP(hipStreamSynchronize(s));
+    unsigned long long steps = 0;
+    CHECK_SIDON(sidon_ls_block_steps(&steps, 1));
+    CHECK_HIP(hipEventRecord(e0, s));
+    CHECK_SIDON(sidon_local_search(pop, nullptr, nullptr, nullptr, nullptr, n, P, it, 8.0f,
+                                   200.0f, 0.05f, 0.005f, 0.9f, 0.999f, 0, s));
+    CHECK_HIP(hipEventRecord(e1, s));
+    CHECK_SIDON(sidon_eval_f64(pop, n, P, F, nullptr, 1e-4, s));
+    CHECK_HIP(hipEventRecord(e2, s));
+    CHECK_HIP(hipEventSynchronize(e2));
+    CHECK_SIDON(sidon_ls_block_steps(&steps, 1));
+    float ls = 0.0f, ev = 0.0f;
+    CHECK_HIP(hipEventElapsedTime(&ls, e0, e1));
+    CHECK_HIP(hipEventElapsedTime(&ev, e1, e2));
+    const double ref_flop = 2.0 * P * it * 2.0 * (double)n * n;
+    std::printf("{\"bench\": {\"n\": %d, \"P\": %d, \"lds_kb\": %lld, \"ls_ms\": %.3f, "
+                "\"ms_per_iter\": %.4f, \"candidate_steps_per_s\": %.1f, "
+                "\"tflops_reference_count\": %.2f, \"tflops_executed\": %.2f, "
+                "\"eval_f64_ms\": %.3f}}\n",
+                n, P, sidon_lds_bytes(0, n, 1024) / 1024, ls, ls / it,
+                (double)P * it / (ls / 1e3), ref_flop / (ls / 1e3) / 1e12,
+                8192.0 * (double)steps / (ls / 1e3) / 1e12, ev);
+    CHECK_HIP(hipFree(pop));
+    CHECK_HIP(hipFree(F));
+  }
+  CHECK_HIP(hipEventDestroy(e0));
+  CHECK_HIP(hipEventDestroy(e1));
+  CHECK_HIP(hipEventDestroy(e2));
+  CHECK_HIP(hipStreamDestroy(s));
+  return 0;
+}
+
 void usage() {
   std::printf(
-      "sidon_search MODE [options]\n"
-      "  --selftest                 device known-answer tests\n"
-      "  --uneven                   island generations of uneven n as HIP graphs\n"
-      "      --P 512 --islands 8 --iters 200 --gens 3 --sigma 0.05 --lengths 600,...,8400\n"
-      "  --ladder                   Frank-Wolfe + n -> 2n split ladder\n"
-      "      --P 256 --levels 1050,2100,4200,8400 --fw-rounds 50 --cp-iters 2000\n"
-      "  --search                   the reference's search loop (policy, 14 operators)\n"
-      "      --P 256 --levels 256,512,1024,2048 --islands 8 --elite 2 --iters 200\n"
-      "      --cold-iters 1000 --pmax 20000 --search-fw 0 --fw-iters 500 --gens 4\n"
-      "      --seconds 60 --out best   (writes best.txt and best.json)\n"
-      "      --resume FILE             (a saved vector: adopted as the incumbent)\n"
-      "      --polish-top 8 --polish-iters 1200   (fp64 polish of the incumbent; 0 = off)\n"
-      "  --search-graph             ladder search on the device generation graph (op 2)\n"
-      "  --seed S  --no-tune\n");
+      "sidon_search [mode] [options]     (no mode: the search, with the reference's defaults)\n"
+      "  --ladder 64,128,...   multiresolution ladder, each level twice the last\n"
+      "                        (default 64..2048)\n"
+      "  --pop N               population size (default 256)\n"
+      "  --islands N           number of islands (default 8)\n"
+      "  --elite N             elites kept per island (default 2)\n"
+      "  --ls-iters N          warm local-search steps per generation (default 4000)\n"
+      "  --cold-iters N        cold local-search steps (default 20000; medium tier: N/3)\n"
+      "  --pmax P              top of the fp32 exponent anneal (default 20000)\n"
+      "  --fw-rounds N         Frank-Wolfe rounds per generation (default 6, 0 = off)\n"
+      "  --fw-iters N          Chambolle-Pock iterations per round (default 500)\n"
+      "  --gens N              generations per level at most (0 = time driven, default)\n"
+      "  --seconds S           total wall-clock budget (default 600; 18 %% for the polish)\n"
+      "  --seed S              RNG seed (default 20260905)\n"
+      "  --out PREFIX          output prefix: PREFIX.txt and PREFIX.json (default 'best')\n"
+      "  --resume FILE         seed the run from a saved vector\n"
+      "  --polish-iters N      fp64 polish steps, over six passes (default 40000, 0 = off)\n"
+      "  --polish-top N        candidates in the polish (default 8)\n"
+      "  --selftest            verify the objective and kernels against known values\n"
+      "  --bench               micro-benchmark the kernels and exit\n"
+      "  --quiet               no per-generation progress records\n"
+      "  -h, --help\n"
+      "modes of this build (the options above apply where they make sense):\n"
+      "  --search              the search (the default)\n"
+      "  --uneven              island generations of uneven n as HIP graphs\n"
+      "                        (defaults --pop 512 --ls-iters 200 --gens 3;\n"
+      "                        --lengths 600,1200,2100,4096,8400 --sigma 0.05 --no-tune)\n"
+      "  --fw-ladder           Frank-Wolfe rounds + exact n -> 2n splits per level\n"
+      "                        (defaults --ladder 1050,2100,4200,8400 --fw-rounds 50\n"
+      "                        --fw-iters 2000)\n"
+      "  --search-graph        search on the device generation graph (operator 2)\n"
+      "                        (defaults --ladder 256,512,1024,2048 --ls-iters 200 --seconds 60)\n"
+      "aliases: --P = --pop, --levels = --ladder, --iters = --ls-iters, --cp-iters = --fw-iters\n");
+}
+
+// Per-mode defaults for the options not given on the command line.
+void mode_defaults(Args& A, const std::vector<std::string>& given) {
+  auto unset = [&](const char* o) {
+    return std::find(given.begin(), given.end(), std::string(o)) == given.end();
+  };
+  if (A.mode == "--uneven") {
+    if (unset("--pop")) A.P = 512;
+    if (unset("--ls-iters")) A.iters = 200;
+    if (unset("--gens")) A.gens = 3;
+  } else if (A.mode == "--fw-ladder") {
+    if (unset("--ladder")) A.ladder = {1050, 2100, 4200, 8400};
+    if (unset("--fw-rounds")) A.fw_rounds = 50;
+    if (unset("--fw-iters")) A.fw_iters = 2000;
+  } else if (A.mode == "--search-graph") {
+    if (unset("--ladder")) A.ladder = {256, 512, 1024, 2048};
+    if (unset("--ls-iters")) A.iters = 200;
+    if (unset("--seconds")) A.seconds = 60.0;
+  }
+}
+
+// Checks that need no device; a message and false on the first failure.
+bool check_args(const Args& A) {
+  auto fail = [](const char* m) { std::fprintf(stderr, "%s\n", m); return false; };
+  if (A.P < 1 || A.islands < 1) return fail("pop and islands must be positive");
+  if (A.P % A.islands != 0) return fail("pop must be divisible by islands");
+  if (A.elite < 0 || A.iters < 0 || A.cold_iters < 0 || A.gens < 0 || A.fw_rounds < 0 ||
+      A.fw_iters < 0 || A.polish_iters < 0 || A.polish_top < 0)
+    return fail("counts must not be negative");
+  if (!(A.seconds >= 0.0)) return fail("seconds must not be negative");
+  if (A.mode == "--search" || A.mode == "--search-graph" || A.mode == "--fw-ladder")
+    for (size_t l = 1; l < A.ladder.size(); ++l)
+      if (A.ladder[l] != 2 * A.ladder[l - 1])
+        return fail("each ladder level must be twice the one before (n -> 2n cell split)");
+  return true;
 }
 
 }  // namespace
@@ -816,44 +965,83 @@ void usage() {
 int main(int argc, char** argv) {
   setvbuf(stdout, nullptr, _IONBF, 0);
   Args A;
+  std::vector<std::string> given;
   for (int i = 1; i < argc; ++i) {
-    const std::string s = argv[i];
+    std::string s = argv[i];
+    if (s == "--P") s = "--pop";  // this driver's earlier spellings
+    else if (s == "--levels") s = "--ladder";
+    else if (s == "--iters") s = "--ls-iters";
+    else if (s == "--cp-iters") s = "--fw-iters";
+    given.push_back(s);
     auto next = [&]() -> const char* {
-      if (i + 1 >= argc) { usage(); std::exit(2); }
+      if (i + 1 >= argc) {
+        std::fprintf(stderr, "option %s needs a value\n", s.c_str());
+        usage();
+        std::exit(1);
+      }
       return argv[++i];
     };
-    if (s == "--selftest" || s == "--uneven" || s == "--ladder" || s == "--search" ||
-        s == "--search-graph")
+    auto list = [&](std::vector<int>& v) {
+      const char* x = next();
+      if (!parse_list(x, v)) {
+        std::fprintf(stderr, "bad list for %s: %s\n", s.c_str(), x);
+        std::exit(1);
+      }
+    };
+    if (s == "--search" || s == "--selftest" || s == "--bench" || s == "--uneven" ||
+        s == "--fw-ladder" || s == "--search-graph")
       A.mode = s;
-    else if (s == "--P") A.P = std::atoi(next());
+    else if (s == "--ladder") list(A.ladder);
+    else if (s == "--pop") A.P = std::atoi(next());
     else if (s == "--islands") A.islands = std::atoi(next());
-    else if (s == "--iters") A.iters = std::atoi(next());
-    else if (s == "--gens") A.gens = std::atoi(next());
     else if (s == "--elite") A.elite = std::atoi(next());
-    else if (s == "--sigma") A.sigma = (float)std::atof(next());
-    else if (s == "--lengths") A.lengths = parse_list(next());
-    else if (s == "--levels") A.ladder = parse_list(next());
-    else if (s == "--fw-rounds") A.fw_rounds = std::atoi(next());
-    else if (s == "--cp-iters") A.cp_iters = std::atoi(next());
-    else if (s == "--seconds") A.seconds = std::atof(next());
+    else if (s == "--ls-iters") A.iters = std::atoi(next());
     else if (s == "--cold-iters") A.cold_iters = std::atoi(next());
     else if (s == "--pmax") A.pmax = std::atof(next());
-    else if (s == "--search-fw") A.search_fw = std::atoi(next());
+    else if (s == "--fw-rounds") A.fw_rounds = std::atoi(next());
     else if (s == "--fw-iters") A.fw_iters = std::atoi(next());
+    else if (s == "--gens") A.gens = std::atoi(next());
+    else if (s == "--seconds") A.seconds = std::atof(next());
     else if (s == "--seed") A.seed = std::strtoull(next(), nullptr, 10);
     else if (s == "--out") A.out = next();
     else if (s == "--resume") A.resume = next();
-    else if (s == "--polish-top") A.polish_top = std::atoi(next());
     else if (s == "--polish-iters") A.polish_iters = std::atoi(next());
+    else if (s == "--polish-top") A.polish_top = std::atoi(next());
+    else if (s == "--quiet") A.verbose = 0;
+    else if (s == "--sigma") A.sigma = (float)std::atof(next());
+    else if (s == "--lengths") list(A.lengths);
     else if (s == "--no-tune") A.tune = false;
-    else { usage(); return 2; }
+    else if (s == "-h" || s == "--help") { usage(); return 0; }
+    else {
+      std::fprintf(stderr, "unknown option %s\n", s.c_str());
+      usage();
+      return 1;
+    }
   }
+  mode_defaults(A, given);
+  if (!check_args(A)) return 1;
+
   CHECK_SIDON(sidon_init(-1));
+  int dev = 0;
+  hipDeviceProp_t prop;
+  CHECK_HIP(hipGetDevice(&dev));
+  CHECK_HIP(hipGetDeviceProperties(&prop, dev));
+  std::printf("GPU: %s  %s  %d CUs  LDS per workgroup %d B\n", prop.name, prop.gcnArchName,
+              prop.multiProcessorCount, sidon_lds_limit());
   if (A.mode == "--selftest") return selftest();
+  if (A.mode == "--bench") return bench_kernels();
+  if (A.mode == "--search") {  // every level must fit the fused search (main.cu:390-399)
+    for (int n : A.ladder) {
+      const long long need = sidon_lds_bytes(0, n, 1024);
+      if (need < 0 || need > sidon_lds_limit()) {
+        std::fprintf(stderr, "n=%d needs %lld B of LDS, the device offers %d\n", n, need,
+                     sidon_lds_limit());
+        return 1;
+      }
+    }
+  }
   if (A.mode == "--uneven") return uneven(A);
-  if (A.mode == "--ladder") return ladder(A);
-  if (A.mode == "--search") return search(A);
+  if (A.mode == "--fw-ladder") return fw_ladder(A);
   if (A.mode == "--search-graph") return search_graph(A);
-  usage();
-  return 2;
+  return search(A);
 }

@@ -30,17 +30,42 @@ def test_selftest():
     assert "FAIL" not in out and "OK" in out
 
 
+STAT_KEYS = ["t", "n", "gen", "best", "lvl", "mean", "evals", "lsiters", "fw", "cells", "pflop",
+             "peaks", "nnz"]   # the reference's progress record, main.cu:778-781
+
+
+def stat_lines(out):
+    recs = []
+    for line in out.splitlines():
+        if line.startswith("STAT "):
+            kv = dict(f.split("=", 1) for f in line.split()[1:])
+            assert list(kv) == STAT_KEYS
+            recs.append(kv)
+    return recs
+
+
 def test_policy_search_improves_and_writes_verifiable_files(O, tmp_path):
+    """The reference's command line (no mode flag): FW rounds on by default,
+    --gens caps each level, the time budget is generous."""
     base = tmp_path / "best"
-    out = run(["--search", "--P", "64", "--islands", "4", "--elite", "2", "--levels", "128,256",
-               "--iters", "60", "--cold-iters", "240", "--gens", "12", "--seconds", "4",
-               "--out", str(base), "--seed", "7"])
+    out = run(["--pop", "64", "--islands", "4", "--elite", "2", "--ladder", "128,256",
+               "--ls-iters", "60", "--cold-iters", "240", "--fw-iters", "200", "--gens", "12",
+               "--seconds", "60", "--polish-iters", "1200", "--out", str(base), "--seed", "7"])
+    assert out.startswith("GPU: ")
     recs = [json.loads(line) for line in out.splitlines() if line.startswith("{")]
     stats = [r["stat"] for r in recs if "stat" in r]
     levels = [r for r in recs if "level" in r]
     final = recs[-1]
     assert [r["level"] for r in levels] == [128, 256]
-    assert all(r["generations"] >= 12 for r in levels)
+    assert all(r["generations"] == 12 for r in levels)
+    # STAT records: six FW rounds a generation for every candidate, before the record
+    st = stat_lines(out)
+    assert len(st) == len(stats) and st
+    for r in st:
+        done = int(r["gen"]) + 1 + (12 if r["n"] == "256" else 0)
+        assert int(r["fw"]) == 6 * 64 * done
+    pf = [float(r["pflop"]) for r in st]
+    assert pf == sorted(pf) and pf[0] > 0
     assert len(levels[0]["op_weights"]) == 14 and levels[0]["cells"] > 1
     assert stats[0]["cold"] > 0                       # fresh seeds take the cold tier
     polish = [r["polish"] for r in recs if "polish" in r]
@@ -57,8 +82,10 @@ def test_policy_search_improves_and_writes_verifiable_files(O, tmp_path):
 
 def test_graph_search_runs(tmp_path):
     out = run(["--search-graph", "--P", "64", "--islands", "4", "--levels", "128,256",
-               "--iters", "50", "--gens", "3", "--seconds", "1", "--out", str(tmp_path / "g")])
-    final = json.loads(out.strip().splitlines()[-1])
+               "--iters", "50", "--gens", "3", "--seconds", "30", "--out", str(tmp_path / "g")])
+    recs = [json.loads(line) for line in out.splitlines() if line.startswith("{")]
+    assert [r["generations"] for r in recs if "level" in r] == [3, 3]
+    final = recs[-1]
     assert final["best_F"] < 1.7 and (tmp_path / "g.json").exists()
 
 
@@ -68,8 +95,10 @@ def test_resume_adopts_the_vector_and_drops_coarser_levels(O, certs, tmp_path):
     O.write_vec(str(src), a, O.F_hostorder(a))
     base = tmp_path / "res"
     out = run(["--search", "--P", "64", "--islands", "4", "--levels", "256,512,1024",
-               "--iters", "60", "--cold-iters", "240", "--gens", "3", "--seconds", "2",
+               "--iters", "60", "--cold-iters", "240", "--fw-rounds", "0", "--gens", "3",
+               "--seconds", "60", "--polish-iters", "600", "--quiet",
                "--resume", str(src), "--out", str(base)])
+    assert "STAT" not in out and '"stat"' not in out    # --quiet
     recs = [json.loads(line) for line in out.splitlines() if line.startswith("{")]
     assert recs[0]["resumed"] == str(src) and recs[0]["levels_dropped"] == 1
     assert recs[0]["F"] == pytest.approx(O.F_hostorder(a), rel=0, abs=1e-14)
@@ -92,8 +121,8 @@ def test_uneven_islands_graph():
 def test_ladder_splits_preserve_F_exactly():
     """Config 5 in small: Frank-Wolfe rounds per level, every n -> 2n split keeps
     the correctly rounded F bit for bit (and certifies the rounding)."""
-    out = run(["--ladder", "--P", "32", "--levels", "256,512,1024", "--fw-rounds", "1",
-               "--cp-iters", "200"])
+    out = run(["--fw-ladder", "--pop", "32", "--ladder", "256,512,1024", "--fw-rounds", "1",
+               "--fw-iters", "200"])
     recs = [json.loads(line) for line in out.splitlines() if line.startswith("{")]
     splits = [r for r in recs if "split" in r]
     levels = [r for r in recs if "level" in r]
@@ -103,3 +132,14 @@ def test_ladder_splits_preserve_F_exactly():
         assert r["F_bit_exact"] == r["of"] == 32 and r["ambiguous"] == 0
     for r in levels:
         assert r["mean_F_after"] <= r["mean_F_before"] + 1e-12
+
+
+def test_bench_mode():
+    """--bench: the reference's micro-benchmark lengths (main.cu:296) plus 8400."""
+    out = run(["--bench"])
+    recs = [json.loads(line)["bench"] for line in out.splitlines() if line.startswith("{")]
+    assert [r["n"] for r in recs] == [128, 256, 512, 1024, 2048, 4096, 8400]
+    for r in recs:
+        assert r["P"] == max(32, min(512, (4 << 20) // r["n"]))
+        assert r["ls_ms"] > 0 and r["eval_f64_ms"] > 0
+        assert 0 < r["tflops_executed"] < 157.3
