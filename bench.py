@@ -281,7 +281,8 @@ def main() -> int:
         torch.cuda.synchronize()
         single = {"value": iters / (ev0.elapsed_time(ev1) / 1e3), "unit": "candidate-steps/s",
                   "form": "one candidate over the whole device in fp64 (sidon_polish_f64: both "
-                          "products cut into (tile, chunk) work units, six kernels a step)",
+                          "products as fp64 matrix-core work units over every CU, five "
+                          "kernels a step)",
                   "dtype": "f64"}
 
     valu_ceiling, mfma_ceiling = S.measure_fp32_ceilings()

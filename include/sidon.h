@@ -104,8 +104,8 @@ int sidon_local_search_f64(double* pop, double* mom, double* vel, const int* ski
  * start at zero; mom/vel, may be NULL, receive the final moments), with both
  * products cut into work units for every CU and summed in a fixed order
  * (deterministic).  `workspace` is a device buffer of at least
- * sidon_polish_f64_workspace(n, K) bytes (about 3n^2/256 + 5n doubles per
- * candidate: 56 MB for K = 8 at n = 8400); it holds no state between calls.
+ * sidon_polish_f64_workspace(n, K) bytes (about 3n^2/1024 + 7n doubles per
+ * candidate: 17 MB for K = 8 at n = 8400); it holds no state between calls.
  * pop is [K x n] row-major. */
 size_t sidon_polish_f64_workspace(int n, int K);
 int sidon_polish_f64(double* pop, double* mom, double* vel, int n, int K, int iters,

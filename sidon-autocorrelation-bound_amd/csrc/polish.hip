@@ -5,21 +5,26 @@
 // elite, passes of rising exponent) and src/kernels.cuh:177-289 (the step).
 // The fused fp64 local search keeps a, b/c and G in one workgroup's LDS and so
 // stops at n = 4560 (3n doubles do not fit 160 KB at n = 8400).  Here one
-// step is six kernels over global memory (L2-resident: ~3n doubles per
-// candidate), and both products are cut into work units that every CU can take:
+// step is five kernels over global memory (L2-resident: ~3n doubles per
+// candidate), and both products run on the fp64 matrix cores
+// (v_mfma_f64_16x16x4f64) as blocked Toeplitz / Hankel tiles cut into work
+// units that every CU can take:
 //
-//   unit (candidate, tile q of 16 outputs, chunk cc of 16 blocks of 16 i)
-//     = one lane, 16 trips of 16 x 16 fp64 FMAs, partial sums to `part`;
+//   unit (candidate, tile j of 16 output segments = 256 outputs, chunk of CH
+//   block steps) = one wave, CH steps of four MFMAs;
 //
-// a workgroup holds 256 consecutive tiles of one chunk, so every lane runs the
-// same 16 trips (products outside a tile's range read zero padding), the
-// chunk's 256 heights are broadcast and the other operand's window sits in
-// LDS with one unused slot per 16 (lanes one tile apart read 17 doubles apart).
-// The next kernel adds a tile's chunk partials in chunk order, so the sums are
-// deterministic.  Step: conv units -> conv reduce (b, M) -> weights (c, R
-// partials) -> corr units -> step (G, R, Adam or mirror step, clamp >= 0,
-// S partials) -> renormalise, the same arithmetic as the fused kernel's step
-// in fp64.
+// a workgroup holds UW consecutive chunks of one tile, stages their operands
+// from L2 into LDS windows (the overlapped segment layout of the fused
+// kernel) and adds its units' partial tiles in chunk order into one partial
+// per chunk group.  The autoconvolution is the symmetric form (block offsets
+// d below ceil(K/2) only); its reduce kernel adds the group partials of an
+// output in order, then the at most 15 remaining products and the square
+// term.  Step: conv units -> conv reduce (b, M) -> weights (c, R partials) ->
+// corr units -> step (G, R, Adam or mirror step, clamp >= 0, S partials).
+// The renormalisation a = x / S (uniform 1/n if S is not positive) is applied
+// by every reader of a from the previous step's S partials, which each adds in
+// the same order, so all sums are deterministic and the arithmetic is the
+// fused kernel's step in fp64.
 #include <cmath>
 
 #include "common.hpp"
@@ -28,37 +33,36 @@
 namespace sidon {
 namespace {
 
-constexpr int PU = 256;     // tiles (lanes) per unit workgroup
-constexpr int CH = 16;      // 16-blocks per chunk
-constexpr int PADA = 576;   // zero heights each side of the padded copy of a
-constexpr int RB = 256;     // threads per reduce / weights workgroup
+constexpr int CH = 16;      // block steps per work unit
+constexpr int UW = 4;       // work units (waves, consecutive chunks) per workgroup
+constexpr int RB = 256;     // threads per reduce / weights / step workgroup
 
 struct Shape {
-  int n, NJ, NTc, NCc, NTg, NCg, LA, LB;
+  int n, N16, NTc, NGc, NTg, NGg, LB;
 };
 
 __host__ __device__ inline Shape shape_of(int n) {
   Shape s;
   s.n = n;
-  s.NJ = (n + 15) / 16;                  // 16-blocks of a
-  s.NTc = (2 * n - 1 + 15) / 16;         // output tiles of b
-  s.NCc = (s.NJ + CH - 1) / CH;          // chunks of the conv reduction
-  s.NTg = s.NJ;                          // output tiles of G
-  s.NCg = s.NCc;                         // chunks of the corr reduction
-  s.LA = n + 2 * PADA;                   // padded copy of a
-  s.LB = 16 * (s.NTg + CH * s.NCg) + 64; // b / c, zero beyond 2n - 1
+  s.N16 = nseg16(n);                            // 16-segments of a
+  s.NTc = conv_tiles(n);                        // autoconvolution tiles (256 outputs each)
+  s.NGc = (s.N16 / CH + 1 + UW - 1) / UW;       // groups of UW chunks of d in [0, N16]
+  s.NTg = corr_tiles(n);                        // correlation tiles (256 outputs each)
+  s.NGg = ((s.N16 + 15 + CH - 1) / CH + UW - 1) / UW;  // groups of a tile's Hankel chunks
+  s.LB = 16 * (16 * s.NTg + CH * UW * s.NGg) + 64;     // b / c, zero beyond 2n - 1
   return s;
 }
 
 // workspace layout (doubles unless noted), per candidate:
-//   apad[LA], bc[LB], part[NCc * NTc * 16], gpart[NCg * NTg * 16], mom[n], vel[n],
-//   rpart[ceil(LB / RB)], mpart[ceil(LB / RB)], spart[ceil(n / RB)], bias[2] (beta1^t,
-//   beta2^t), then K u64 max bits, K ints: step counter, stopped.
+//   x[n] (the unnormalised heights of the last step), bc[LB], part[NGc * 256 NTc],
+//   gpart[NGg * 256 NTg], mom[n], vel[n], rpart[ceil(LB / RB)], mpart[ceil(LB / RB)],
+//   spart[2][ceil(n / RB)] (S partials of even / odd steps), then K u64 max bits and
+//   K ints: 0 while running, 1 + the step at which M was not positive.
 struct Ws {
-  double *apad, *bc, *part, *gpart, *mom, *vel, *rpart, *mpart, *spart, *bias;
+  double *x, *bc, *part, *gpart, *mom, *vel, *rpart, *mpart, *spart;
   unsigned long long* mbits;
-  int *step, *stopped;
-  long long sa, sb, sp, sg, sn, sr, ss;  // per-candidate strides
+  int* stopped;
+  long long sb, sp, sg, sn, sr, ss;  // per-candidate strides
 };
 
 __host__ __device__ inline int rblocks(const Shape& s) { return (s.LB + RB - 1) / RB; }
@@ -66,23 +70,23 @@ __host__ __device__ inline int sblocks(const Shape& s) { return (s.n + RB - 1) /
 
 __host__ __device__ inline size_t ws_bytes(int n, int K) {
   const Shape s = shape_of(n);
-  const size_t per = (size_t)s.LA + s.LB + (size_t)s.NCc * s.NTc * 16 + (size_t)s.NCg * s.NTg * 16 +
-                     2 * (size_t)n + 2 * rblocks(s) + sblocks(s) + 2;
-  return per * K * sizeof(double) + K * (sizeof(unsigned long long) + 2 * sizeof(int)) + 256;
+  const size_t per = (size_t)n + s.LB + (size_t)s.NGc * 256 * s.NTc +
+                     (size_t)s.NGg * 256 * s.NTg + 2 * (size_t)n + 2 * rblocks(s) +
+                     2 * sblocks(s);
+  return per * K * sizeof(double) + K * (sizeof(unsigned long long) + sizeof(int)) + 256;
 }
 
 __host__ __device__ inline Ws ws_of(void* base, int n, int K) {
   const Shape s = shape_of(n);
   Ws w;
-  w.sa = s.LA;
   w.sb = s.LB;
-  w.sp = (long long)s.NCc * s.NTc * 16;
-  w.sg = (long long)s.NCg * s.NTg * 16;
+  w.sp = (long long)s.NGc * 256 * s.NTc;
+  w.sg = (long long)s.NGg * 256 * s.NTg;
   w.sn = n;
   w.sr = rblocks(s);
   w.ss = sblocks(s);
   double* d = reinterpret_cast<double*>(base);
-  w.apad = d; d += w.sa * K;
+  w.x = d; d += w.sn * K;
   w.bc = d; d += w.sb * K;
   w.part = d; d += w.sp * K;
   w.gpart = d; d += w.sg * K;
@@ -90,11 +94,9 @@ __host__ __device__ inline Ws ws_of(void* base, int n, int K) {
   w.vel = d; d += w.sn * K;
   w.rpart = d; d += w.sr * K;
   w.mpart = d; d += w.sr * K;
-  w.spart = d; d += w.ss * K;
-  w.bias = d; d += 2 * K;
+  w.spart = d; d += 2 * w.ss * K;
   w.mbits = reinterpret_cast<unsigned long long*>(d);
-  w.step = reinterpret_cast<int*>(w.mbits + K);
-  w.stopped = w.step + K;
+  w.stopped = reinterpret_cast<int*>(w.mbits + K);
   return w;
 }
 
@@ -110,132 +112,233 @@ __device__ __forceinline__ void sched_at(const Sched& S, int it, double& p, doub
   lr = S.lrStart * exp(S.lnLr * frac);
 }
 
-// LDS slot of window element e (one unused slot after every 16)
-__device__ __forceinline__ int wslot(int e) { return e + (e >> 4); }
+// Sum of v[0 .. m) in one fixed order, the same in every lane and every
+// caller (lane l adds v[l], v[l + 64], ... in ascending order, then a
+// butterfly); the calling wave must have all lanes active.
+__device__ __forceinline__ double fixed_sum(const double* v, int m) {
+  double r = 0.0;
+  for (int b = (int)threadIdx.x & (kWave - 1); b < m; b += kWave) r += v[b];
+  return wave_sum(r);
+}
 
-// sum of p[c * stride] for c = c0 .. c1 in ascending c, loaded eight at a time
-// so the loads are in flight together (absent terms add an exact zero)
+// The normalisation of the heights entering step `it`: a_i = reset ? 1/n :
+// x_i * inv with S the fixed-order sum of step it - 1's partials (the fused
+// kernel's renormalisation); step 0 takes the input as it is.
+struct Norm {
+  double inv, uni;
+  bool reset;
+  __device__ __forceinline__ double of(double x) const { return reset ? uni : x * inv; }
+};
+__device__ __forceinline__ Norm norm_at(const Ws& w, int cand, int it, int n) {
+  Norm r{1.0, 1.0 / (double)n, false};
+  if (it == 0) return r;
+  const double S = fixed_sum(w.spart + w.ss * (2 * cand + ((it - 1) & 1)), (int)w.ss);
+  r.reset = !(S > 0.0);
+  r.inv = r.reset ? 1.0 : 1.0 / S;
+  return r;
+}
+
+// sum of p[c * stride] for c = c0 .. c1 in ascending c, loaded sixteen at a
+// time so the loads are in flight together (absent terms add an exact zero)
 __device__ __forceinline__ double chunk_sum(const double* p, long long stride, int c0, int c1) {
   double r = 0.0;
-  for (int c = c0; c <= c1; c += 8) {
-    double v[8];
+  for (int c = c0; c <= c1; c += 16) {
+    double v[16];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = c + u <= c1 ? p[(long long)(c + u) * stride] : 0.0;
+    for (int u = 0; u < 16; ++u) v[u] = c + u <= c1 ? p[(long long)(c + u) * stride] : 0.0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) r += v[u];
+    for (int u = 0; u < 16; ++u) r += v[u];
   }
   return r;
 }
 
-// ---------------------------------------------------------------- setup --
+// LDS windows of a in the overlapped segment layout, without its left
+// padding: window element x (a_{16 g + x}) sits at wpos(x) = pad_addr(x) -
+// 15 kSegStride for x in [-4, 16 LEN), so the buffer holds (LEN + 2) segments,
+// then one zero segment (the masked reads of the symmetric tile land there).
+constexpr int kWinShift = 15 * kSegStride;
+__host__ __device__ constexpr int win_len(int len) { return (len + 3) * kSegStride; }
+__device__ __forceinline__ int wpos(int x) { return pad_addr(x) - kWinShift; }
+
+// Matrix B-operand windows (a lane reads four heights inside one segment, so
+// no copies are needed): segment x/16 of the window at 18 (x/16) doubles.
+// Sixteen lanes one segment apart then cover all 64 LDS banks once per
+// 16-byte read (a 20-double stride would put lanes t and t + 8 on the same
+// banks).  LEN segments, then one zero segment.
+constexpr int kBStride = 18;
+__host__ __device__ constexpr int bwin_len(int len) { return (len + 1) * kBStride; }
+__device__ __forceinline__ int bpos(int x) { return (x >> 4) * kBStride + (x & 15); }
+
+template <int NT, int LEN>
+__device__ __forceinline__ void fill_bwindow(double* win, const double* __restrict__ xs, int n,
+                                             int g, const Norm& nm) {
+  constexpr int E = 16 * LEN, R = (E + NT - 1) / NT;
+  double v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int x = (int)threadIdx.x + r * NT;
+    const int i = 16 * g + x;
+    v[r] = (x < E && i >= 0 && i < n) ? nm.of(xs[i]) : 0.0;
+  }
+  if (threadIdx.x < kBStride) win[LEN * kBStride + threadIdx.x] = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int x = (int)threadIdx.x + r * NT;
+    if (x < E) win[bpos(x)] = v[r];
+  }
+}
+
+// Fills a window from the unnormalised heights x (normalised on the way in,
+// zero outside [0, n)); NT threads, every load in flight before the first
+// store.
+template <int NT, int LEN>
+__device__ __forceinline__ void fill_window(double* win, const double* __restrict__ xs, int n,
+                                            int g, const Norm& nm) {
+  constexpr int E = 16 * LEN + 4, R = (E + NT - 1) / NT;
+  double v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int x = (int)threadIdx.x + r * NT - 4;
+    const int i = 16 * g + x;
+    v[r] = (x < 16 * LEN && i >= 0 && i < n) ? nm.of(xs[i]) : 0.0;
+  }
+  if (threadIdx.x < kSegStride) win[(LEN + 2) * kSegStride + threadIdx.x] = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int x = (int)threadIdx.x + r * NT - 4;
+    if (x < 16 * LEN) {
+      const int p = wpos(x);
+      win[p] = v[r];
+      if (((x + kPadSeg * kSeg) & 15) >= 12) win[p + 4] = v[r];  // copy heading the next segment
+    }
+  }
+}
+
+// Adds the UW units' 16 x 16 partial tiles of a workgroup in unit order and
+// stores the sum (256 doubles at `o`, one per thread, coalesced).  `red` is
+// UW * 256 doubles of LDS that no wave reads any more; every thread calls it.
+// Lane (kk, t) of a unit holds rows kk + 4r of column t: outputs 16 t + kk + 4 r.
+__device__ __forceinline__ void group_sum_store(double* red, const f64x4& acc, double* o) {
+  const int lane = (int)threadIdx.x & (kWave - 1), wv = (int)threadIdx.x / kWave;
+  double* p = red + 256 * wv + 16 * (lane & 15) + (lane >> 4);
+  p[0] = acc[0]; p[4] = acc[1]; p[8] = acc[2]; p[12] = acc[3];
+  __syncthreads();
+  const int k = threadIdx.x;  // UW * kWave == 256 threads, one output each
+  double r = red[k];
+#pragma unroll
+  for (int u = 1; u < UW; ++u) r += red[256 * u + k];
+  o[k] = r;
+}
+static_assert(UW * kWave == 256, "one thread per tile output in group_sum_store");
+
 __global__ void k_polish_setup(const double* __restrict__ pop, int n, Ws w) {
   const int cand = blockIdx.y;
-  double* ap = w.apad + w.sa * cand;
-  const int LA = n + 2 * PADA;
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < LA; e += gridDim.x * blockDim.x) {
-    const int i = e - PADA;
-    ap[e] = (i >= 0 && i < n) ? pop[(size_t)cand * n + i] : 0.0;
-  }
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    w.x[w.sn * cand + i] = pop[(size_t)cand * n + i];
     w.mom[w.sn * cand + i] = 0.0;
     w.vel[w.sn * cand + i] = 0.0;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     w.mbits[cand] = 0ull;
-    w.step[cand] = 0;
     w.stopped[cand] = 0;
-    w.bias[2 * cand] = 1.0;
-    w.bias[2 * cand + 1] = 1.0;
   }
 }
 
 // --------------------------------------------------------- conv units ----
-// part[cand][cc][q][t] = sum over i in chunk cc of a_i a_{16q+t-i}.  Tiles of
-// chunk cc whose range of i meets the chunk are [qlo, qhi]; workgroup
-// blockIdx.x takes 256 of them.
-__device__ __forceinline__ void conv_tile_range(int n, int q, int& J0, int& J1) {
-  const int k0 = 16 * q;
-  J0 = (k0 - (n - 1) > 0 ? k0 - (n - 1) : 0) >> 4;
-  J1 = (k0 + 15 < n - 1 ? k0 + 15 : n - 1) >> 4;
+// Matrix share of the symmetric autoconvolution (the fused kernel's
+// autoconv_tile) for output segments K = K0 + t over block offsets d in
+// [da, db), from windows: wa starts at segment ga, wb at segment gb, and wb's
+// zero segment follows its LENB segments.  Column t takes block d only while
+// d < ceil((K0 + t) / 2); past that its B reads come from the zero segment.
+template <int D, int LENB>
+__device__ __forceinline__ f64x4 polish_conv_tile(const double* wa, int ga, const double* wb,
+                                                  int gb, int K0, int da, int db) {
+  constexpr int S = kSegStride;
+  const int lane = opaque((int)threadIdx.x) & (kWave - 1);
+  const int t = lane & 15, kk = lane >> 4;
+  const int pa = wpos(16 * (da - ga) + t - 4 * kk);         // a_{16d + t - s}
+  const int pb = bpos(16 * (K0 + t - da - gb)) + 4 * kk;    // a_{16(K0 + t - d) + s}
+  const int live = ((K0 + t + 1) >> 1) - da;
+  constexpr int zero = LENB * kBStride;
+  return run_steps<double, D>(db - da, [&](int i, double (&x)[4], double (&y)[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) x[q] = wa[pa + i * S - q];
+    load4(wb + (i < live ? pb - i * kBStride : zero), y);
+  });
 }
 
-__device__ void conv_chunk_tiles(const Shape& s, int cc, int& qlo, int& qhi) {
-  // J1(q) and J0(q) do not decrease with q: binary searches
-  int lo = 0, hi = s.NTc;  // first q with J1(q) >= CH cc
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    int a, b;
-    conv_tile_range(s.n, mid, a, b);
-    if (b >= CH * cc) hi = mid; else lo = mid + 1;
-  }
-  qlo = lo;
-  lo = 0;
-  hi = s.NTc;  // first q with J0(q) > CH cc + CH - 1
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    int a, b;
-    conv_tile_range(s.n, mid, a, b);
-    if (a > CH * cc + CH - 1) hi = mid; else lo = mid + 1;
-  }
-  qhi = lo - 1;
-}
-
-__global__ __launch_bounds__(PU) void k_polish_conv_units(Shape s, Ws w) {
-  __shared__ double sx[CH * 16 + CH];             // the chunk's 16 CH heights (padded)
-  __shared__ double sw[(16 * (PU + CH) + 32) * 17 / 16 + 16];  // a window (padded)
-  const int cand = blockIdx.z, cc = blockIdx.y;
-  int qlo, qhi;
-  conv_chunk_tiles(s, cc, qlo, qhi);
-  const int qa = qlo + PU * blockIdx.x;
-  if (qa > qhi) return;
-  const int qb = qa + PU - 1 < qhi ? qa + PU - 1 : qhi;
-  const double* ap = w.apad + w.sa * cand + PADA;  // ap[i] = a_i, zero for -PADA <= i < n + PADA
-  const int J0c = CH * cc;
-  for (int e = threadIdx.x; e < CH * 16; e += PU) sx[wslot(e)] = ap[16 * J0c + e];
-  // window: a_x for x in [16 (qa - J0c - CH + 1) - 15, 16 (qb - J0c) + 15]
-  const int wlo = 16 * (qa - J0c - (CH - 1)) - 15;
-  const int wlen = 16 * (qb - qa + CH) + 15;
-  for (int e = threadIdx.x; e < wlen; e += PU) sw[wslot(e)] = ap[wlo + e];
+// part[cand][g][256 j + 16 t + row] = matrix share of output 16 (16 j + t) + row
+// over the block offsets of tile j in chunk group g (d in [CH UW g, CH UW (g + 1))).
+// Workgroup (j, g, cand); wave u takes chunk UW g + u.
+__global__ __launch_bounds__(UW * kWave) void k_polish_conv_units(Shape s, Ws w, int it) {
+  constexpr int LENA = CH * UW + 1, LENB = 16 + CH * UW - 1;
+  __shared__ __align__(16) double wa[win_len(LENA)];
+  __shared__ __align__(16) double wb[bwin_len(LENB)];
+  static_assert(win_len(LENA) >= 256 * UW, "the group sum reuses wa");
+  const int cand = blockIdx.z, g = blockIdx.y, j = blockIdx.x;
+  int lo, hi;
+  autoconv_drange(j, s.N16, lo, hi);
+  const int d0 = CH * UW * g;
+  if (!(lo <= hi && lo < d0 + CH * UW && hi >= d0)) return;  // the tile has no block here
+  const double* xs = w.x + w.sn * cand;
+  const Norm nm = norm_at(w, cand, it, s.n);
+  const int ga = d0 - 1;                       // segments of a_{16d + t - s}
+  const int gb = 16 * j - d0 - CH * UW + 1;    // segments of a_{16(K - d) + s}
+  fill_window<UW * kWave, LENA>(wa, xs, s.n, ga, nm);
+  fill_bwindow<UW * kWave, LENB>(wb, xs, s.n, gb, nm);
   __syncthreads();
-  const int q = qa + (int)threadIdx.x;
-  if (q > qb) return;
-  double acc[16];
-#pragma unroll
-  for (int t = 0; t < 16; ++t) acc[t] = 0.0;
-  for (int jj = 0; jj < CH; ++jj) {
-    double x[16], v[31];
-#pragma unroll
-    for (int u = 0; u < 16; ++u) x[u] = sx[wslot(16 * jj + u)];
-    const int base = 16 * (q - J0c - jj) - 15 - wlo;  // window offset of a_{16(q-J)-15}
-#pragma unroll
-    for (int m = 0; m < 31; ++m) v[m] = sw[wslot(base + m)];
-#pragma unroll
-    for (int u = 0; u < 16; ++u)
-#pragma unroll
-      for (int t = 0; t < 16; ++t) acc[t] = fma(x[u], v[t - u + 15], acc[t]);
-  }
-  double* o = w.part + w.sp * cand + ((size_t)cc * s.NTc + q) * 16;
-#pragma unroll
-  for (int t = 0; t < 16; ++t) o[t] = acc[t];
+  const int c0 = d0 + CH * wave_id();
+  const int da = lo > c0 ? lo : c0;
+  const int db = hi + 1 < c0 + CH ? hi + 1 : c0 + CH;
+  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+  if (da < db && !w.stopped[cand])
+    acc = polish_conv_tile<Pipe<double>::kDepth, LENB>(wa, ga, wb, gb, 16 * j, da, db);
+  __syncthreads();  // every wave is done with the windows
+  group_sum_store(wa, acc, w.part + w.sp * cand + (size_t)g * 256 * s.NTc + 256 * (size_t)j);
 }
 
 // ---------------------------------------------------------- conv reduce --
-// b_k = sum of the chunk partials of tile k/16 in chunk order; each
-// workgroup's maximum goes to mpart (the weights take the max of those).
-__global__ __launch_bounds__(RB) void k_polish_conv_reduce(Shape s, Ws w) {
+// b_k = 2 (sum of the group partials of k in group order + the products
+// 16(dp-1) + row < i < k/2, at most 15, in ascending i) + [k even] a_{k/2}^2
+// (dp = ceil(K/2), K = k/16, row = k%16: the fused kernel's autoconv_finish).
+// Workgroup x holds outputs [256 x, 256 x + 256), whose finishing operands
+// lie in a_{128 x - 24 .. 128 x + 151}; its maximum goes to mpart (the weights
+// take the max of those).
+__global__ __launch_bounds__(RB) void k_polish_conv_reduce(Shape s, Ws w, int it) {
   constexpr int NW = RB / kWave;
+  constexpr int NA = RB / 2 + 48;
   __shared__ double slots[NW];
+  __shared__ double sa[NA];
   const int cand = blockIdx.y;
   if (w.stopped[cand]) return;
+  const double* xs = w.x + w.sn * cand;
+  const Norm nm = norm_at(w, cand, it, s.n);
+  const int base = 128 * (int)blockIdx.x - 24;
+  for (int e = threadIdx.x; e < NA; e += RB) {
+    const int i = base + e;
+    sa[e] = (i >= 0 && i < s.n) ? nm.of(xs[i]) : 0.0;
+  }
+  __syncthreads();
   const int k = blockIdx.x * RB + threadIdx.x;
   double b = 0.0;
-  if (k < 16 * s.NTc) {
-    const int q = k >> 4, t = k & 15;
-    int J0, J1;
-    conv_tile_range(s.n, q, J0, J1);
-    b = chunk_sum(w.part + w.sp * cand + (size_t)q * 16 + t, (long long)s.NTc * 16, J0 / CH,
-                  J1 / CH);
-    if (k >= 2 * s.n - 1) b = 0.0;
+  if (k < 2 * s.n - 1) {
+    const int j = k >> 8;
+    int lo, hi;
+    autoconv_drange(j, s.N16, lo, hi);
+    double r = lo <= hi ? chunk_sum(w.part + w.sp * cand + k, 256LL * s.NTc, lo / (CH * UW),
+                                    hi / (CH * UW))
+                        : 0.0;
+    const int K = k >> 4, row = k & 15, dp = (K + 1) >> 1;
+    const int i0 = 16 * (dp - 1) + row + 1;
+    const int cnt = ((k + 1) >> 1) - i0;
+#pragma unroll
+    for (int u = 0; u < 15; ++u)
+      if (u < cnt) r += sa[i0 + u - base] * sa[k - i0 - u - base];
+    b = r + r;
+    if (!(k & 1)) {
+      const double x = sa[(k >> 1) - base];
+      b += x * x;
+    }
   }
   if (k < s.LB) w.bc[w.sb * cand + k] = b;
   const double m = block_max<double, NW>(b, slots);
@@ -244,8 +347,8 @@ __global__ __launch_bounds__(RB) void k_polish_conv_reduce(Shape s, Ws w) {
 
 // -------------------------------------------------------------- weights --
 // c_k = exp((p-1) log1p((b_k - M) / M)) in place of b_k; R partial sums of
-// c (1 + d) per workgroup (added in workgroup order by the update).
-__global__ __launch_bounds__(RB) void k_polish_weights(Shape s, Ws w, Sched S) {
+// c (1 + d) per workgroup (added in workgroup order by the step kernel).
+__global__ __launch_bounds__(RB) void k_polish_weights(Shape s, Ws w, Sched S, int it) {
   constexpr int NW = RB / kWave;
   __shared__ double slots[NW];
   __shared__ double mslots[NW];
@@ -260,12 +363,12 @@ __global__ __launch_bounds__(RB) void k_polish_weights(Shape s, Ws w, Sched S) {
   const double M = block_max<double, NW>(mloc, mslots);
   if (blockIdx.x == 0 && threadIdx.x == 0)  // for the step kernel
     w.mbits[cand] = (unsigned long long)__double_as_longlong(M);
-  if (!(M > 0.0)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) w.stopped[cand] = 1;  // the fused kernel's break
+  if (!(M > 0.0)) {  // the fused kernel's break: later kernels of the call skip this candidate
+    if (blockIdx.x == 0 && threadIdx.x == 0) w.stopped[cand] = it + 1;
     return;
   }
   double p, lr;
-  sched_at(S, w.step[cand], p, lr);
+  sched_at(S, it, p, lr);
   const double pm1 = p - 1.0;
   const int k = blockIdx.x * RB + threadIdx.x;
   double r = 0.0;
@@ -284,74 +387,94 @@ __global__ __launch_bounds__(RB) void k_polish_weights(Shape s, Ws w, Sched S) {
 }
 
 // --------------------------------------------------------- corr units ----
-// gpart[cand][cc][q][t] = sum over j in chunk cc of c_{16q+t+j} a_j.
-__global__ __launch_bounds__(PU) void k_polish_corr_units(Shape s, Ws w) {
-  __shared__ double sx[CH * 16 + CH];
-  __shared__ double sw[(16 * (PU + CH) + 32) * 17 / 16 + 16];
-  const int cand = blockIdx.z, cc = blockIdx.y;
-  if (w.stopped[cand]) return;
-  const int qa = PU * blockIdx.x;
-  if (qa >= s.NTg) return;
-  const int qb = qa + PU - 1 < s.NTg - 1 ? qa + PU - 1 : s.NTg - 1;
-  const double* ap = w.apad + w.sa * cand + PADA;
-  const double* cp = w.bc + w.sb * cand;
-  const int J0c = CH * cc;
-  for (int e = threadIdx.x; e < CH * 16; e += PU) sx[wslot(e)] = ap[16 * J0c + e];
-  // window: c_x for x in [16 (qa + J0c), 16 (qb + J0c + CH - 1) + 30]
-  const int wlo = 16 * (qa + J0c);
-  const int wlen = 16 * (qb - qa + CH) + 15;
-  for (int e = threadIdx.x; e < wlen; e += PU) sw[wslot(e)] = cp[wlo + e];
-  __syncthreads();
-  const int q = qa + (int)threadIdx.x;
-  if (q > qb) return;
-  double acc[16];
+// Hankel tile (the fused kernel's corr_tile) of output segments I0 + t over
+// steps e in [ea, eb): c from the plain window wc (element 0 = c_{cb}), a from
+// the B-operand window wa starting at segment ga.
+template <int D>
+__device__ __forceinline__ f64x4 polish_corr_tile(const double* wc, int cb, const double* wa,
+                                                  int ga, int I0, int ea, int eb) {
+  const int lane = opaque((int)threadIdx.x) & (kWave - 1);
+  const int t = lane & 15, kk = lane >> 4;
+  const int pc = 16 * ea + t + 4 * kk - cb;                 // c_{16e + t + s}
+  const int pb = bpos(16 * (ea - I0 - t - ga)) + 4 * kk;    // a_{16(e - I0 - t) + s}
+  return run_steps<double, D>(eb - ea, [&](int i, double (&x)[4], double (&y)[4]) {
 #pragma unroll
-  for (int t = 0; t < 16; ++t) acc[t] = 0.0;
-  for (int jj = 0; jj < CH; ++jj) {
-    double x[16], v[31];
-#pragma unroll
-    for (int u = 0; u < 16; ++u) x[u] = sx[wslot(16 * jj + u)];
-    const int base = 16 * (q + J0c + jj) - wlo;  // window offset of c_{16(q+J)}
-#pragma unroll
-    for (int m = 0; m < 31; ++m) v[m] = sw[wslot(base + m)];
-#pragma unroll
-    for (int u = 0; u < 16; ++u)
-#pragma unroll
-      for (int t = 0; t < 16; ++t) acc[t] = fma(x[u], v[t + u], acc[t]);
-  }
-  double* o = w.gpart + w.sg * cand + ((size_t)cc * s.NTg + q) * 16;
-#pragma unroll
-  for (int t = 0; t < 16; ++t) o[t] = acc[t];
+    for (int q = 0; q < 4; ++q) x[q] = wc[pc + 16 * i + q];
+    load4(wa + pb + i * kBStride, y);
+  });
 }
 
-// --------------------------------------------------------------- update --
-// Many workgroups per candidate.  First: R (rpart in order), G_i (chunks in
+// gpart[cand][g][256 j + 16 t + row] = the share of G_{16(16 j + t) + row} of
+// the Hankel steps e in [16 j + CH UW g, 16 j + CH UW (g + 1)) (within the
+// tile's N16 + 15).  Workgroup (j, g, cand); wave u takes chunk UW g + u.  A
+// unit whose weights are all zero (most of them at high p) adds zeros.
+__global__ __launch_bounds__(UW * kWave) void k_polish_corr_units(Shape s, Ws w, int it) {
+  constexpr int LENC = 16 * CH * UW + 31, LENA = CH * UW + 15;
+  constexpr int NT = UW * kWave, RC = (LENC + NT - 1) / NT;
+  __shared__ __align__(16) double wc[LENC + 1];
+  __shared__ __align__(16) double wa[bwin_len(LENA)];
+  static_assert(bwin_len(LENA) >= 256 * UW, "the group sum reuses wa");
+  const int cand = blockIdx.z, g = blockIdx.y, j = blockIdx.x;
+  const double* xs = w.x + w.sn * cand;
+  const double* c = w.bc + w.sb * cand;
+  const Norm nm = norm_at(w, cand, it, s.n);
+  const int I0 = 16 * j, e0 = I0 + CH * UW * g, eend = I0 + s.N16 + 15;
+  const int cb = 16 * e0;
+  const int ga = CH * UW * g - 15;
+  double cv[RC];
+#pragma unroll
+  for (int r = 0; r < RC; ++r) {
+    const int e = (int)threadIdx.x + r * NT;
+    cv[r] = (e < LENC && cb + e < s.LB) ? c[cb + e] : 0.0;
+  }
+  fill_bwindow<NT, LENA>(wa, xs, s.n, ga, nm);
+#pragma unroll
+  for (int r = 0; r < RC; ++r) {
+    const int e = (int)threadIdx.x + r * NT;
+    if (e < LENC) wc[e] = cv[r];
+  }
+  __syncthreads();
+  const int ea = e0 + CH * wave_id();
+  const int eb = ea + CH < eend ? ea + CH : eend;
+  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+  if (ea < eb && !w.stopped[cand]) {
+    // the weights this unit reads: wc[16 (ea - e0) .. + 16 (eb - ea) + 15)
+    const int c0 = 16 * (ea - e0), cl = 16 * (eb - ea) + 15;
+    bool nz = false;
+    for (int e = (int)threadIdx.x & (kWave - 1); e < cl; e += kWave) nz |= wc[c0 + e] != 0.0;
+    if (__ballot(nz)) acc = polish_corr_tile<Pipe<double>::kDepth>(wc, cb, wa, ga, I0, ea, eb);
+  }
+  __syncthreads();  // every wave is done with the windows
+  group_sum_store(wa, acc, w.gpart + w.sg * cand + (size_t)g * 256 * s.NTg + 256 * (size_t)j);
+}
+
+// ----------------------------------------------------------------- step --
+// Many workgroups per candidate: R (rpart, fixed order), G_i (group partials in
 // order), g = G/R - M, the Adam step (lr/n) mhat / (sqrt(vhat) + 1e-30) and
 // clamp >= 0 (mode 0) or the mirror step a exp(clamp(-lr g / M, +-4)) (mode
-// 1) into `out`, partial sums of the new heights per workgroup.  The bias
-// corrections use beta^t kept in the workspace, multiplied once a step as the
-// fused kernel does.
-__global__ __launch_bounds__(RB) void k_polish_step(Shape s, Ws w, Sched S, int mode,
-                                                    double* __restrict__ pop) {
+// 1) into x, and partial sums of the new heights per workgroup (the next
+// step's S).  b1t, b2t are beta^(it+1), multiplied up once a step by the host
+// as the fused kernel does.
+__global__ __launch_bounds__(RB) void k_polish_step(Shape s, Ws w, Sched S, int mode, int it,
+                                                    double b1t, double b2t) {
   constexpr int NW = RB / kWave;
   __shared__ double slots[NW];
   const int cand = blockIdx.y;
   if (w.stopped[cand]) return;
   const int n = s.n;
+  const Norm nm = norm_at(w, cand, it, n);
   const double M = __longlong_as_double((long long)w.mbits[cand]);
-  double R = 0.0;
-  for (int b = 0; b < w.sr; ++b) R += w.rpart[w.sr * cand + b];
+  const double R = fixed_sum(w.rpart + w.sr * cand, (int)w.sr);
   double p, lr;
-  sched_at(S, w.step[cand], p, lr);
-  const double b1t = w.bias[2 * cand] * S.beta1, b2t = w.bias[2 * cand + 1] * S.beta2;
+  sched_at(S, it, p, lr);
   const double bc1 = 1.0 / (1.0 - b1t), bc2 = 1.0 / (1.0 - b2t);
   const int i = blockIdx.x * RB + threadIdx.x;
   double xx = 0.0;
   if (i < n) {
-    const double G = chunk_sum(w.gpart + w.sg * cand + (size_t)(i >> 4) * 16 + (i & 15),
-                               (long long)s.NTg * 16, 0, s.NCg - 1);
+    const double G = chunk_sum(w.gpart + w.sg * cand + i, 256LL * s.NTg, 0, s.NGg - 1);
     const double gi = G / R - M;
-    const double ai = w.apad[w.sa * cand + PADA + i];
+    double* xs = w.x + w.sn * cand;
+    const double ai = nm.of(xs[i]);
     if (mode == 0) {
       double* mo = w.mom + w.sn * cand;
       double* ve = w.vel + w.sn * cand;
@@ -368,39 +491,19 @@ __global__ __launch_bounds__(RB) void k_polish_step(Shape s, Ws w, Sched S, int 
       if (e < -4.0) e = -4.0;
       xx = ai * exp(e);
     }
-    pop[(size_t)cand * n + i] = xx;
+    xs[i] = xx;  // this thread alone reads and writes x_i
   }
   const double part = block_sum<double, NW>(xx, slots);
-  if (threadIdx.x == 0) w.spart[w.ss * cand + blockIdx.x] = part;
+  if (threadIdx.x == 0) w.spart[w.ss * (2 * cand + (it & 1)) + blockIdx.x] = part;
 }
 
-// Second: S (spart in order), renormalise sum a = 1 (uniform 1/n if S is not
-// positive) into pop and the padded copy; the last workgroup's thread 0
-// resets M and advances the step and the bias products.
-__global__ __launch_bounds__(RB) void k_polish_renorm(Shape s, Ws w, Sched S,
-                                                      double* __restrict__ pop) {
+// The heights after the last step a candidate took, normalised, into pop.
+__global__ void k_polish_finish(Shape s, Ws w, int iters, double* __restrict__ pop) {
   const int cand = blockIdx.y;
-  if (w.stopped[cand]) return;
-  const int n = s.n;
-  double Ssum = 0.0;
-  for (int b = 0; b < w.ss; ++b) Ssum += w.spart[w.ss * cand + b];
-  const bool reset = !(Ssum > 0.0);
-  const double inv = reset ? 1.0 : 1.0 / Ssum;
-  const int i = blockIdx.x * RB + threadIdx.x;
-  if (i < n) {
-    double* out = pop + (size_t)cand * n;
-    const double x = reset ? 1.0 / (double)n : out[i] * inv;
-    out[i] = x;
-    w.apad[w.sa * cand + PADA + i] = x;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    // every workgroup of this candidate has read M and the step in the
-    // previous kernel; this one reads neither
-    w.mbits[cand] = 0ull;
-    w.step[cand] += 1;
-    w.bias[2 * cand] *= S.beta1;
-    w.bias[2 * cand + 1] *= S.beta2;
-  }
+  const int st = w.stopped[cand];
+  const Norm nm = norm_at(w, cand, st ? st - 1 : iters, s.n);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x)
+    pop[(size_t)cand * s.n + i] = nm.of(w.x[w.sn * cand + i]);
 }
 
 __global__ void k_polish_moments(const Ws w, int n, int K, double* mom, double* vel) {
@@ -445,23 +548,26 @@ extern "C" int sidon_polish_f64(double* pop, double* mom, double* vel, int n, in
   S.lnLr = log(lrEnd / lrStart);
   S.beta1 = beta1;
   S.beta2 = beta2;
-  const int setup_x = (s.LA + 255) / 256 < 64 ? (s.LA + 255) / 256 : 64;
-  hipLaunchKernelGGL(k_polish_setup, dim3(setup_x, K), dim3(256), 0, stream, pop, n, w);
+  const int ux = (n + 255) / 256 < 64 ? (n + 255) / 256 : 64;
+  hipLaunchKernelGGL(k_polish_setup, dim3(ux, K), dim3(256), 0, stream, pop, n, w);
   SIDON_HIP_TRY(hipGetLastError());
-  // conv units: tiles meeting a chunk number at most NJ + 35
-  const dim3 gconv((s.NJ + 35 + PU - 1) / PU, s.NCc, K);
-  const dim3 gcorr((s.NTg + PU - 1) / PU, s.NCg, K);
+  const dim3 gconv(s.NTc, s.NGc, K);
+  const dim3 gcorr(s.NTg, s.NGg, K);
   const dim3 gred(rblocks(s), K);
   const dim3 gupd(sblocks(s), K);
+  double b1t = 1.0, b2t = 1.0;
   for (int it = 0; it < iters; ++it) {
-    hipLaunchKernelGGL(k_polish_conv_units, gconv, dim3(PU), 0, stream, s, w);
-    hipLaunchKernelGGL(k_polish_conv_reduce, gred, dim3(RB), 0, stream, s, w);
-    hipLaunchKernelGGL(k_polish_weights, gred, dim3(RB), 0, stream, s, w, S);
-    hipLaunchKernelGGL(k_polish_corr_units, gcorr, dim3(PU), 0, stream, s, w);
-    hipLaunchKernelGGL(k_polish_step, gupd, dim3(RB), 0, stream, s, w, S, mode, pop);
-    hipLaunchKernelGGL(k_polish_renorm, gupd, dim3(RB), 0, stream, s, w, S, pop);
+    b1t *= beta1;
+    b2t *= beta2;
+    hipLaunchKernelGGL(k_polish_conv_units, gconv, dim3(UW * kWave), 0, stream, s, w, it);
+    hipLaunchKernelGGL(k_polish_conv_reduce, gred, dim3(RB), 0, stream, s, w, it);
+    hipLaunchKernelGGL(k_polish_weights, gred, dim3(RB), 0, stream, s, w, S, it);
+    hipLaunchKernelGGL(k_polish_corr_units, gcorr, dim3(UW * kWave), 0, stream, s, w, it);
+    hipLaunchKernelGGL(k_polish_step, gupd, dim3(RB), 0, stream, s, w, S, mode, it, b1t, b2t);
     SIDON_HIP_TRY(hipGetLastError());
   }
+  hipLaunchKernelGGL(k_polish_finish, dim3(ux, K), dim3(256), 0, stream, s, w, iters, pop);
+  SIDON_HIP_TRY(hipGetLastError());
   if (mom || vel) {
     hipLaunchKernelGGL(k_polish_moments, dim3(64), dim3(256), 0, stream, w, n, K, mom, vel);
     SIDON_HIP_TRY(hipGetLastError());
