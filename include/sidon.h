@@ -264,6 +264,10 @@ int sidon_measure_fp32_ceilings(float* scratch, int scratch_len, double* valu_tf
  * count excludes correlation steps skipped because their weights are all
  * zero.  Synchronous.  For roofline accounting; no reference counterpart. */
 int sidon_ls_block_steps(unsigned long long* steps, int reset);
+/* The same count for sidon_frank_wolfe: Toeplitz steps of the dual updates,
+ * of the M and line-search autoconvolutions, and the correlation steps run
+ * (steps over all-zero multipliers are skipped).  No reference counterpart. */
+int sidon_fw_block_steps(unsigned long long* steps, int reset);
 
 /* ---- autotuner ---------------------------------------------------------- */
 

@@ -38,7 +38,7 @@ EXPORTS = (
     "sidon_generation_create", "sidon_generation_run", "sidon_generation_current",
     "sidon_generation_enqueue", "sidon_generation_destroy", "sidon_tune_local_search",
     "sidon_measure_fp32_ceilings",
-    "sidon_ls_block_steps",
+    "sidon_ls_block_steps", "sidon_fw_block_steps",
     "sidon_policy_create", "sidon_policy_destroy", "sidon_policy_adopt", "sidon_policy_new_level",
     "sidon_policy_observe", "sidon_policy_plan", "sidon_policy_credit", "sidon_policy_stats",
     "sidon_policy_tiers",
@@ -130,6 +130,7 @@ def _declare(L):
                                     _i),
         "sidon_measure_fp32_ceilings": ([_vp, _i, C.POINTER(_d), C.POINTER(_d), _vp], _i),
         "sidon_ls_block_steps": ([C.POINTER(C.c_ulonglong), _i], _i),
+        "sidon_fw_block_steps": ([C.POINTER(C.c_ulonglong), _i], _i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -466,6 +467,15 @@ def ls_block_steps(reset: bool = False) -> int:
     _need_gpu()
     out = C.c_ulonglong()
     _check(lib().sidon_ls_block_steps(C.byref(out), int(reset)), "sidon_ls_block_steps")
+    return out.value
+
+
+def fw_block_steps(reset: bool = False) -> int:
+    """Matrix block steps (4 x 2048 FLOP each) the Frank-Wolfe kernel has
+    executed on the current device since the last reset (synchronous)."""
+    _need_gpu()
+    out = C.c_ulonglong()
+    _check(lib().sidon_fw_block_steps(C.byref(out), int(reset)), "sidon_fw_block_steps")
     return out.value
 
 

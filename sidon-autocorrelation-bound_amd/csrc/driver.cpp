@@ -411,6 +411,9 @@ int fw_ladder(const Args& A) {
     CHECK_HIP(hipEventCreate(&e0));
     CHECK_HIP(hipEventCreate(&e1));
     CHECK_SIDON(sidon_eval_f64(pop, n, P, F, nullptr, 1e-9, s));
+    CHECK_HIP(hipStreamSynchronize(s));
+    unsigned long long steps = 0;
+    CHECK_SIDON(sidon_fw_block_steps(&steps, 1));
     CHECK_HIP(hipEventRecord(e0, s));
     const int rc = sidon_frank_wolfe(pop, nullptr, n, P, A.fw_rounds, A.fw_iters, A.nT, A.t_max, s);
     CHECK_HIP(hipEventRecord(e1, s));
@@ -424,6 +427,8 @@ int fw_ladder(const Args& A) {
     CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
     CHECK_SIDON(sidon_eval_f64(pop, n, P, F2, nullptr, 1e-9, s));
     CHECK_HIP(hipStreamSynchronize(s));
+    CHECK_SIDON(sidon_fw_block_steps(&steps, 1));
+    const double exec_tflops = 8192.0 * (double)steps / (ms / 1e3) / 1e12;
     std::vector<double> f0(P), f1(P);
     CHECK_HIP(hipMemcpy(f0.data(), F, P * sizeof(double), hipMemcpyDeviceToHost));
     CHECK_HIP(hipMemcpy(f1.data(), F2, P * sizeof(double), hipMemcpyDeviceToHost));
@@ -434,10 +439,11 @@ int fw_ladder(const Args& A) {
     const double cp_flop = 4.0 * n * (double)n * A.fw_iters * A.fw_rounds * P;
     std::printf("{\"level\": %d, \"fw_rounds\": %d, \"cp_iters\": %d, \"ms\": %.1f, "
                 "\"cp_iterations_per_s\": %.1f, \"cp_dense_equivalent_tflops\": %.2f, "
-                "\"mean_F_before\": %.9f, "
+                "\"executed_block_steps\": %llu, \"executed_tflops\": %.2f, "
+                "\"executed_frac_fp32_peak\": %.4f, \"mean_F_before\": %.9f, "
                 "\"mean_F_after\": %.9f, \"best_F\": %.12f}\n",
                 n, A.fw_rounds, A.fw_iters, ms, (double)A.fw_iters * A.fw_rounds * P / (ms / 1e3),
-                cp_flop / (ms / 1e3) / 1e12, m0, m1, b1);
+                cp_flop / (ms / 1e3) / 1e12, steps, exec_tflops, exec_tflops / 157.3, m0, m1, b1);
     if (lvl + 1 < L.size()) {
       if (L[lvl + 1] != 2 * n) { std::fprintf(stderr, "ladder must double\n"); return 2; }
       CHECK_SIDON(sidon_eval_exact(pop, n, P, F, nullptr, nullptr, nullptr, s));

@@ -29,6 +29,12 @@ namespace {
 
 constexpr int kMaxConvTiles = 80;   // n <= 10240
 
+// Matrix block steps (four MFMAs, 8192 FLOP a wave) the Frank-Wolfe kernel has
+// run since the last sidon_fw_block_steps(reset): Toeplitz steps of the dual
+// updates, the M and line-search autoconvolutions, and the correlation steps
+// actually executed (steps over all-zero multipliers are skipped).
+__device__ unsigned long long g_fw_block_steps = 0ull;
+
 template <int BD>
 __host__ __device__ inline long long fw_lds_bytes(int n) {
   const long long f = 2LL * pad_len(n) + cbuf_len(n) + 2 * (BD / kWave);
@@ -63,6 +69,9 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
   const int LA = pad_len(n), LC = cbuf_len(n);
   const int N16 = nseg16(n), NTc = conv_tiles(n), NTg = corr_tiles(n);
   const int nb = 2 * n - 1;
+  int U = 0;  // block steps of one full autoconvolution
+  for (int j = 0; j < NTc; ++j) U += conv_cost(j, N16);
+  unsigned long long ran = 0ull;  // block steps this wave ran (wave 0 also counts conv_mfma)
 
   for (int i = tid; i < LA; i += BD) { sA[i] = 0.0f; sH[i] = 0.0f; }
   for (int i = tid; i < LC; i += BD) sL[i] = 0.0f;
@@ -96,6 +105,7 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
 
   for (int r = 0; r < A.rounds; ++r) {
     conv_mfma<float>(sA, sA, sL, n, wave, NW);
+    if (wave == 0) ran += U;
     __syncthreads();
     float loc = 0.0f;
     for (int k = tid; k < nb; k += BD) loc = sL[k] > loc ? sL[k] : loc;
@@ -127,6 +137,7 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
         int dlo, dhi;
         conv_drange(j, N16, dlo, dhi);
         const f32x4 acc = conv_tile<float>(sA, sH, 16 * j, dlo, dhi + 1);
+        ran += dhi + 1 - dlo;
         const int k0 = 16 * (16 * j + t) + 4 * kk;
         bool nz = false;
 #pragma unroll
@@ -153,6 +164,7 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
         const int I0 = 16 * j;
         int steps = 0;
         const f32x4 c = corr_tile_sparse<float>(sL, sA, I0, I0, I0 + N16 + 15, mcur, steps);
+        ran += steps;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int i = 16 * (I0 + t) + 4 * kk + q;
@@ -198,6 +210,7 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
       for (int k = tid; k < LC; k += BD) sL[k] = 0.0f;
       __syncthreads();
       conv_mfma<float>(sH, sH, sL, n, wave, NW);
+      if (wave == 0) ran += U;
       __syncthreads();
       float lc = 0.0f;
       for (int k = tid; k < nb; k += BD) lc = sL[k] > lc ? sL[k] : lc;
@@ -220,6 +233,7 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
   }
   __syncthreads();
   for (int i = tid; i < n; i += BD) a[i] = sA[pad_addr(i)];
+  if (lane == 0 && ran) atomicAdd(&g_fw_block_steps, ran);
 }
 
 template <int BD, int GT>
@@ -255,6 +269,18 @@ extern "C" long long sidon_fw_lds_bytes_impl(int n, int block) {
     case 1024: return fw_lds_bytes<1024>(n);
     default: return -1;
   }
+}
+
+extern "C" int sidon_fw_block_steps(unsigned long long* out, int reset) {
+  if (!out) return SIDON_ERR_ARG;
+  int rc = SIDON_OK;
+  if (!device_state(&rc)) return rc;
+  SIDON_HIP_TRY(hipMemcpyFromSymbol(out, HIP_SYMBOL(g_fw_block_steps), sizeof(*out)));
+  if (reset) {
+    const unsigned long long zero = 0ull;
+    SIDON_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_fw_block_steps), &zero, sizeof(zero)));
+  }
+  return SIDON_OK;
 }
 
 extern "C" int sidon_frank_wolfe(float* pop, const int* skip, int n, int P, int rounds,

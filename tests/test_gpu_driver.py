@@ -132,6 +132,7 @@ def test_ladder_splits_preserve_F_exactly():
         assert r["F_bit_exact"] == r["of"] == 32 and r["ambiguous"] == 0
     for r in levels:
         assert r["mean_F_after"] <= r["mean_F_before"] + 1e-12
+        assert r["executed_block_steps"] > 0 and 0 < r["executed_tflops"] < 157.3
 
 
 def test_bench_mode():
