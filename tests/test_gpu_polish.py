@@ -40,6 +40,25 @@ def test_polish_matches_fused_fp64(S, W, n, mode):
                                    atol=1e-8 * np.abs(m1.cpu().numpy()).max())
 
 
+@pytest.mark.parametrize("n", [1, 7, 17, 255, 257, 1000])
+def test_polish_ragged_lengths_match_fused_fp64(S, n):
+    """Lengths that end inside a 16-segment, a 256-output tile or a chunk group;
+    uniform-random starts with exact zeros (not the structured prior)."""
+    print(f"[polish] ragged n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    a = rng.random((2, n)) * (rng.random((2, n)) > 0.3)
+    a[:, 0] += 0.1                                        # no all-zero row
+    a /= a.sum(axis=1, keepdims=True)
+    sched = (200.0, 6000.0, 0.01, 0.0002)
+    t1, t2 = dev(a), dev(a)
+    S.local_search(t1, 15, *sched)
+    S.polish_f64(t2, 15, *sched)
+    torch.cuda.synchronize()
+    x1, x2 = t1.cpu().numpy(), t2.cpu().numpy()
+    assert (x2 >= 0).all() and np.allclose(x2.sum(axis=1), 1.0, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(x2, x1, rtol=0, atol=1e-10 * x1.max())
+
+
 def test_polish_at_8400_matches_oracle(S, O, W):
     print("[polish] n=8400 vs oracle", flush=True)
     n = 8400
