@@ -288,6 +288,114 @@ __global__ __launch_bounds__(BD) void k_eval(const Tin* __restrict__ pop, long l
   }
 }
 
+// ------------------------------------------------------- packed, small n --
+// With fewer than 128 tiles a 64-lane workgroup deals one tile a lane, and
+// the lanes wait for the longest (tile work is a triangle: 1 .. n/16 trips).
+// Here G candidates share a 64-lane workgroup, L = 64 / G lanes each, so every
+// lane walks at least two tiles in the balanced order of k_eval.  Each
+// candidate has its own LDS region with k_eval's layout (sized for n_region,
+// the largest length); per output the arithmetic is k_eval's, and the
+// reductions run over the candidate's L lanes (xor butterflies below L stay
+// inside the group).
+template <int L>
+__device__ __forceinline__ double lanes_max(double v) {
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) {
+    const double w = __shfl_xor(v, o, kWave);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+template <int L>
+__device__ __forceinline__ double lanes_sum(double v) {
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+template <typename Tin, int G>
+__global__ __launch_bounds__(kWave) void k_eval_packed(const Tin* __restrict__ pop,
+                                                       long long stride,
+                                                       const int* __restrict__ ns, int n_region,
+                                                       int P, double* F, int* nActive,
+                                                       double tol) {
+  constexpr int L = kWave / G;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x, g = tid / L, l = tid % L;
+  const int c = G * (int)blockIdx.x + g;
+  const bool live = c < P;
+  const int n = live ? (ns ? ns[c] : n_region) : 1;
+  double* E = reinterpret_cast<double*>(smem) + (long long)g * eval_lds_elems(n_region, kWave);
+  const int NT = eval_tiles(n);
+  const int nb = 2 * n - 1;
+  float* tmaxf = reinterpret_cast<float*>(E + eval_slots(n));  // NT floats
+  double* cand = E + eval_slots(n) + (NT + 1) / 2;
+  const int C = NT - (NT + 1) / 2 + 16;
+  double* mrun = cand + C + 2;                  // (k_eval's two reduction slots are unused)
+  int* cnt = reinterpret_cast<int*>(mrun + 1);  // [1] candidates
+  if (live) load_heights(pop + (size_t)c * stride, n, E, l, L);
+  if (l == 0) {
+    cnt[1] = 0;
+    *mrun = 0.0;
+  }
+  __syncthreads();
+  const bool listed = nActive && tol >= 0.0 && tol <= 1.0;
+  const double keep = 1.0 - tol;
+  double loc = 0.0;
+  double acc[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) acc[t] = 0.0;
+  if (live)
+    walk_tiles(
+        n, NT, l, L, [&](int J, int d) { eval_trip(E, J, d, acc); },
+        [&](int tq) {
+          double mx = 0.0;
+#pragma unroll
+          for (int t = 0; t < 16; ++t) mx = acc[t] > mx ? acc[t] : mx;
+          float mf = (float)mx;
+          if ((double)mf < mx) mf = nextafterf(mf, HUGE_VALF);
+          tmaxf[tq] = mf;
+          loc = mx > loc ? mx : loc;
+          if (listed) {
+            const double old = __longlong_as_double((long long)atomicMax(
+                reinterpret_cast<unsigned long long*>(mrun),
+                static_cast<unsigned long long>(__double_as_longlong(mx))));
+            const double lb = (old > mx ? old : mx) * keep;
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+              if (16 * tq + t < nb && acc[t] >= lb) {
+                const int pos = atomicAdd(&cnt[1], 1);
+                if (pos < C) cand[pos] = acc[t];
+              }
+          }
+#pragma unroll
+          for (int t = 0; t < 16; ++t) acc[t] = 0.0;
+        });
+  const double S = (live && l == 0) ? height_sum(E, n) : 0.0;  // sequential, hostF order
+  const double M = lanes_max<L>(loc);
+  __syncthreads();  // every tile's list entries and maxima are in LDS
+  if (nActive) {
+    const double thr = M * keep;
+    const int nc = cnt[1];
+    double k_on = 0.0;
+    if (live) {
+      if (listed && nc <= C) {
+        for (int j = l; j < nc; j += L) k_on += cand[j] >= thr ? 1.0 : 0.0;
+      } else {
+        for (int q = l; q < NT; q += L)
+          if ((double)tmaxf[q] >= thr)
+            for (int t = 0; t < 16; ++t) {
+              const int k = 16 * q + t;
+              if (k < nb && output_b(E, n, k) >= thr) k_on += 1.0;
+            }
+      }
+    }
+    const double total = lanes_sum<L>(k_on);
+    if (live && l == 0) nActive[c] = (int)(total + 0.5);
+  }
+  if (live && l == 0) F[c] = S > 0.0 ? 2.0 * (double)n * M / (S * S) : 1e30;
+}
+
 // ------------------------------------------------------------ exact F ----
 #pragma clang fp contract(off)
 __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
@@ -459,11 +567,34 @@ int launch_eval(const char* what, Kernel kfn, int n, int P, hipStream_t stream, 
   return SIDON_OK;
 }
 
+// G candidates a 64-lane workgroup (k_eval_packed) for n with fewer than 128
+// tiles: the smallest G with at least two tiles a lane (G <= 8).
+template <int G, typename Tin>
+int launch_packed(const Tin* pop, long long stride, const int* ns, int n, int P, double* F,
+                  int* nActive, double tol, hipStream_t stream) {
+  int rc = SIDON_OK;
+  DeviceState* ds = device_state(&rc);
+  if (!ds) return rc;
+  const long long lds = G * eval_lds_elems(n, kWave) * (long long)sizeof(double);
+  if (lds > ds->lds_max) return SIDON_ERR_LDS;  // not reached: n < 1032 here
+  auto kfn = k_eval_packed<Tin, G>;
+  SIDON_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kfn),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kfn, dim3((P + G - 1) / G), dim3(kWave), (size_t)lds, stream, pop, stride,
+                     ns, n, P, F, nActive, tol);
+  SIDON_HIP_TRY(hipGetLastError());
+  return SIDON_OK;
+}
+
 template <typename Tin>
 int eval_common(const Tin* pop, long long stride, const int* ns, int n, int P, double* F,
                 int* nActive, double tol, hipStream_t stream) {
   if (!pop || !F || n < 1 || P < 0 || stride < n) return SIDON_ERR_ARG;
   if (P == 0) return SIDON_OK;
+  const int NT = eval_tiles(n);
+  if (NT < 32) return launch_packed<8>(pop, stride, ns, n, P, F, nActive, tol, stream);
+  if (NT < 64) return launch_packed<4>(pop, stride, ns, n, P, F, nActive, tol, stream);
+  if (NT < 128) return launch_packed<2>(pop, stride, ns, n, P, F, nActive, tol, stream);
   switch (eval_block(n, 1024)) {
     case 1024:
       return launch_eval<1024>("eval", k_eval<Tin, 1024>, n, P, stream, pop, stride, ns, n, F,
