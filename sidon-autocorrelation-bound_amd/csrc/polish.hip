@@ -152,19 +152,19 @@ __device__ __forceinline__ double chunk_sum(const double* p, long long stride, i
   return r;
 }
 
-// LDS windows of a in the overlapped segment layout, without its left
-// padding: window element x (a_{16 g + x}) sits at wpos(x) = pad_addr(x) -
-// 15 kSegStride for x in [-4, 16 LEN), so the buffer holds (LEN + 2) segments,
-// then one zero segment (the masked reads of the symmetric tile land there).
+// Toeplitz A-operand windows: a in the overlapped segment layout, without its
+// left padding: window element x (a_{16 g + x}) sits at wpos(x) = pad_addr(x) -
+// 15 kSegStride for x in [-4, 16 LEN), so the buffer holds LEN + 2 segments.
 constexpr int kWinShift = 15 * kSegStride;
-__host__ __device__ constexpr int win_len(int len) { return (len + 3) * kSegStride; }
+__host__ __device__ constexpr int win_len(int len) { return (len + 2) * kSegStride; }
 __device__ __forceinline__ int wpos(int x) { return pad_addr(x) - kWinShift; }
 
 // Matrix B-operand windows (a lane reads four heights inside one segment, so
 // no copies are needed): segment x/16 of the window at 18 (x/16) doubles.
 // Sixteen lanes one segment apart then cover all 64 LDS banks once per
 // 16-byte read (a 20-double stride would put lanes t and t + 8 on the same
-// banks).  LEN segments, then one zero segment.
+// banks).  LEN segments, then one zero segment (the masked reads of the
+// symmetric tile land there).
 constexpr int kBStride = 18;
 __host__ __device__ constexpr int bwin_len(int len) { return (len + 1) * kBStride; }
 __device__ __forceinline__ int bpos(int x) { return (x >> 4) * kBStride + (x & 15); }
@@ -188,9 +188,9 @@ __device__ __forceinline__ void fill_bwindow(double* win, const double* __restri
   }
 }
 
-// Fills a window from the unnormalised heights x (normalised on the way in,
-// zero outside [0, n)); NT threads, every load in flight before the first
-// store.
+// Fills an A-operand window from the unnormalised heights x (normalised on the
+// way in, zero outside [0, n)); NT threads, every load in flight before the
+// first store.  (fill_bwindow above does the same for B-operand windows.)
 template <int NT, int LEN>
 __device__ __forceinline__ void fill_window(double* win, const double* __restrict__ xs, int n,
                                             int g, const Norm& nm) {
@@ -202,7 +202,6 @@ __device__ __forceinline__ void fill_window(double* win, const double* __restric
     const int i = 16 * g + x;
     v[r] = (x < 16 * LEN && i >= 0 && i < n) ? nm.of(xs[i]) : 0.0;
   }
-  if (threadIdx.x < kSegStride) win[(LEN + 2) * kSegStride + threadIdx.x] = 0.0;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int x = (int)threadIdx.x + r * NT - 4;
