@@ -84,6 +84,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
   for (int i = tid; i < LA; i += BD) sA[i] = T(0);
   for (int i = tid; i < LC; i += BD) sC[i] = T(0);
   for (int i = tid; i < LG; i += BD) sG[i] = T(0);
+  for (int w = tid; w < SW; w += BD) segmask[w] = 0u;
   __syncthreads();
   for (int i = tid; i < n; i += BD) pad_put(sA, i, a[i]);
 
@@ -108,7 +109,6 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     __syncthreads();
 
     if (tid == 0) ctr[0] = 0;  // every wave is past the autoconvolution
-    for (int w = tid; w < SW; w += BD) segmask[w] = 0u;  // the correlation is done with it
     const int st = opaque(tid);  // per-step copy: nothing derived from it is hoisted
     const T M = *bmax;
     if (!(M > T(0))) break;
@@ -143,6 +143,10 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     __syncthreads();
 
     if (tid == 0) { ctr[1] = 0; *bmax = T(0); }  // every wave is past the correlation (and M)
+    // the correlation is done with the segment bits; clearing them here, three
+    // barriers before the next step's weight pass sets them, keeps the clear
+    // from racing with another wave's atomicOr
+    for (int w = tid; w < SW; w += BD) segmask[w] = 0u;
     T sloc = T(0);
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) {
@@ -199,10 +203,10 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
   __syncthreads();  // LDS is reused by the next candidate of a persistent block
 }
 
+// One candidate per workgroup (grid = candidates), or workgroups pulling
+// candidates from a device queue (persistent).
 template <typename T, int BD, int QMAX>
-__global__ __launch_bounds__(BD) void k_local_search(LsArgs<T> A) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int s_item;
+__device__ __forceinline__ void ls_body(const LsArgs<T>& A, unsigned char* smem, int& s_item) {
   if (!A.queue) {
     const int cand = A.order ? A.order[blockIdx.x] : (int)blockIdx.x;
     ls_candidate<T, BD, QMAX>(A, cand, smem);
@@ -217,6 +221,24 @@ __global__ __launch_bounds__(BD) void k_local_search(LsArgs<T> A) {
     const int cand = A.order ? A.order[item] : item;
     ls_candidate<T, BD, QMAX>(A, cand, smem);
   }
+}
+
+template <typename T, int BD, int QMAX>
+__global__ __launch_bounds__(BD) void k_local_search(LsArgs<T> A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int s_item;
+  ls_body<T, BD, QMAX>(A, smem, s_item);
+}
+
+// The same kernel held to 128 registers a lane (four waves a SIMD): two
+// 512-thread candidates then share a CU where their LDS allows it (n <= 4096),
+// and one candidate's elementwise phases overlap the other's matrix work.
+template <typename T, int BD, int QMAX>
+__global__ __launch_bounds__(BD) __attribute__((amdgpu_waves_per_eu(4, 4))) void
+k_local_search_pair(LsArgs<T> A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int s_item;
+  ls_body<T, BD, QMAX>(A, smem, s_item);
 }
 
 // ---------------------------------------------------------------- lp_grad --
@@ -291,7 +313,10 @@ struct BlockRule { int n, block; };
 BlockRule g_rules[64];
 int g_nrules = 0;
 
-int default_block(int n) { return n >= 3072 ? 1024 : (n >= 1024 ? 512 : 256); }
+// 512 threads from n = 512 up to 4096, where the register-capped kernel fits
+// two candidates a CU (measured faster than 256 and 1024 at n = 600 ... 4096);
+// 1024 above, where one candidate fills a CU's LDS.
+int default_block(int n) { return n > 4096 ? 1024 : (n >= 512 ? 512 : 256); }
 
 int pick_block(int nmax) {
   for (int r = 0; r < g_nrules; ++r)
@@ -299,7 +324,7 @@ int pick_block(int nmax) {
   return default_block(nmax);
 }
 
-template <typename T, int BD, int QMAX>
+template <typename T, int BD, int QMAX, bool PAIR = false>
 int launch_ls(const LsArgs<T>& A, int nmax, int grid, hipStream_t stream) {
   if (nmax > BD * QMAX) return SIDON_ERR_UNSUPPORTED;
   int rc = SIDON_OK;
@@ -310,7 +335,10 @@ int launch_ls(const LsArgs<T>& A, int nmax, int grid, hipStream_t stream) {
     set_last_error_msg("local search: n needs more LDS than a workgroup may use");
     return SIDON_ERR_LDS;
   }
-  auto kfn = k_local_search<T, BD, QMAX>;
+  auto kfn = [] {
+    if constexpr (PAIR) return k_local_search_pair<T, BD, QMAX>;
+    else return k_local_search<T, BD, QMAX>;
+  }();
   SIDON_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kfn),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(kfn, dim3(grid), dim3(BD), (size_t)lds, stream, A);
@@ -333,7 +361,10 @@ int dispatch_ls(LsArgs<T>& A, int nmax, int block, bool persistent, hipStream_t 
   switch (block) {
     case 256: return launch_ls<T, 256, 16>(A, nmax, grid, stream);
     case 512:
-      if constexpr (sizeof(T) == 4) return launch_ls<T, 512, 20>(A, nmax, grid, stream);
+      if constexpr (sizeof(T) == 4) {
+        if (nmax <= 512 * 8) return launch_ls<T, 512, 8, true>(A, nmax, grid, stream);
+        return launch_ls<T, 512, 20>(A, nmax, grid, stream);
+      }
       return launch_ls<T, 512, 12>(A, nmax, grid, stream);
     case 1024:
       if constexpr (sizeof(T) == 4) return launch_ls<T, 1024, 10>(A, nmax, grid, stream);

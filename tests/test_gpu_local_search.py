@@ -127,6 +127,29 @@ def test_deterministic_and_ragged_equals_uniform(S, W):
     assert torch.equal(tr[:, :n], t1)
 
 
+@pytest.mark.parametrize("n,block", [(600, 256), (600, 512), (2100, 512), (2100, 1024),
+                                     (4096, 512)])
+def test_repeated_launches_bitwise_every_block(S, W, n, block):
+    """Regression: the weight-segment bits were once cleared in the same barrier
+    interval in which other waves set them, so a fast wave's bits could be lost
+    and correlation steps with non-zero weights skipped (seen at small n as
+    launches that disagreed).  Many candidates, the full anneal to p = 1e4
+    (sparse weights), every workgroup size, three launches: identical bits."""
+    print(f"[ls] repeated launches n={n} block={block}", flush=True)
+    a = dev(W.structured_prior(256, n, seed=11))
+    S.set_ls_block(n, block)
+    try:
+        runs = []
+        for _ in range(3):
+            t = a.clone()
+            S.local_search(t, 200, 4.0, 1e4, 0.085, 0.0015)
+            runs.append(t)
+        torch.cuda.synchronize()
+    finally:
+        S.set_ls_block(n, 0)
+    assert torch.equal(runs[0], runs[1]) and torch.equal(runs[0], runs[2])
+
+
 def test_ragged_uneven_lengths(S, W):
     """Uneven n in one persistent launch == each candidate run alone."""
     print("[ls] ragged uneven", flush=True)
