@@ -544,11 +544,12 @@ int search(const Args& A) {
   // and levels coarser than it are dropped (a split preserves F exactly, so
   // they cannot beat it)
   std::vector<double> seedVec;
-  if (!A.resume.empty()) {
-    if (!load_vec(A.resume, seedVec)) {
-      std::fprintf(stderr, "could not read %s\n", A.resume.c_str());
-      return 2;
-    }
+  if (!A.resume.empty() && !load_vec(A.resume, seedVec)) {
+    // the reference goes on without the vector (main.cu:424-426)
+    std::fprintf(stderr, "could not read %s, ignoring\n", A.resume.c_str());
+    seedVec.clear();
+  }
+  if (!seedVec.empty()) {
     size_t drop = 0;
     while (drop + 1 < L.size() && L[drop] < (int)seedVec.size()) ++drop;
     L.erase(L.begin(), L.begin() + drop);
@@ -1032,8 +1033,9 @@ int main(int argc, char** argv) {
   hipDeviceProp_t prop;
   CHECK_HIP(hipGetDevice(&dev));
   CHECK_HIP(hipGetDeviceProperties(&prop, dev));
-  std::printf("GPU: %s  %s  %d CUs  LDS per workgroup %d B\n", prop.name, prop.gcnArchName,
-              prop.multiProcessorCount, sidon_lds_limit());
+  std::printf("GPU: %s  %s  %d CUs  LDS per workgroup %d B\n",
+              prop.name[0] ? prop.name : "(unnamed)", prop.gcnArchName, prop.multiProcessorCount,
+              sidon_lds_limit());
   if (A.mode == "--selftest") return selftest();
   if (A.mode == "--bench") return bench_kernels();
   if (A.mode == "--search") {  // every level must fit the fused search (main.cu:390-399)

@@ -109,6 +109,19 @@ def test_resume_adopts_the_vector_and_drops_coarser_levels(O, certs, tmp_path):
     assert v["rc"] == 0 and v["n"] == final["n"]
 
 
+def test_unreadable_resume_is_ignored(tmp_path):
+    """As the reference (main.cu:424-426): a missing --resume file is reported
+    and the search runs without it."""
+    out = subprocess.run([str(DRIVER), "--pop", "32", "--islands", "4", "--ladder", "64,128",
+                          "--ls-iters", "20", "--cold-iters", "60", "--fw-rounds", "0",
+                          "--gens", "2", "--seconds", "30", "--polish-iters", "0",
+                          "--resume", str(tmp_path / "missing.txt"), "--out", str(tmp_path / "r")],
+                         capture_output=True, text=True, timeout=150)
+    assert out.returncode == 0, out.stderr[-2000:]
+    assert "could not read" in out.stderr and "ignoring" in out.stderr
+    assert '"resumed"' not in out.stdout and (tmp_path / "r.json").exists()
+
+
 def test_uneven_islands_graph():
     """Config 3 in small: uneven lengths, generations replayed from a HIP graph."""
     out = run(["--uneven", "--P", "64", "--islands", "4", "--iters", "40", "--gens", "2",
