@@ -27,13 +27,9 @@ void set_last_error(const char* where, hipError_t e) {
 }
 void set_last_error_msg(const char* msg) { std::snprintf(g_err, sizeof g_err, "%s", msg); }
 
-static int init_device(int dev) {
-  if (dev < 0 || dev >= kMaxDevices) return SIDON_ERR_ARG;
+// Fills g_states[dev] with `dev` current; on failure frees what it allocated.
+static int init_current(int dev) {
   DeviceState& s = g_states[dev];
-  if (s.device == dev) return SIDON_OK;
-  int cur = 0;
-  SIDON_HIP_TRY(hipGetDevice(&cur));
-  if (cur != dev) SIDON_HIP_TRY(hipSetDevice(dev));
   int lds_block = 0, lds_optin = 0, cus = 0;
   SIDON_HIP_TRY(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   if (hipDeviceGetAttribute(&lds_optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess)
@@ -42,14 +38,36 @@ static int init_device(int dev) {
   SIDON_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   int* q = nullptr;
   SIDON_HIP_TRY(hipMalloc(&q, sizeof(int) * kQueueSlots));
-  SIDON_HIP_TRY(hipMemset(q, 0, sizeof(int) * kQueueSlots));
-  SIDON_HIP_TRY(hipDeviceSynchronize());
+  hipError_t e = hipMemset(q, 0, sizeof(int) * kQueueSlots);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    set_last_error("hipMemset(work-queue counters)", e);
+    (void)hipFree(q);
+    return SIDON_ERR_HIP;
+  }
   s.lds_max = lds_optin > lds_block ? lds_optin : lds_block;
   s.cu_count = cus;
   s.queue = q;
-  s.device = dev;
-  if (cur != dev) SIDON_HIP_TRY(hipSetDevice(cur));
+  s.device.store(dev, std::memory_order_release);
   return SIDON_OK;
+}
+
+// Called with g_mu held.  Leaves the caller's current device as it was.
+static int init_device(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return SIDON_ERR_ARG;
+  if (g_states[dev].device.load(std::memory_order_acquire) == dev) return SIDON_OK;
+  int cur = 0;
+  SIDON_HIP_TRY(hipGetDevice(&cur));
+  if (cur != dev) SIDON_HIP_TRY(hipSetDevice(dev));
+  const int rc = init_current(dev);
+  if (cur != dev) {
+    const hipError_t e = hipSetDevice(cur);
+    if (rc == SIDON_OK && e != hipSuccess) {
+      set_last_error("hipSetDevice(restore)", e);
+      return SIDON_ERR_HIP;
+    }
+  }
+  return rc;
 }
 
 DeviceState* device_state(int* rc) {
@@ -61,7 +79,7 @@ DeviceState* device_state(int* rc) {
     return nullptr;
   }
   if (dev < 0 || dev >= kMaxDevices) { *rc = SIDON_ERR_ARG; return nullptr; }
-  if (g_states[dev].device != dev) {
+  if (g_states[dev].device.load(std::memory_order_acquire) != dev) {
     std::lock_guard<std::mutex> lk(g_mu);
     const int r = init_device(dev);
     if (r) { *rc = r; return nullptr; }

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
 
@@ -10,9 +11,11 @@
 namespace sidon {
 
 // Per-device persistent workspace (allocated once by sidon_init or on first
-// use on a device): work-queue counters for persistent launches.
+// use on a device): work-queue counters for persistent launches.  `device`
+// is stored last, with release order, once the other fields are complete;
+// readers that see it equal to their device may use the rest.
 struct DeviceState {
-  int device = -1;
+  std::atomic<int> device{-1};
   int lds_max = 0;        // max dynamic LDS bytes per workgroup
   int cu_count = 0;
   int* queue = nullptr;   // kQueueSlots work-queue counters
