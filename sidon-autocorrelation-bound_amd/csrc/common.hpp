@@ -606,6 +606,41 @@ __device__ __forceinline__ T autoconv_finish(const T* __restrict__ sA, T h, int 
   return b;
 }
 
+// autoconv_finish for the four outputs of a lane in the fp32 accumulator
+// layout (rows 4 kk + r of column K): output r adds a_{i0 + r + u} a_{16(K-dp)
+// + 15 - u} for u < cnt(r), with i0 = 16(dp-1) + 4kk + 1 the first index of
+// output 0, so the eighteen heights a_{i0 .. i0+17} and the fifteen a_{16(K-dp)
+// + 15 - u} are read once for all four outputs (the same elements, the same
+// products in the same order as autoconv_finish).
+template <typename T>
+__device__ __forceinline__ void autoconv_finish_lane(const T* __restrict__ sA,
+                                                     typename Acc4<T>::type& acc, int K, int kk) {
+  static_assert(Acc4<T>::kRowMul == 4 && Acc4<T>::kRowStride == 1, "fp32 accumulator layout");
+  const int dp = (K + 1) >> 1;
+  const int i0 = 16 * (dp - 1) + 4 * kk + 1;
+  const int xs = pad_addr(i0), r1 = (i0 + kPadSeg * kSeg) & 15;
+  const int ys = pad_addr(16 * (K - dp) + 15);
+  T xv[18], yv[15];
+#pragma unroll
+  for (int v = 0; v < 18; ++v) xv[v] = sA[xs + v + (((r1 + v) >> 4) << 2)];
+#pragma unroll
+  for (int u = 0; u < 15; ++u) yv[u] = sA[ys - u];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int k = 16 * K + 4 * kk + r;
+    const int cnt = ((k + 1) >> 1) - (i0 + r);
+    T s = acc[r];
+#pragma unroll
+    for (int u = 0; u < 15; ++u) s += u < cnt ? xv[r + u] * yv[u] : T(0);
+    T b = s + s;
+    if (!(k & 1)) {
+      const T x = sA[pad_addr(k >> 1)];
+      b += x * x;
+    }
+    acc[r] = b;
+  }
+}
+
 // Atomic max of a non-negative value into an LDS slot holding one (IEEE bits
 // of non-negative floats order like unsigned integers).
 __device__ __forceinline__ void atomic_max_nonneg(float* p, float v) {
@@ -648,11 +683,17 @@ __device__ void autoconv_mfma(const T* __restrict__ sA, T* out, int n, int* ctr,
       if (bmax) {  // whole tiles: this wave owns the outputs, so it finishes them
         const int kk = (opaque((int)threadIdx.x) & (kWave - 1)) >> 4;
         T m = T(0);
+        if constexpr (Acc4<T>::kRowStride == 1) {
+          autoconv_finish_lane<T>(sA, acc, K0 + t, kk);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int k = 16 * (K0 + t) + Acc4<T>::kRowMul * kk + Acc4<T>::kRowStride * r;
-          acc[r] = autoconv_finish(sA, acc[r], k);
-          m = acc[r] > m ? acc[r] : m;
+          for (int r = 0; r < 4; ++r) m = acc[r] > m ? acc[r] : m;
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int k = 16 * (K0 + t) + Acc4<T>::kRowMul * kk + Acc4<T>::kRowStride * r;
+            acc[r] = autoconv_finish(sA, acc[r], k);
+            m = acc[r] > m ? acc[r] : m;
+          }
         }
         m = wave_max(m);
         if ((threadIdx.x & (kWave - 1)) == 0) atomic_max_nonneg(bmax, m);
