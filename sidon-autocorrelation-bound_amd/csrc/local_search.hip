@@ -120,12 +120,15 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     T cut = T(0);
     if constexpr (sizeof(T) == 4)
       if (pm1 >= T(1)) cut = M * T(0.999) * dexp(T(-87.33654475055310898657) / pm1);
+    // fp32: d from one reciprocal of M a step (within an ulp of the quotient);
+    // fp64 keeps the quotient of the reference's fp64 polish
+    const T invM = T(1) / M;
     T rloc = T(0);
     for (int k = st; k < nb; k += BD) {
       const T b = sC[k];
       T c = T(0);
       if (b >= cut) {
-        const T d = (b - M) / M;
+        const T d = sizeof(T) == 4 ? (b - M) * invM : (b - M) / M;
         c = lp_weight_fast(d, pm1);
         rloc += c * (T(1) + d);
       }
