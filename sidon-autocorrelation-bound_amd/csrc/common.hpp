@@ -786,5 +786,9 @@ __device__ __forceinline__ float dexp(float x) { return expf(x); }
 __device__ __forceinline__ double dexp(double x) { return exp(x); }
 __device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
+// a / b for the search update: fp32 through the hardware reciprocal (about one
+// ulp, for normal b), fp64 the correctly rounded quotient
+__device__ __forceinline__ float dquot(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double dquot(double a, double b) { return a / b; }
 
 }  // namespace sidon

@@ -156,14 +156,14 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
       const int i = ut + BD * q;
       if (i < n) {
         const T ai = sA[pad_addr(i)];
-        const T gi = sG[i] / R - M;
+        const T gi = dquot(sG[i], R) - M;
         T xx;
         if (mode == 0) {
           const T mm = beta1 * m[q] + (T(1) - beta1) * gi;
           const T vv = beta2 * v[q] + (T(1) - beta2) * gi * gi;
           m[q] = mm;
           v[q] = vv;
-          const T step = (lr / nT) * (mm * bc1) / (dsqrt(vv * bc2) + T(1e-30));
+          const T step = dquot((lr / nT) * (mm * bc1), dsqrt(vv * bc2) + T(1e-30));
           xx = ai - step;
           if (xx < T(0)) xx = T(0);
         } else {
