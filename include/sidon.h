@@ -8,9 +8,10 @@
  *  - all array arguments are DEVICE pointers on the current HIP device,
  *    owned by the caller; population arrays are row-major [P x n];
  *  - work is enqueued on `stream` (0 = the null stream) and is asynchronous;
- *  - nothing here allocates device memory except sidon_init() (a small
- *    per-device work-queue workspace, created once) and the generation-graph
- *    constructor;
+ *  - nothing here allocates device memory; the generation-graph constructor
+ *    allocates only its graph objects.  Work-queue counters and workspaces
+ *    are caller-owned arguments, so launches on different streams with
+ *    distinct buffers never share state;
  *  - return value: SIDON_OK (0) or one of the sidon_status codes; the entry
  *    points never exit or throw.  sidon_last_error() describes the last error.
  *
@@ -36,8 +37,9 @@ typedef enum {
   SIDON_ERR_UNSUPPORTED = 4  /* configuration not built (e.g. block size) */
 } sidon_status;
 
-/* Initialise the per-device workspace of `device` (-1 = current device).
- * Called implicitly on first use; no reference counterpart. */
+/* Reads and caches the attributes of `device` (-1 = current device) that the
+ * launches need (LDS per workgroup, CU count).  Allocates nothing; called
+ * implicitly on first use.  No reference counterpart. */
 int sidon_init(int device);
 /* Human-readable text of the last error on this thread. */
 const char* sidon_last_error(void);
@@ -115,12 +117,16 @@ int sidon_polish_f64(double* pop, double* mom, double* vel, int n, int K, int it
 /* Uneven lengths in one persistent launch: candidate p occupies
  * pop[p*stride .. p*stride + ns[p]) (ns on the device, ns[p] <= nmax <= stride);
  * `order` (may be NULL) lists the processing order, longest first gives the
- * best balance.  Workgroups pull candidates from a device work queue.  No
- * reference counterpart (the reference runs one n per launch). */
+ * best balance.  Workgroups pull candidates from the work-queue counter
+ * `queue`: one caller-owned device int, set to zero on `stream` before the
+ * launch (no memset node).  Launches that may overlap need distinct queues.
+ * As many workgroups as fit the device at once (occupancy x CUs, at most P).
+ * No reference counterpart (the reference runs one n per launch). */
 int sidon_local_search_ragged(float* pop, int stride, const int* ns, int P, int nmax,
                               const int* order, const int* skip, const int* mode, int iters,
                               float pStart, float pEnd, float lrStart, float lrEnd,
-                              float beta1, float beta2, int modeDefault, hipStream_t stream);
+                              float beta1, float beta2, int modeDefault, int* queue,
+                              hipStream_t stream);
 /* Choose the workgroup size the fp32 local search uses at length n
  * (256, 512 or 1024; 0 restores the built-in rule).  Set by the autotuner. */
 int sidon_set_ls_block(int n, int block);
@@ -215,7 +221,9 @@ int sidon_order_by_length(const int* ns, int P, int* order, hipStream_t stream);
 /* Device buffers (caller-owned) and parameters of a generation of the island
  * search.  Populations are [P x stride] fp32 with per-slot lengths ns (<= nmax);
  * a generation reads population A (ns_a) and writes population B (ns_b), the
- * next one the other way round.  gen points to a device uint64 counter. */
+ * next one the other way round.  gen points to a device uint64 counter, queue
+ * to the device int the local search's work queue counts in (one per
+ * generation object that may run concurrently with another). */
 typedef struct {
   float* pop_a; float* pop_b;
   int* ns_a; int* ns_b;
@@ -223,6 +231,7 @@ typedef struct {
   int* p1; int* p2; int* op; float* strength;
   int* order;
   unsigned long long* gen;
+  int* queue;
   int P, stride, nmax, islands, elite, migrate_every, iters;
   float pStart, pEnd, lrStart, lrEnd, sigma;
   double activeTol;
@@ -262,7 +271,10 @@ int sidon_measure_fp32_ceilings(float* scratch, int scratch_len, double* valu_tf
  * lane-group of 64, i.e. 4 * 2048 FLOP) that the local-search kernels have
  * executed on the current device since the last call with reset != 0; the
  * count excludes correlation steps skipped because their weights are all
- * zero.  Synchronous.  For roofline accounting; no reference counterpart. */
+ * zero.  The counter is device-wide: launches on every stream add to it, and
+ * the read (with reset, an atomic exchange with zero on the device) loses no
+ * step that a concurrently running launch adds.  Synchronous.  For roofline
+ * accounting; no reference counterpart. */
 int sidon_ls_block_steps(unsigned long long* steps, int reset);
 /* The same count for sidon_frank_wolfe: Toeplitz steps of the dual updates,
  * of the M and line-search autoconvolutions, and the correlation steps run

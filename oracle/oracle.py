@@ -68,11 +68,17 @@ def lib():
     L.so_local_search.argtypes = [_dp, C.c_int, C.c_int] + [C.c_double] * 6 + \
         [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
     L.so_local_search.restype = C.c_double
+    L.so_local_search_gmin.argtypes = [_dp, C.c_int, C.c_int] + [C.c_double] * 6 + \
+        [C.c_int, C.c_void_p, C.c_void_p, _dp]
+    L.so_local_search_gmin.restype = C.c_double
     L.so_local_search_batch.argtypes = [_dp, C.c_int, C.c_int, C.c_int] + [C.c_double] * 6 + \
         [C.c_int, C.c_int, C.c_int, C.c_void_p]
     L.so_frank_wolfe.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                  C.POINTER(C.c_double)]
     L.so_frank_wolfe.restype = C.c_int
+    L.so_frank_wolfe_trace.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                       C.POINTER(C.c_double), _dp, _dp]
+    L.so_frank_wolfe_trace.restype = C.c_int
     L.so_pdhg.argtypes = [_dp, C.c_int, C.c_double, C.c_int, _dp, _dp]
     L.so_refine.argtypes = [_dp, C.c_int, _dp]
     L.so_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
@@ -221,6 +227,18 @@ def local_search(a, iters, pA, pB, lrA, lrB, b1=0.9, b2=0.999, mode=0, probe_sem
     return a, F, m, v, done.value
 
 
+def local_search_gmin(a, iters, pA, pB, lrA, lrB, b1=0.9, b2=0.999, mode=0):
+    """Kernel-semantics local search; returns (a_out, F, m, v, gmin) with
+    gmin[i] = min over the steps of |g_i| / M."""
+    a = _d(a).copy()
+    m = np.zeros_like(a)
+    v = np.zeros_like(a)
+    gmin = np.zeros_like(a)
+    F = lib().so_local_search_gmin(a, a.size, iters, pA, pB, lrA, lrB, b1, b2, mode,
+                                   m.ctypes.data, v.ctypes.data, gmin)
+    return a, F, m, v, gmin
+
+
 def local_search_batch(pop, iters, pA, pB, lrA, lrB, b1=0.9, b2=0.999, mode=0,
                        probe_semantics=True, nthreads=0):
     pop = _d(pop).copy()
@@ -236,6 +254,17 @@ def frank_wolfe(a, rounds, cp_iters, nT=12, tMax=0.30):
     M = C.c_double()
     acc = lib().so_frank_wolfe(a, a.size, rounds, cp_iters, nT, tMax, C.byref(M))
     return a, acc, M.value
+
+
+def frank_wolfe_trace(a, rounds, cp_iters, nT=12, tMax=0.30):
+    """(a_out, accepted, M, g, lsM): g the last round's normalised LP
+    direction, lsM the line-search maxima of t = tMax s/nT, s = 1..nT."""
+    a = _d(a).copy()
+    M = C.c_double()
+    g = np.zeros_like(a)
+    lsM = np.zeros(nT)
+    acc = lib().so_frank_wolfe_trace(a, a.size, rounds, cp_iters, nT, tMax, C.byref(M), g, lsM)
+    return a, acc, M.value, g, lsM
 
 
 def pdhg(a, M, cp_iters):

@@ -330,9 +330,10 @@ long double so_Jp_grad_ld(const long double* a, int n, long double p, long doubl
  * divide-vs-reciprocal renormalisation and the autoconvolution loop order
  * (probe.cpp:32-41 scatters i-major; both add terms for each k in ascending
  * i, so the sums are identical). */
-double so_local_search(double* a, int n, int iters, double pA, double pB, double lrA,
-                       double lrB, double b1, double b2, int mode, int probe_sem,
-                       double* m_out, double* v_out, int* iters_done) {
+static double local_search_impl(double* a, int n, int iters, double pA, double pB,
+                                double lrA, double lrB, double b1, double b2, int mode,
+                                int probe_sem, double* m_out, double* v_out, int* iters_done,
+                                double* gmin) {
   const int nb = 2 * n - 1;
   double* b = (double*)calloc((size_t)nb, sizeof(double));
   double* cc = (double*)calloc((size_t)nb, sizeof(double));
@@ -371,6 +372,10 @@ double so_local_search(double* a, int n, int iters, double pA, double pB, double
       const double* cp = cc + i;
       for (int j = 0; j < n; ++j) s += cp[j] * a[j];
       g[i] = s / R - M;
+      if (gmin) {
+        const double r = fabs(g[i]) / M;
+        if (it == 0 || r < gmin[i]) gmin[i] = r;
+      }
     }
     b1t *= b1;
     b2t *= b2;
@@ -407,6 +412,20 @@ double so_local_search(double* a, int n, int iters, double pA, double pB, double
   if (iters_done) *iters_done = done;
   free(b); free(cc); free(m); free(v); free(g);
   return so_F_hostorder(a, n);
+}
+
+double so_local_search(double* a, int n, int iters, double pA, double pB, double lrA,
+                       double lrB, double b1, double b2, int mode, int probe_sem,
+                       double* m_out, double* v_out, int* iters_done) {
+  return local_search_impl(a, n, iters, pA, pB, lrA, lrB, b1, b2, mode, probe_sem, m_out, v_out,
+                           iters_done, NULL);
+}
+
+double so_local_search_gmin(double* a, int n, int iters, double pA, double pB, double lrA,
+                            double lrB, double b1, double b2, int mode, double* m_out,
+                            double* v_out, double* gmin) {
+  return local_search_impl(a, n, iters, pA, pB, lrA, lrB, b1, b2, mode, 0, m_out, v_out, NULL,
+                           gmin);
 }
 
 void so_local_search_batch(double* pop, int P, int n, int iters, double pA, double pB,
@@ -462,7 +481,8 @@ void so_pdhg(const double* a, int n, double M, int cpIters, double* g, double* l
 }
 
 /* kernels.cuh:427-532. */
-int so_frank_wolfe(double* a, int n, int rounds, int cpIters, int nT, double tMax, double* M_out) {
+static int frank_wolfe_impl(double* a, int n, int rounds, int cpIters, int nT, double tMax,
+                            double* M_out, double* g_out, double* lsM_out) {
   const int nb = 2 * n - 1;
   double* b = (double*)malloc(sizeof(double) * (size_t)nb);
   double* g = (double*)malloc(sizeof(double) * (size_t)n);
@@ -495,8 +515,10 @@ int so_frank_wolfe(double* a, int n, int rounds, int cpIters, int nT, double tMa
       so_autoconv(h, n, b);
       double mm = 0.0;
       for (int k = 0; k < nb; ++k) mm = b[k] > mm ? b[k] : mm;
+      if (lsM_out) lsM_out[st - 1] = mm;
       if (mm < bestM) { bestM = mm; bestT = t; }
     }
+    if (g_out) memcpy(g_out, g, sizeof(double) * (size_t)n);
     if (!(bestT > 0.0)) break;
     for (int i = 0; i < n; ++i) a[i] = (1.0 - bestT) * a[i] + bestT * g[i];
     Mlast = bestM;
@@ -506,6 +528,15 @@ out:
   if (M_out) *M_out = Mlast;
   free(b); free(g); free(h);
   return accepted;
+}
+
+int so_frank_wolfe(double* a, int n, int rounds, int cpIters, int nT, double tMax, double* M_out) {
+  return frank_wolfe_impl(a, n, rounds, cpIters, nT, tMax, M_out, NULL, NULL);
+}
+
+int so_frank_wolfe_trace(double* a, int n, int rounds, int cpIters, int nT, double tMax,
+                         double* M_out, double* g_out, double* lsM_out) {
+  return frank_wolfe_impl(a, n, rounds, cpIters, nT, tMax, M_out, g_out, lsM_out);
 }
 
 /* ------------------------------------------------------------ refine ---- */

@@ -83,6 +83,15 @@ double so_local_search(double* a, int n, int iters, double pA, double pB,
                        double lrA, double lrB, double b1, double b2, int mode,
                        int probe_semantics, double* m_out, double* v_out, int* iters_done);
 
+/* The device-kernel semantics of so_local_search (probe_semantics = 0) that
+ * also records, per height i, min over the steps of |g_i| / M, g_i = G_i/R - M
+ * the direction (gmin, length n).  A height whose direction came near zero
+ * may take either Adam sign under another summation order; parity tests
+ * exclude those by this record. */
+double so_local_search_gmin(double* a, int n, int iters, double pA, double pB, double lrA,
+                            double lrB, double b1, double b2, int mode, double* m_out,
+                            double* v_out, double* gmin);
+
 /* Batched form of the above over P candidates (row-major P x n), OpenMP over
  * candidates with nthreads threads (0 = OpenMP default).  Used as the timed
  * CPU baseline.  F_out (length P) may be NULL. */
@@ -96,6 +105,14 @@ void so_local_search_batch(double* pop, int P, int n, int iters, double pA, doub
  * final max_k (a*a)_k with sum a = 1. */
 int so_frank_wolfe(double* a, int n, int rounds, int cpIters, int nT, double tMax,
                    double* M_out);
+
+/* so_frank_wolfe that also reports, for the last round that reached its line
+ * search, the normalised LP direction g (g_out, length n) and max_k (h*h)_k of
+ * each trial h = (1-t) a + t g, t = tMax s/nT, s = 1..nT (lsM_out, length nT).
+ * Either may be NULL.  With them a test can tell a near-tie of the line
+ * search (two t within rounding of each other) from a wrong step. */
+int so_frank_wolfe_trace(double* a, int n, int rounds, int cpIters, int nT, double tMax,
+                         double* M_out, double* g_out, double* lsM_out);
 
 /* One PDHG solve for the LP of a fixed a (sum a = 1, M = max a*a):
  * returns g (not normalised) after cpIters iterations (kernels.cuh:451-503). */

@@ -49,7 +49,7 @@ class GenerationDesc(C.Structure):
     """sidon_generation_desc (include/sidon.h)."""
     _fields_ = [(name, C.c_void_p) for name in (
         "pop_a", "pop_b", "ns_a", "ns_b", "fit", "nact", "p1", "p2", "op", "strength", "order",
-        "gen")] + [(name, C.c_int) for name in (
+        "gen", "queue")] + [(name, C.c_int) for name in (
             "P", "stride", "nmax", "islands", "elite", "migrate_every", "iters")] + [
         (name, C.c_float) for name in ("pStart", "pEnd", "lrStart", "lrEnd", "sigma")] + [
         ("activeTol", C.c_double), ("seed", C.c_uint64)]
@@ -94,7 +94,7 @@ def _declare(L):
         "sidon_local_search_f64": ([_vp, _vp, _vp, _vp, _vp, _i, _i, _i] + [_d] * 6 + [_i, _vp],
                                    _i),
         "sidon_local_search_ragged": ([_vp, _i, _vp, _i, _i, _vp, _vp, _vp, _i] + [_f] * 6 +
-                                      [_i, _vp], _i),
+                                      [_i, _vp, _vp], _i),
         "sidon_set_ls_block": ([_i, _i], _i),
         "sidon_get_ls_block": ([_i], _i),
         "sidon_lp_grad": ([_vp, _i, _i, _f, _vp, _vp, _vp], _i),
@@ -260,12 +260,19 @@ def local_search_ragged(pop: torch.Tensor, ns: torch.Tensor, nmax: int, iters: i
                         p_start: float, p_end: float, lr_start: float, lr_end: float,
                         beta1: float = 0.9, beta2: float = 0.999, mode: int = 0,
                         order: torch.Tensor | None = None, skip: torch.Tensor | None = None,
-                        mode_arr: torch.Tensor | None = None, stream=None) -> None:
+                        mode_arr: torch.Tensor | None = None, queue: torch.Tensor | None = None,
+                        stream=None) -> None:
+    """Persistent uneven-length local search.  `queue` is the int32 device
+    counter the workgroups pull candidates from (a fresh one when not given;
+    launches that may overlap need distinct counters)."""
     _need_gpu()
     P, stride = pop.shape
+    if queue is None:
+        queue = torch.zeros(1, dtype=torch.int32, device=pop.device)
     _check(lib().sidon_local_search_ragged(_ptr(pop), stride, _ptr(ns), P, nmax, _ptr(order),
                                            _ptr(skip), _ptr(mode_arr), iters, p_start, p_end,
-                                           lr_start, lr_end, beta1, beta2, mode, _stream(stream)),
+                                           lr_start, lr_end, beta1, beta2, mode, _ptr(queue),
+                                           _stream(stream)),
            "sidon_local_search_ragged")
 
 
@@ -523,10 +530,11 @@ class Generation:
         self.strength = torch.zeros(P, dtype=torch.float32, device=dev)
         self.order = torch.zeros(P, **i32)
         self.gen = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.queue = torch.zeros(1, **i32)
         nmax = int(self.ns_a.max().item())
         d = GenerationDesc()
         for name in ("pop_a", "pop_b", "ns_a", "ns_b", "fit", "nact", "p1", "p2", "op",
-                     "strength", "order", "gen"):
+                     "strength", "order", "gen", "queue"):
             setattr(d, name, _ptr(getattr(self, name)))
         d.P, d.stride, d.nmax, d.islands, d.elite = P, stride, nmax, islands, elite
         d.migrate_every, d.iters = migrate_every, iters

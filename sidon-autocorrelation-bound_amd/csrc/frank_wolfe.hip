@@ -21,6 +21,8 @@
 // correlation skips the Hankel steps that would read only zeros; two flag
 // arrays alternate between iterations, so one is cleared while the other is
 // in use.
+#include <mutex>
+
 #include "common.hpp"
 #include "runtime.hpp"
 
@@ -34,6 +36,13 @@ constexpr int kMaxConvTiles = 80;   // n <= 10240
 // updates, the M and line-search autoconvolutions, and the correlation steps
 // actually executed (steps over all-zero multipliers are skipped).
 __device__ unsigned long long g_fw_block_steps = 0ull;
+__device__ unsigned long long g_fw_block_steps_taken = 0ull;
+
+// Reads the counter (and with `reset` zeroes it) in one device atomic.
+__global__ void k_take_fw_block_steps(int reset) {
+  g_fw_block_steps_taken =
+      reset ? atomicExch(&g_fw_block_steps, 0ull) : atomicAdd(&g_fw_block_steps, 0ull);
+}
 
 template <int BD>
 __host__ __device__ inline long long fw_lds_bytes(int n) {
@@ -275,11 +284,11 @@ extern "C" int sidon_fw_block_steps(unsigned long long* out, int reset) {
   if (!out) return SIDON_ERR_ARG;
   int rc = SIDON_OK;
   if (!device_state(&rc)) return rc;
-  SIDON_HIP_TRY(hipMemcpyFromSymbol(out, HIP_SYMBOL(g_fw_block_steps), sizeof(*out)));
-  if (reset) {
-    const unsigned long long zero = 0ull;
-    SIDON_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_fw_block_steps), &zero, sizeof(zero)));
-  }
+  static std::mutex mu;  // one reader of g_fw_block_steps_taken at a time
+  std::lock_guard<std::mutex> lk(mu);
+  hipLaunchKernelGGL(k_take_fw_block_steps, dim3(1), dim3(1), 0, 0, reset);
+  SIDON_HIP_TRY(hipGetLastError());
+  SIDON_HIP_TRY(hipMemcpyFromSymbol(out, HIP_SYMBOL(g_fw_block_steps_taken), sizeof(*out)));
   return SIDON_OK;
 }
 

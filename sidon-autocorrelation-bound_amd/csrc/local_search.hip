@@ -11,6 +11,7 @@
 //   barrier -> Adam/mirror update -> S -> a = x/S, clear buffers -> barrier.
 #include <cmath>
 #include <cstdio>
+#include <mutex>
 
 #include "common.hpp"
 #include "runtime.hpp"
@@ -22,6 +23,13 @@ namespace sidon {
 // kernels actually executed, for roofline accounting (correlation steps that
 // would read only zero weights are skipped and not counted).
 __device__ unsigned long long g_block_steps = 0ull;
+__device__ unsigned long long g_block_steps_taken = 0ull;
+
+// Reads the counter (and with `reset` zeroes it) in one device atomic, so no
+// step a running launch adds in between is lost.
+__global__ void k_take_block_steps(int reset) {
+  g_block_steps_taken = reset ? atomicExch(&g_block_steps, 0ull) : atomicAdd(&g_block_steps, 0ull);
+}
 
 template <typename T>
 struct LsArgs {
@@ -328,7 +336,7 @@ int pick_block(int nmax) {
 }
 
 template <typename T, int BD, int QMAX, bool PAIR = false>
-int launch_ls(const LsArgs<T>& A, int nmax, int grid, hipStream_t stream) {
+int launch_ls(const LsArgs<T>& A, int nmax, hipStream_t stream) {
   if (nmax > BD * QMAX) return SIDON_ERR_UNSUPPORTED;
   int rc = SIDON_OK;
   DeviceState* ds = device_state(&rc);
@@ -344,33 +352,39 @@ int launch_ls(const LsArgs<T>& A, int nmax, int grid, hipStream_t stream) {
   }();
   SIDON_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kfn),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int grid = A.P;
+  if (A.queue) {
+    // persistent: as many workgroups as are resident at once (two a CU for
+    // the register-capped 512-thread kernel where LDS allows), at most P
+    int per_cu = 0;
+    SIDON_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(kfn), BD, (size_t)lds));
+    if (per_cu < 1) per_cu = 1;
+    const long long resident = (long long)per_cu * ds->cu_count;
+    grid = resident < A.P ? (int)resident : A.P;
+  }
   hipLaunchKernelGGL(kfn, dim3(grid), dim3(BD), (size_t)lds, stream, A);
   SIDON_HIP_TRY(hipGetLastError());
   return SIDON_OK;
 }
 
+// `queue` non-null: persistent launch pulling candidates from that counter.
 template <typename T>
-int dispatch_ls(LsArgs<T>& A, int nmax, int block, bool persistent, hipStream_t stream) {
-  int rc = SIDON_OK;
-  DeviceState* ds = device_state(&rc);
-  if (!ds) return rc;
-  int grid = A.P;
-  if (persistent) {
-    rc = reset_queue(ds, 0, stream);
-    if (rc) return rc;
-    A.queue = ds->queue;
-    grid = ds->cu_count < A.P ? ds->cu_count : A.P;  // LDS allows one block per CU at large n
+int dispatch_ls(LsArgs<T>& A, int nmax, int block, int* queue, hipStream_t stream) {
+  if (queue) {
+    if (int rc = reset_counter(queue, stream)) return rc;
   }
+  A.queue = queue;
   switch (block) {
-    case 256: return launch_ls<T, 256, 16>(A, nmax, grid, stream);
+    case 256: return launch_ls<T, 256, 16>(A, nmax, stream);
     case 512:
       if constexpr (sizeof(T) == 4) {
-        if (nmax <= 512 * 8) return launch_ls<T, 512, 8, true>(A, nmax, grid, stream);
-        return launch_ls<T, 512, 20>(A, nmax, grid, stream);
+        if (nmax <= 512 * 8) return launch_ls<T, 512, 8, true>(A, nmax, stream);
+        return launch_ls<T, 512, 20>(A, nmax, stream);
       }
-      return launch_ls<T, 512, 12>(A, nmax, grid, stream);
+      return launch_ls<T, 512, 12>(A, nmax, stream);
     case 1024:
-      if constexpr (sizeof(T) == 4) return launch_ls<T, 1024, 10>(A, nmax, grid, stream);
+      if constexpr (sizeof(T) == 4) return launch_ls<T, 1024, 10>(A, nmax, stream);
       return SIDON_ERR_UNSUPPORTED;
     default: return SIDON_ERR_UNSUPPORTED;
   }
@@ -432,7 +446,7 @@ extern "C" int sidon_local_search(float* pop, float* mom, float* vel, const int*
   if (P == 0 || iters == 0) return SIDON_OK;
   LsArgs<float> A{pop, n, nullptr, n, P, mom, vel, skip, mode, modeDefault, nullptr, nullptr,
                   iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2};
-  return dispatch_ls<float>(A, n, pick_block(n), false, stream);
+  return dispatch_ls<float>(A, n, pick_block(n), nullptr, stream);
 }
 
 extern "C" int sidon_local_search_f64(double* pop, double* mom, double* vel, const int* skip,
@@ -445,20 +459,20 @@ extern "C" int sidon_local_search_f64(double* pop, double* mom, double* vel, con
   LsArgs<double> A{pop, n, nullptr, n, P, mom, vel, skip, mode, modeDefault, nullptr, nullptr,
                    iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2};
   const int block = n >= 1024 ? 512 : 256;
-  return dispatch_ls<double>(A, n, block, false, stream);
+  return dispatch_ls<double>(A, n, block, nullptr, stream);
 }
 
 extern "C" int sidon_local_search_ragged(float* pop, int stride, const int* ns, int P, int nmax,
                                          const int* order, const int* skip, const int* mode,
                                          int iters, float pStart, float pEnd, float lrStart,
                                          float lrEnd, float beta1, float beta2, int modeDefault,
-                                         hipStream_t stream) {
-  if (!pop || !ns || nmax < 1 || stride < nmax || P < 0) return SIDON_ERR_ARG;
+                                         int* queue, hipStream_t stream) {
+  if (!pop || !ns || !queue || nmax < 1 || stride < nmax || P < 0) return SIDON_ERR_ARG;
   if (int rc = check_schedule<float>(iters, pStart, pEnd, lrStart, lrEnd)) return rc;
   if (P == 0 || iters == 0) return SIDON_OK;
   LsArgs<float> A{pop, stride, ns, nmax, P, nullptr, nullptr, skip, mode, modeDefault, order,
                   nullptr, iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2};
-  return dispatch_ls<float>(A, nmax, pick_block(nmax), true, stream);
+  return dispatch_ls<float>(A, nmax, pick_block(nmax), queue, stream);
 }
 
 extern "C" int sidon_lp_grad(const float* a, int n, int B, float p, float* J, float* grad,
@@ -491,10 +505,10 @@ extern "C" int sidon_ls_block_steps(unsigned long long* out, int reset) {
   if (!out) return SIDON_ERR_ARG;
   int rc = SIDON_OK;
   if (!device_state(&rc)) return rc;
-  SIDON_HIP_TRY(hipMemcpyFromSymbol(out, HIP_SYMBOL(g_block_steps), sizeof(*out)));
-  if (reset) {
-    const unsigned long long zero = 0ull;
-    SIDON_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_block_steps), &zero, sizeof(zero)));
-  }
+  static std::mutex mu;  // one reader of g_block_steps_taken at a time
+  std::lock_guard<std::mutex> lk(mu);
+  hipLaunchKernelGGL(k_take_block_steps, dim3(1), dim3(1), 0, 0, reset);
+  SIDON_HIP_TRY(hipGetLastError());
+  SIDON_HIP_TRY(hipMemcpyFromSymbol(out, HIP_SYMBOL(g_block_steps_taken), sizeof(*out)));
   return SIDON_OK;
 }

@@ -524,19 +524,22 @@ extern "C" size_t sidon_polish_f64_workspace(int n, int K) {
   return ws_bytes(n, K);
 }
 
-extern "C" int sidon_polish_f64(double* pop, double* mom, double* vel, int n, int K, int iters,
-                                double pStart, double pEnd, double lrStart, double lrEnd,
-                                double beta1, double beta2, int mode, void* workspace,
-                                size_t workspace_bytes, hipStream_t stream) {
-  if (!pop || n < 1 || K < 0 || iters < 0 || !workspace) return SIDON_ERR_ARG;
+namespace {
+
+int check_polish_args(const double* pop, int n, int K, int iters, double pStart, double pEnd,
+                      double lrStart, double lrEnd, int mode) {
+  if (!pop || n < 1 || K < 0 || iters < 0) return SIDON_ERR_ARG;
   if (!(pStart >= 1.0) || !(pEnd >= 1.0) || !(lrStart > 0.0) || !(lrEnd > 0.0) ||
       (mode != 0 && mode != 1))
     return SIDON_ERR_ARG;
-  if (K == 0 || iters == 0) return SIDON_OK;
-  if (workspace_bytes < ws_bytes(n, K)) {
-    set_last_error_msg("polish: workspace smaller than sidon_polish_f64_workspace(n, K)");
-    return SIDON_ERR_ARG;
-  }
+  return SIDON_OK;
+}
+
+// The polish's kernel sequence on a workspace of at least ws_bytes(n, K)
+// bytes (arguments already checked, K and iters positive).
+int polish_run(double* pop, double* mom, double* vel, int n, int K, int iters, double pStart,
+               double pEnd, double lrStart, double lrEnd, double beta1, double beta2, int mode,
+               void* workspace, hipStream_t stream) {
   const Shape s = shape_of(n);
   const Ws w = ws_of(workspace, n, K);
   Sched S;
@@ -572,4 +575,21 @@ extern "C" int sidon_polish_f64(double* pop, double* mom, double* vel, int n, in
     SIDON_HIP_TRY(hipGetLastError());
   }
   return SIDON_OK;
+}
+
+}  // namespace
+
+extern "C" int sidon_polish_f64(double* pop, double* mom, double* vel, int n, int K, int iters,
+                                double pStart, double pEnd, double lrStart, double lrEnd,
+                                double beta1, double beta2, int mode, void* workspace,
+                                size_t workspace_bytes, hipStream_t stream) {
+  if (int rc = check_polish_args(pop, n, K, iters, pStart, pEnd, lrStart, lrEnd, mode)) return rc;
+  if (!workspace) return SIDON_ERR_ARG;
+  if (K == 0 || iters == 0) return SIDON_OK;
+  if (workspace_bytes < ws_bytes(n, K)) {
+    set_last_error_msg("polish: workspace smaller than sidon_polish_f64_workspace(n, K)");
+    return SIDON_ERR_ARG;
+  }
+  return polish_run(pop, mom, vel, n, K, iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2, mode,
+                    workspace, stream);
 }

@@ -1,4 +1,4 @@
-// runtime.hip -- per-device workspace, error reporting and queries of the C ABI.
+// runtime.hip -- per-device attributes, error reporting and queries of the C ABI.
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -27,7 +27,7 @@ void set_last_error(const char* where, hipError_t e) {
 }
 void set_last_error_msg(const char* msg) { std::snprintf(g_err, sizeof g_err, "%s", msg); }
 
-// Fills g_states[dev] with `dev` current; on failure frees what it allocated.
+// Fills g_states[dev] with `dev` current.
 static int init_current(int dev) {
   DeviceState& s = g_states[dev];
   int lds_block = 0, lds_optin = 0, cus = 0;
@@ -36,18 +36,8 @@ static int init_current(int dev) {
     lds_optin = 0;
   (void)hipGetLastError();
   SIDON_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  int* q = nullptr;
-  SIDON_HIP_TRY(hipMalloc(&q, sizeof(int) * kQueueSlots));
-  hipError_t e = hipMemset(q, 0, sizeof(int) * kQueueSlots);
-  if (e == hipSuccess) e = hipDeviceSynchronize();
-  if (e != hipSuccess) {
-    set_last_error("hipMemset(work-queue counters)", e);
-    (void)hipFree(q);
-    return SIDON_ERR_HIP;
-  }
   s.lds_max = lds_optin > lds_block ? lds_optin : lds_block;
   s.cu_count = cus;
-  s.queue = q;
   s.device.store(dev, std::memory_order_release);
   return SIDON_OK;
 }
@@ -88,8 +78,8 @@ DeviceState* device_state(int* rc) {
   return &g_states[dev];
 }
 
-int reset_queue(DeviceState* ds, int slot, hipStream_t stream) {
-  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, stream, ds->queue + slot);
+int reset_counter(int* counter, hipStream_t stream) {
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, stream, counter);
   SIDON_HIP_TRY(hipGetLastError());
   return SIDON_OK;
 }

@@ -10,17 +10,15 @@
 
 namespace sidon {
 
-// Per-device persistent workspace (allocated once by sidon_init or on first
-// use on a device): work-queue counters for persistent launches.  `device`
-// is stored last, with release order, once the other fields are complete;
-// readers that see it equal to their device may use the rest.
+// Per-device attributes the launches need (read once by sidon_init or on
+// first use on a device; nothing is allocated).  `device` is stored last,
+// with release order, once the other fields are complete; readers that see
+// it equal to their device may use the rest.
 struct DeviceState {
   std::atomic<int> device{-1};
   int lds_max = 0;        // max dynamic LDS bytes per workgroup
   int cu_count = 0;
-  int* queue = nullptr;   // kQueueSlots work-queue counters
 };
-constexpr int kQueueSlots = 64;
 
 // Returns the state of the current device (initialising it on first use), or
 // nullptr with *rc set on failure.
@@ -41,8 +39,9 @@ void set_last_error_msg(const char* msg);
     }                                                          \
   } while (0)
 
-// Resets work-queue counter `slot` to zero with a one-thread kernel (no
-// hipMemsetAsync, so the sequence is safe inside captured graphs).
-int reset_queue(DeviceState* ds, int slot, hipStream_t stream);
+// Sets the caller-owned device counter *counter to zero on `stream` with a
+// one-thread kernel (no hipMemsetAsync, so the sequence is safe inside
+// captured graphs).
+int reset_counter(int* counter, hipStream_t stream);
 
 }  // namespace sidon

@@ -48,6 +48,13 @@ def autoconv64(a: np.ndarray) -> np.ndarray:
     return np.convolve(a, a)
 
 
+def _b_at(a: np.ndarray, k: int) -> float:
+    """b_k = sum_i a_i a_{k-i} alone (O(n))."""
+    n = a.size
+    lo, hi = max(0, k - n + 1), min(k, n - 1)
+    return float(np.dot(a[lo:hi + 1], a[k - hi:k - lo + 1][::-1]))
+
+
 def plant_peaks(a: np.ndarray, n_peaks: int = 5, spread: float = 1e-7, cells: int = 48,
                 seed: int = SEED, round_fp32: bool = True, tol: float = 1e-6):
     """Adjust a (fp64 copy) so that at least n_peaks values b_k lie within tol of
@@ -74,8 +81,7 @@ def plant_peaks(a: np.ndarray, n_peaks: int = 5, spread: float = 1e-7, cells: in
                 free.update(rng.choice(ok, size=min(per, ok.size), replace=False).tolist())
         free = np.array(sorted(free))
         for _ in range(12):
-            b = autoconv64(a)
-            r = target - b[ks]
+            r = target - np.array([_b_at(a, k) for k in ks])
             if np.max(np.abs(r) / L) < 1e-14:
                 break
             J = np.zeros((n_peaks, free.size))

@@ -3,6 +3,8 @@ torch.autograd.Function harness: vs the fp64 / long-double oracle gradient and
 central finite differences, including five planted near-tie peaks."""
 from __future__ import annotations
 
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import pytest
 import torch
@@ -18,35 +20,79 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("p", [8.0, 64.0, 1024.0])
-def test_adjoint_planted_ties_n8400(S, O, W, p):
-    """Adjoint check at n = 8400 (batch 64 in the configuration; 4 rows here,
-    the rest run in the benchmark).  fp32 b carries ~1e-6 relative error, so
-    the weights (b/M)^(p-1) near the planted ties move by ~(p-1) 1e-6: the
-    tolerance grows with p."""
-    print(f"[grad] n=8400 p={p}", flush=True)
-    base = W.structured_prior(8, 8400, seed=77)
+@pytest.fixture(scope="module")
+def c4_rows(W):
+    """Config 4's batch: 64 structured-prior rows at n = 8400, each with five
+    b_k planted within 1e-6 (relative, fp64, after rounding to fp32) of the
+    max; rows where planting reached fewer than five ties are passed over."""
+    base = W.structured_prior(96, 8400, seed=77)
     rows = []
-    for r in range(8):  # the first four rows where planting reached five ties
+    for r in range(base.shape[0]):
         a, near = W.plant_peaks(base[r], seed=r)
         if near.size >= 5:
             rows.append(a)
-        if len(rows) == 4:
+        if len(rows) == 64:
             break
-    assert len(rows) == 4
-    A = np.stack(rows)
+    assert len(rows) == 64
+    return np.stack(rows)
+
+
+# per-height bounds on |g - g_ref| / max|g_ref|, about 3x the worst error
+# measured over the 64 rows on MI355X (2.1e-6, 4.1e-5, 6.4e-4): fp32 b carries
+# ~1e-7 relative error, which moves (b/M)^(p-1) near the planted ties by
+# ~(p-1) 1e-7
+C4_BOUND = {8.0: 7e-6, 64.0: 1.2e-4, 1024.0: 2e-3}
+
+
+@pytest.mark.parametrize("p", [8.0, 64.0, 1024.0])
+def test_adjoint_planted_ties_n8400(S, O, c4_rows, p):
+    """Adjoint check at n = 8400, batch 64, through the autograd harness: J_p to
+    1e-6 and every gradient component against the fp64 oracle, per height."""
+    print(f"[grad] n=8400 batch 64 p={p}", flush=True)
+    A = c4_rows
     t = dev(A).requires_grad_(True)
     J = S.lp_objective(t, p)
     J.sum().backward()
     g = t.grad.cpu().numpy().astype(np.float64)
     Jg = J.detach().cpu().numpy()
-    tol = {8.0: 1e-4, 64.0: 1e-3, 1024.0: 2e-2}[p]
-    for r in range(4):
-        Jr, gr = O.Jp_grad(A[r], p)
-        assert abs(Jg[r] - Jr) < 1e-5 * Jr
-        e = rel(g[r], gr)
-        print(f"   row {r}: rel grad err {e:.3e}")
-        assert e < tol
+    with ThreadPoolExecutor(8) as ex:
+        ref = list(ex.map(lambda a: O.Jp_grad(a, p), list(A)))
+    worst = 0.0
+    for r, (Jr, gr) in enumerate(ref):
+        assert abs(Jg[r] - Jr) < 1e-6 * Jr, (r, Jg[r], Jr)
+        e = float(np.abs(g[r] - gr).max() / np.abs(gr).max())
+        worst = max(worst, e)
+        assert e < C4_BOUND[p], (r, e)
+    print(f"   worst per-height error over 64 rows: {worst:.3e} of max|g|")
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 15, 16, 17, 31, 33, 63, 65, 255, 257, 1000, 2047,
+                               4095])
+def test_lp_grad_fp32_ragged_lengths(S, O, n):
+    """fp32 J_p and gradient at p = 8 at every tile edge the kernel has (the
+    symmetric autoconvolution's zone and finish, Hankel tile tails), per
+    height against the fp64 oracle."""
+    print(f"[grad] fp32 ragged n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    A = rng.uniform(0.1, 1.0, size=(3, n))
+    A[1][rng.uniform(size=n) < 0.33] = 0.0
+    A[1][0] = 1.0
+    A[2] = rng.lognormal(0.0, 1.0, size=n)
+    A = A.astype(np.float32).astype(np.float64)
+    J, g = S.lp_grad(dev(A), 8.0)
+    J = J.cpu().numpy().astype(np.float64)
+    g = g.cpu().numpy().astype(np.float64)
+    for r in range(3):
+        Jr, gr = O.Jp_grad(A[r], 8.0)
+        assert abs(J[r] - Jr) < 2e-6 * Jr, (r, J[r], Jr)
+        # the natural size of a component: scale * M / S (the two terms of
+        # G_i/R - M/S are both of that order)
+        S_ = A[r].sum()
+        M = np.convolve(A[r], A[r]).max()
+        nat = 4.0 * n * Jr / (2.0 * n * M) * M / S_   # = 4 n R^{1/p} M / S^3
+        e = float(np.abs(g[r] - gr).max() / nat)
+        print(f"   row {r}: per-height err {e:.2e}")
+        assert e < 6e-6, (r, e)            # measured up to 1.8e-6 (n = 4095)
 
 
 def test_grad_fp64_kernel_and_fd(S, O, W):

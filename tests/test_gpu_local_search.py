@@ -42,10 +42,32 @@ def test_plain_config_one_step_fp64(S, O, W):
     assert checked > 0.95 * 64 * 512
 
 
+# fp32 vs fp64 after 10 steps of the anneal, per height: the worst error over
+# the heights K whose direction g_i stayed at least VANISH * M away from zero
+# at every step (by the fp64 oracle), relative to the largest height.  A
+# height whose direction came closer may take the other Adam sign in fp32 (a
+# step of +-lr/n); through the renormalisation such a flip rescales every
+# other height by the same factor, so both vectors are compared after
+# rescaling each to sum 1 over K.
+VANISH = 1e-3
+
+
+def ls_errors(got, want, gmin):
+    keep = gmin >= VANISH
+    if not keep.any():
+        return 0.0, keep
+    g = got / got[keep].sum()
+    w = want / want[keep].sum()
+    err = np.abs(g - w)[keep] / np.abs(w).max()
+    return float(err.max()), keep
+
+
 @pytest.mark.parametrize("n,block", [(128, None), (600, None), (2100, None), (4096, None),
                                      (8400, None), (1200, 1024), (2100, 256)])
 def test_fp32_short_trajectory_vs_fp64(S, O, W, n, block):
-    """10 fp32 steps track the fp64 oracle (same start, kernel semantics)."""
+    """10 fp32 steps track the fp64 oracle (same start, kernel semantics), per
+    height, excluding only heights whose direction came within VANISH * M of
+    zero."""
     print(f"[ls] fp32 trajectory n={n} block={block}", flush=True)
     a = W.structured_prior(2, n, seed=n).astype(np.float64)
     t = dev(a)
@@ -58,16 +80,18 @@ def test_fp32_short_trajectory_vs_fp64(S, O, W, n, block):
             S.set_ls_block(n, 0)
     got = t.cpu().numpy().astype(np.float64)
     for p in range(2):
-        want, *_ = O.local_search(a[p].astype(np.float32).astype(np.float64), 10, 4.0, 40.0,
-                                  0.05, 0.01)
-        # a coordinate whose direction is ~0 can take the opposite Adam sign in
-        # fp32; all but a handful of coordinates must agree closely
-        err = np.abs(got[p] - want) / np.abs(want).max()
-        assert np.quantile(err, 0.999) < 2e-3, err.max()
+        want, Fw, _, _, gmin = O.local_search_gmin(a[p].astype(np.float32).astype(np.float64),
+                                                   10, 4.0, 40.0, 0.05, 0.01)
+        worst, keep = ls_errors(got[p], want, gmin)
+        print(f"   row {p}: max rel err {worst:.3e} over {keep.sum()}/{n} heights")
+        assert worst < LS10_BOUND, worst
         Fg = O.F_hostorder(got[p])
-        Fw = O.F_hostorder(want)
-        assert abs(Fg - Fw) < 1e-4 * Fw
+        assert abs(Fg - Fw) < 2e-6 * Fw, (Fg, Fw)
         assert (got[p] >= 0).all() and abs(got[p].sum() - 1) < 1e-5
+
+
+# about 4x the worst error measured on MI355X (5e-7)
+LS10_BOUND = 2e-6
 
 
 def test_selftest_improvement(S, O):
@@ -219,8 +243,159 @@ def test_high_p_skips_zero_weights_and_tracks_oracle(S, O, W):
     assert steps < 0.9 * 2 * 4 * _dense_block_steps(n)
     got = t.cpu().numpy().astype(np.float64)
     for p in range(2):
-        want, *_ = O.local_search(a[p].astype(np.float32).astype(np.float64), 4, 1e4, 1e4,
-                                  0.02, 0.02)
-        err = np.abs(got[p] - want) / np.abs(want).max()
-        assert np.quantile(err, 0.999) < 2e-3, err.max()
-        assert abs(O.F_hostorder(got[p]) - O.F_hostorder(want)) < 1e-4 * O.F_hostorder(want)
+        want, Fw, _, _, gmin = O.local_search_gmin(a[p].astype(np.float32).astype(np.float64), 4,
+                                                   1e4, 1e4, 0.02, 0.02)
+        worst, keep = ls_errors(got[p], want, gmin)
+        print(f"   row {p}: max rel err {worst:.3e} over {keep.sum()}/{n} heights")
+        assert worst < LS10_BOUND, worst
+        assert abs(O.F_hostorder(got[p]) - Fw) < 2e-6 * Fw
+
+
+# One fp32 step from zero moments at every residue of n mod 16 the tile code
+# distinguishes, and at the smallest lengths (the symmetric autoconvolution's
+# zone and finish edges, the Hankel tiles' tails): the first Adam step is
+# (lr/n) sign(g) exactly, so every height whose direction is clear of zero must
+# match the fp64 oracle to fp32 rounding, and the returned moment m = 0.1 g
+# must match per height to fp32 accumulation error.
+RAGGED_NS = [1, 2, 3, 7, 15, 16, 17, 31, 33, 63, 65, 255, 257, 1000, 2047, 4095]
+
+
+def ragged_vectors(n, rows=3, seed=0):
+    rng = np.random.default_rng(seed + n)
+    a = rng.uniform(0.1, 1.0, size=(rows, n))
+    a[1][rng.uniform(size=n) < 0.33] = 0.0
+    a[1][0] = 1.0                                   # never all zero
+    if rows > 2 and n >= 16:                        # (shorter priors can be all zero)
+        a[2] = W_structured(n, seed + n)
+    return a / a.sum(axis=1, keepdims=True)
+
+
+def W_structured(n, seed):
+    import importlib
+    W = importlib.import_module("sidon-autocorrelation-bound_amd.workloads")
+    return W.structured_prior(1, n, seed=seed)[0].astype(np.float64)
+
+
+@pytest.mark.parametrize("n", RAGGED_NS)
+def test_fp32_one_step_ragged_lengths(S, O, n):
+    print(f"[ls] fp32 one step n={n}", flush=True)
+    a = ragged_vectors(n).astype(np.float32).astype(np.float64)
+    t = dev(a)
+    mom = torch.zeros_like(t)
+    S.local_search(t, 1, 8.0, 8.0, 0.05, 0.05, mom=mom)
+    got = t.cpu().numpy().astype(np.float64)
+    gm = mom.cpu().numpy().astype(np.float64)
+    for p in range(a.shape[0]):
+        want, _, m, _, gmin = O.local_search_gmin(a[p], 1, 8.0, 8.0, 0.05, 0.05)
+        M = np.convolve(a[p], a[p]).max()
+        # direction: fp32 products summed in fp32 (|G|/R <= 2M), so per height
+        merr = np.abs(gm[p] - m).max() / (0.1 * M)
+        herr, keep = ls_errors(got[p], want, gmin)
+        print(f"   row {p}: direction err {merr:.2e} M, height err {herr:.2e} "
+              f"({int((~keep).sum())} vanishing)")
+        # bounds about 3x the worst measured on MI355X (1.8e-6 M at n = 4095,
+        # 8.5e-8 of the largest height)
+        assert merr < 6e-6, merr
+        assert herr < 3e-7, herr
+        assert (got[p] >= 0).all() and abs(got[p].sum() - 1) < 1e-5
+
+
+def exact_b_vectors(n, rows, seed):
+    """Heights that are small integers times one power of two (sum near 1):
+    every product and partial sum of b is such an integer below 2^24 times
+    that power squared, so fp32 forms b exactly in any order and the search
+    kernel's weights differ from the fp64 oracle's only by the weight formula
+    (and fp32 rounding of c, R and G).  Five peaks are planted within 1e-6 of
+    the max in fp64 before rounding to integers, which leaves them 1e-4 ... 6e-4
+    below M: the range where fl(1+d) rounds and p ~ 1e3 ... 1e4 weights them."""
+    import importlib
+    W = importlib.import_module("sidon-autocorrelation-bound_amd.workloads")
+    base = W.structured_prior(rows, n, seed=seed).astype(np.float64)
+    out = []
+    for r in range(rows):
+        a, _ = W.plant_peaks(base[r], seed=r, round_fp32=False)
+        A = np.round(a * np.sqrt(1.5e7 / np.convolve(a, a).max()))
+        assert np.convolve(A, A).max() < 2 ** 24
+        out.append(A * 2.0 ** -np.round(np.log2(A.sum())))
+    return np.stack(out)
+
+
+@pytest.mark.parametrize("p", [8.0, 64.0, 1e3, 1e4])
+def test_search_weights_exact_b(S, O, p):
+    """The search kernel's weights 2^((p-1) log2(1+d)) (hardware log2 plus the
+    exact rounding term of fl(1+d)) against the oracle's exp((p-1) log1p(d)),
+    with b exact in fp32: the returned first moment m = 0.1 g per height.  At
+    p = 1e4 the planted peaks sit 1e-4 ... 6e-4 below M, where dropping the
+    rounding term would move their weights by up to (p-1) 2^-25 ~ 3e-4
+    relative and g by 1.4e-4 ... 6e-4 M, well above the bound (checked on the
+    CPU below).  Weights of b below the prefilter cut
+    (0.99 M at p = 1e4) flush to zero in fp32 either way (< 2^-126)."""
+    print(f"[ls] search weights on exact b, p={p}", flush=True)
+    n = 2000
+    a = exact_b_vectors(n, 4, seed=5)
+    t = dev(a)
+    mom = torch.zeros_like(t)
+    S.local_search(t, 1, p, p, 0.01, 0.01, mom=mom)
+    gm = mom.cpu().numpy().astype(np.float64)
+    worst_dropped = 0.0
+    for r in range(a.shape[0]):
+        # the kernel does not normalise at entry; neither does the oracle here
+        _, _, m, _, _ = O.local_search_gmin(a[r], 1, p, p, 0.01, 0.01)
+        b = np.convolve(a[r], a[r])
+        M = b.max()
+        near = int((b >= 0.99 * M).sum())
+        err = np.abs(gm[r] - m).max() / (0.1 * M)
+        print(f"   row {r}: direction err {err:.2e} M ({near} peaks within 1% of M)")
+        assert err < 2e-5, err          # measured up to 5.9e-6 M at p = 1e4
+        worst_dropped = max(worst_dropped, dropped_term_error(a[r], b, M, p, m))
+    if p >= 1e4:
+        # the bound is tight enough to see the mistake it guards against
+        print(f"   without the rounding term of fl(1+d): {worst_dropped:.2e} M")
+        assert worst_dropped > 5 * 2e-5
+
+
+def dropped_term_error(a, b, M, p, m_ref):
+    """Direction error (in units of M) of a weight formula that forgets the
+    rounding of fl(1+d) in fp32: c = (fl32(1 + d))^(p-1), everything else
+    exact.  Computed on the CPU; m_ref is the oracle's first moment."""
+    d = (b - M) / M
+    one_d = (np.float32(1.0) + d.astype(np.float32)).astype(np.float64)
+    c = np.where(one_d > 0, np.exp((p - 1) * np.log(np.maximum(one_d, 1e-300))), 0.0)
+    R = float((c * (1.0 + d)).sum())
+    n = a.size
+    G = np.array([np.dot(c[i:i + n], a) for i in range(n)])
+    g = G / R - M
+    return float(np.abs(0.1 * g - m_ref).max() / (0.1 * M))
+
+
+def test_concurrent_ragged_launches_match_serial(S, W):
+    """Two persistent ragged launches on two streams, each with its own queue
+    counter, running at the same time: each result equals the same launch run
+    alone, bit for bit."""
+    print("[ls] concurrent ragged launches", flush=True)
+    ns = np.array([600, 1200, 2100, 4096, 600, 1200, 2100, 4096] * 8, np.int32)
+    nmax = 4096
+    pops = []
+    for k in range(2):
+        pop = np.zeros((ns.size, nmax), np.float32)
+        for p, n in enumerate(ns):
+            pop[p, :n] = W.structured_prior(1, int(n), seed=300 + 100 * k + p)[0]
+        pops.append(pop)
+    nsd = dev(ns, torch.int32)
+    kw = dict(iters=40, p_start=4.0, p_end=1e3, lr_start=0.05, lr_end=0.01)
+    serial = []
+    for k in range(2):
+        t = dev(pops[k])
+        S.local_search_ragged(t, nsd, nmax, **kw)
+        serial.append(t)
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    queues = [torch.zeros(1, dtype=torch.int32, device="cuda") for _ in range(2)]
+    conc = [dev(pops[0]), dev(pops[1])]
+    torch.cuda.synchronize()
+    for k in range(2):
+        with torch.cuda.stream(streams[k]):
+            S.local_search_ragged(conc[k], nsd, nmax, queue=queues[k], stream=streams[k], **kw)
+    torch.cuda.synchronize()
+    for k in range(2):
+        assert torch.equal(conc[k], serial[k]), k

@@ -48,6 +48,9 @@ def test_argument_validation_without_device():
     assert L.sidon_select(fake, None, None, 0, 10, 3, 1, 0, 0.05, 1, fake, fake, fake, fake,
                           fake, None) == S.SIDON_ERR_ARG  # P not divisible by NI
     assert L.sidon_set_ls_block(4096, 300) == S.SIDON_ERR_ARG
+    # the persistent launch needs a caller-owned work-queue counter
+    assert L.sidon_local_search_ragged(fake, 64, fake, 4, 64, None, None, None, 10, 4.0, 8.0,
+                                       0.1, 0.01, 0.9, 0.999, 0, None, None) == S.SIDON_ERR_ARG
     assert S.lib().sidon_status_string(S.SIDON_ERR_LDS).startswith(b"length needs more LDS")
 
 
@@ -59,3 +62,32 @@ def test_lds_footprints():
     assert S.lds_bytes(2, 8400, 256) <= 160 * 1024
     assert S.lds_bytes(1, 8400, 512) > 160 * 1024   # fp64 local search does not fit at 8400
     assert S.lds_bytes(9, 100, 256) == -1
+
+
+def test_generation_desc_matches_header():
+    """The ctypes mirror of sidon_generation_desc lists the header's fields in
+    the header's order with the same C types."""
+    S = importlib.import_module("sidon-autocorrelation-bound_amd")
+    text = (REPO / "include" / "sidon.h").read_text()
+    body = re.search(r"typedef struct \{(.*?)\} sidon_generation_desc;", text, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    kinds = {"float*": ctypes.c_void_p, "int*": ctypes.c_void_p, "double*": ctypes.c_void_p,
+             "unsigned long long*": ctypes.c_void_p, "int": ctypes.c_int,
+             "float": ctypes.c_float, "double": ctypes.c_double,
+             "unsigned long long": ctypes.c_uint64}
+    want = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        m = re.match(r"((?:unsigned long long|int|float|double))\s*(\*?)\s*(.*)", decl)
+        base, star, names = m.group(1), m.group(2), m.group(3)
+        for part in names.split(","):
+            part = part.strip()
+            ptr = star or part.startswith("*")
+            name = part.lstrip("*").strip()
+            if part.startswith("*") is False and "*" in part:
+                ptr = True
+            want.append((name, kinds[base + ("*" if ptr else "")]))
+    got = [(f[0], f[1]) for f in S.GenerationDesc._fields_]
+    assert got == want

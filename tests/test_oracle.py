@@ -174,6 +174,27 @@ def test_local_search_kernel_semantics_first_step(O):
     np.testing.assert_allclose(m, 0.1 * g, rtol=1e-15)
 
 
+def test_local_search_gmin_records_the_direction(O):
+    """The gmin record is min over the steps of |g_i| / M: with a constant
+    schedule the per-step directions come from the first moments of 1- and
+    2-step runs (m1 = 0.1 g1, m2 = 0.9 m1 + 0.1 g2), and M from the heights
+    each step starts from.  The trajectory is the plain run's."""
+    rng = np.random.default_rng(41)
+    a = rng.uniform(size=90)
+    a /= a.sum()
+    kw = (8.0, 8.0, 0.02, 0.02)
+    one, _, m1, _, gmin1 = O.local_search_gmin(a, 1, *kw)
+    two, _, m2, _, gmin2 = O.local_search_gmin(a, 2, *kw)
+    assert np.array_equal(two, O.local_search(a, 2, *kw)[0])
+    g1 = m1 / (1 - 0.9)
+    g2 = (m2 - 0.9 * m1) / (1 - 0.9)
+    M1 = np.convolve(a, a).max()
+    M2 = np.convolve(one, one).max()
+    np.testing.assert_allclose(gmin1, np.abs(g1) / M1, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(gmin2, np.minimum(np.abs(g1) / M1, np.abs(g2) / M2),
+                               rtol=1e-9, atol=1e-12)
+
+
 def test_local_search_improves(O):
     """Self-test 8 (src/main.cu:257-288) on the fp64 oracle."""
     rng = np.random.default_rng(5)
@@ -232,6 +253,52 @@ def test_pdhg_lp_properties(O):
         viol.append(max(0.0, (np.convolve(a, g) - M).max() / M))
     assert viol[-1] < viol[0] and viol[-1] < 5e-3
     assert g.sum() > 1.0
+
+
+def test_pdhg_reaches_the_lp_optimum(O):
+    """After 4000 iterations PDHG's sum g is within 1e-3 of the LP optimum
+    max sum g s.t. (a*g)_k <= M, g >= 0 that scipy's HiGHS solver finds
+    (an extrapolation or step-order mistake leaves it far off)."""
+    from scipy.optimize import linprog
+    rng = np.random.default_rng(31)
+    for n in (24, 48):
+        a = rng.uniform(size=n)
+        a /= a.sum()
+        M = np.convolve(a, a).max()
+        A = np.zeros((2 * n - 1, n))
+        for i in range(n):
+            A[i:i + n, i] = a
+        res = linprog(-np.ones(n), A_ub=A, b_ub=np.full(2 * n - 1, M), bounds=(0, None),
+                      method="highs")
+        assert res.status == 0
+        opt = -res.fun
+        g, _ = O.pdhg(a, M, 4000)
+        assert opt >= 1.0 - 1e-12          # g = a is feasible
+        assert abs(g.sum() - opt) < 1e-3 * opt, (n, g.sum(), opt)
+
+
+def test_frank_wolfe_trace(O):
+    """The traced Frank-Wolfe round is the plain one, its direction is the
+    normalised PDHG solution, and the accepted step is the line search's
+    minimum."""
+    rng = np.random.default_rng(43)
+    n = 80
+    a = rng.uniform(size=n)
+    a /= a.sum()
+    out, acc, M1 = O.frank_wolfe(a, 1, 200, 12, 0.30)
+    out2, acc2, M2, g, lsM = O.frank_wolfe_trace(a, 1, 200, 12, 0.30)
+    assert np.array_equal(out, out2) and acc == acc2 == 1 and M1 == M2
+    # (the oracle rescales a by 1/sum a first: equal to rounding)
+    gp, _ = O.pdhg(a, np.convolve(a, a).max(), 200)
+    np.testing.assert_allclose(g, gp / gp.sum(), rtol=1e-12, atol=1e-16)
+    s = int(np.argmin(lsM))
+    assert M1 == lsM[s]
+    t = 0.30 * (s + 1) / 12
+    np.testing.assert_allclose(out, (1.0 - t) * a + t * g, rtol=1e-12, atol=1e-16)
+    for k in range(12):
+        tt = 0.30 * (k + 1) / 12
+        h = (1.0 - tt) * a + tt * g
+        assert abs(np.convolve(h, h).max() - lsM[k]) <= 1e-15
 
 
 def test_frank_wolfe_decreases_max(O):
