@@ -93,6 +93,66 @@ void structured_prior(std::mt19937_64& rng, int n, float* out) {
   for (int i = 0; i < n; ++i) out[i] = (float)(a[i] / S);
 }
 
+// A fresh host seed of the reference's level start (src/main.cu:505-551): with
+// probability 0.6 the structured shape every record shares, else a smooth
+// random Fourier function with noise; 2..7 cosine modes either way; heights
+// clamped at zero, then scaled to sum 1 in fp64 (uniform 1/n if all zero).
+// The structured shape, over x = (d + 1/2)/n with d = i or n-1-i (a mirror
+// image half the time): five bands [0, e1) [e1, e2) [e2, e3) [e3, e4)
+// [e4, e5) of base heights 4.2, 0.7, 5.2, 2.0, 0 with a relative ripple of
+// 0.45, 0.80, 0.30, 0.70, 0 times the mean of the modes, then a tail band of
+// height t (ripple 0.20); an atom of mass A on cell d = 0 and A/(10 d) on
+// d = 1..3.
+void host_seed(std::mt19937_64& rng, int n, float* out) {
+  std::uniform_real_distribution<double> U(0.0, 1.0);
+  std::normal_distribution<double> N01(0.0, 1.0);
+  const double kTwoPi = 6.283185307;
+  const int modes = 2 + (int)(U(rng) * 6);
+  std::vector<double> freq(modes), phase(modes);
+  for (int f = 0; f < modes; ++f) {
+    freq[f] = 1.0 + 10.0 * U(rng);
+    phase[f] = kTwoPi * U(rng);
+  }
+  std::vector<double> v(n, 0.0);
+  if (U(rng) < 0.6) {
+    const bool mirror = U(rng) < 0.5;
+    const double atom = 30.0 + 250.0 * U(rng);
+    double edge[5];
+    edge[0] = 0.28 + 0.16 * U(rng);
+    edge[1] = edge[0] + 0.10 + 0.14 * U(rng);
+    edge[2] = edge[1] + 0.02 + 0.08 * U(rng);
+    edge[3] = 0.70 + 0.16 * U(rng);
+    edge[4] = edge[3] + 0.05 + 0.15 * U(rng);
+    const double tail = 3.0 + 14.0 * U(rng);
+    const double height[6] = {4.2, 0.7, 5.2, 2.0, 0.0, tail};
+    const double ripple[6] = {0.45, 0.80, 0.30, 0.70, 0.0, 0.20};
+    for (int i = 0; i < n; ++i) {
+      const int d = mirror ? n - 1 - i : i;
+      const double x = (d + 0.5) / n;
+      double wave = 0.0;
+      for (int f = 0; f < modes; ++f) wave += std::cos(kTwoPi * (2.0 + 6.0 * freq[f]) * x + phase[f]);
+      wave /= modes;
+      int band = 0;
+      while (band < 5 && x >= edge[band]) ++band;
+      double h = height[band] * (1.0 + ripple[band] * wave);
+      if (d == 0) h += atom;
+      else if (d < 4) h += atom * 0.10 / d;
+      v[i] = (double)(float)std::max(0.0, h);
+    }
+  } else {
+    const double base = 0.3 + 0.7 * U(rng);
+    const double amp = 0.2 + 1.2 * U(rng);
+    for (int i = 0; i < n; ++i) {
+      double wave = 0.0;
+      for (int f = 0; f < modes; ++f) wave += std::cos(kTwoPi * freq[f] * i / n + phase[f]);
+      v[i] = (double)(float)std::max(0.0, base + amp * wave / modes + 0.3 * N01(rng));
+    }
+  }
+  double S = 0.0;
+  for (double x : v) S += x;
+  for (int i = 0; i < n; ++i) out[i] = S > 0.0 ? (float)(v[i] / S) : 1.0f / n;
+}
+
 template <typename T>
 T* dalloc(size_t count) {
   T* p = nullptr;
@@ -347,6 +407,7 @@ int uneven(const Args& A) {
   d.strength = dalloc<float>(P);
   d.order = dalloc<int>(P);
   d.gen = dalloc<unsigned long long>(1);
+  d.queue = dalloc<int>(1);
   d.P = P; d.stride = stride; d.nmax = nmax; d.islands = A.islands; d.elite = A.elite;
   d.migrate_every = 12; d.iters = A.iters;
   d.pStart = 400.0f; d.pEnd = 6e4f; d.lrStart = 0.012f; d.lrEnd = 0.0003f;  // warm tier, main.cu:629
@@ -575,14 +636,14 @@ int search(const Args& A) {
   std::vector<int> hmode(P);
   for (int p = 0; p < P; ++p) hmode[p] = (p / PI) % 4 == 3 ? 1 : 0;  // mirror descent island
   CHECK_HIP(hipMemcpy(mode, hmode.data(), P * sizeof(int), hipMemcpyHostToDevice));
-  {
+  {  // every slot of the first level starts from a fresh host seed (main.cu:502-551)
     std::vector<float> h((size_t)P * L[0]);
-    for (int p = 0; p < P; ++p) structured_prior(rng, L[0], &h[(size_t)p * L[0]]);
+    for (int p = 0; p < P; ++p) host_seed(rng, L[0], &h[(size_t)p * L[0]]);
     CHECK_HIP(hipMemcpy(pop, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
   }
   sidon_policy* pol = nullptr;
   CHECK_SIDON(sidon_policy_create(P, NI, A.elite, A.seed, &pol));
-  std::vector<int> curOp(P, 11);  // fresh structured seeds: cold / medium tiers first
+  std::vector<int> curOp(P, 9);  // fresh seeds: the cold tier (up to 18 % of P), then medium
   std::vector<double> hfit(P), cfit(P);
   std::vector<unsigned long long> hsig(P);
   std::vector<int> hnnz(P), hnact(P), p1(P), p2(P), op(P), tier(P), hskip(P), msrc(NI), mdst(NI);
@@ -725,12 +786,36 @@ int search(const Args& A) {
                 "\"cells\": %d, \"op_weights\": [", n, gen, seconds_since(t0), bestF, cells);
     for (int o = 0; o < SIDON_POLICY_OPS; ++o) std::printf("%s%.3f", o ? ", " : "", w[o]);
     std::printf("]}\n");
-    if (lvl + 1 < L.size()) {  // split every candidate n -> 2n
-      if (L[lvl + 1] != 2 * n) { std::fprintf(stderr, "ladder must double\n"); return 2; }
-      CHECK_SIDON(sidon_refine(pop, n, P, nxt, s));
-      CHECK_HIP(hipStreamSynchronize(s));
+    if (lvl + 1 < L.size()) {  // carry the population to the next length (main.cu:489-501)
+      const int m = L[lvl + 1];
+      if (m == 2 * n) {  // cell split: F preserved exactly
+        CHECK_SIDON(sidon_refine(pop, n, P, nxt, s));
+        CHECK_HIP(hipStreamSynchronize(s));
+      } else {  // any other ratio: nearest-neighbour resample a'_i = a_{min(n-1, i n / m)}
+        std::vector<float> h((size_t)P * n), r((size_t)P * m);
+        CHECK_HIP(hipMemcpy(h.data(), pop, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (int p = 0; p < P; ++p)
+          for (int i = 0; i < m; ++i)
+            r[(size_t)p * m + i] =
+                h[(size_t)p * n + std::min<long long>(n - 1, (long long)i * n / m)];
+        CHECK_HIP(hipMemcpy(nxt, r.data(), r.size() * sizeof(float), hipMemcpyHostToDevice));
+      }
+      // the last 20 % of the slots start afresh, to keep the population diverse
+      std::vector<float> fresh(m);
+      for (int p = 0; p < P; ++p) {
+        if (p >= 0.8 * P) {
+          host_seed(rng, m, fresh.data());
+          CHECK_HIP(hipMemcpy(nxt + (size_t)p * m, fresh.data(), m * sizeof(float),
+                              hipMemcpyHostToDevice));
+          curOp[p] = 9;   // cold / medium anneal
+        } else {
+          curOp[p] = 20;  // carried parents: warm refinement
+        }
+      }
       std::swap(pop, nxt);
-      std::fill(curOp.begin(), curOp.end(), 20);  // refined parents: warm refinement
+      std::printf("{\"carry\": \"%d -> %d\", \"form\": \"%s\", \"fresh\": %d}\n", n, m,
+                  m == 2 * n ? "cell split" : "nearest-neighbour resample",
+                  P - (int)std::ceil(0.8 * P));
     }
   }
   sidon_policy_destroy(pol);
@@ -764,6 +849,7 @@ int search_graph(const Args& A) {
   d.p1 = dalloc<int>(P); d.p2 = dalloc<int>(P); d.op = dalloc<int>(P);
   d.strength = dalloc<float>(P); d.order = dalloc<int>(P);
   d.gen = dalloc<unsigned long long>(1);
+  d.queue = dalloc<int>(1);
   d.P = P; d.stride = nmax; d.islands = A.islands; d.elite = A.elite; d.migrate_every = 12;
   d.iters = A.iters; d.sigma = A.sigma; d.activeTol = 1e-4; d.seed = A.seed;
   const auto t0 = Clock::now();
@@ -822,8 +908,10 @@ int search_graph(const Args& A) {
                                  2 * n * sizeof(float), P, hipMemcpyDeviceToDevice, s));
       CHECK_HIP(hipStreamSynchronize(s));
       CHECK_HIP(hipFree(packed));
-      unsigned long long zero = 0;
-      CHECK_HIP(hipMemcpy(d.gen, &zero, sizeof zero, hipMemcpyHostToDevice));
+      // mutation streams are keyed by gen: level l counts from 1000 l, as the
+      // reference keys its variation by gen + 1000 lvl (main.cu:741)
+      const unsigned long long gen0 = 1000ull * (lvl + 1);
+      CHECK_HIP(hipMemcpy(d.gen, &gen0, sizeof gen0, hipMemcpyHostToDevice));
     }
   }
   const double hf = host_F(bestA);
@@ -960,10 +1048,15 @@ bool check_args(const Args& A) {
       A.fw_iters < 0 || A.polish_iters < 0 || A.polish_top < 0)
     return fail("counts must not be negative");
   if (!(A.seconds >= 0.0)) return fail("seconds must not be negative");
-  if (A.mode == "--search" || A.mode == "--search-graph" || A.mode == "--fw-ladder")
+  // this build's own ladder modes split cells between levels; the search
+  // carries any ratio (nearest-neighbour resample), as the reference does
+  if (A.mode == "--search-graph" || A.mode == "--fw-ladder")
     for (size_t l = 1; l < A.ladder.size(); ++l)
       if (A.ladder[l] != 2 * A.ladder[l - 1])
-        return fail("each ladder level must be twice the one before (n -> 2n cell split)");
+        return fail("--search-graph and --fw-ladder need each ladder level twice the one "
+                    "before (n -> 2n cell split)");
+  for (int n : A.ladder)
+    if (n < 1) return fail("ladder lengths must be positive");
   return true;
 }
 

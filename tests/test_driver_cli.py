@@ -36,7 +36,7 @@ def test_help_lists_every_reference_flag():
     (["--bogus"], "unknown option --bogus"),
     (["--ladder", "--pop", "32"], "bad list for --ladder"),     # a list is required
     (["--ladder", "64,0"], "bad list for --ladder"),
-    (["--ladder", "64,100"], "twice the one before"),
+    (["--search-graph", "--ladder", "64,100"], "twice the one before"),
     (["--levels", "x"], "bad list for --ladder"),               # alias spelling
     (["--pop", "100", "--islands", "8"], "pop must be divisible by islands"),
     (["--gens", "-1"], "must not be negative"),

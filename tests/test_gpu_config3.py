@@ -8,7 +8,9 @@ read-back after each stage, and every stage is checked against the oracle
 applied to that stage's GPU input (reference: the generation loop body of
 src/main.cu:586-785):
   * local search: one candidate of every length class against the fp64
-    oracle's 200-step trajectory (kernel semantics);
+    oracle's 200-step trajectory (kernel semantics): F to 1.5e-3 (the fp32
+    and fp64 trajectories part once a direction passes through zero; the
+    per-height bound is the 10-step test's);
   * fp64 fitness: equal to hostF for all 512 candidates, bit for bit;
   * selection: the plan (parents, operators, strengths) and the migrated
     fitness equal the oracle's plan bit for bit, and migrated vectors and
@@ -174,8 +176,9 @@ def test_config3_local_search_vs_oracle(runs, O):
     assert worst < C3_F_BOUND
 
 
-# 200 fp32 steps against fp64: about 3x the worst F difference measured
-C3_F_BOUND = 1e-3
+# 200 fp32 steps against fp64: about 3x the worst F difference measured on
+# MI355X (4.5e-4, n = 8400)
+C3_F_BOUND = 1.5e-3
 
 
 @pytest.mark.parametrize("g", CHECKED)

@@ -157,3 +157,24 @@ def test_bench_mode():
         assert r["P"] == max(32, min(512, (4 << 20) // r["n"]))
         assert r["ls_ms"] > 0 and r["eval_f64_ms"] > 0
         assert 0 < r["tflops_executed"] < 157.3
+
+
+def test_search_any_ladder_ratio_and_fresh_seeds(O, tmp_path):
+    """A ladder that does not double (src/main.cu:491-501): each level carries
+    the population by nearest-neighbour resampling, the last 20 % of the
+    slots start from fresh host seeds, and the output verifies."""
+    base = tmp_path / "lad"
+    out = run(["--pop", "64", "--islands", "4", "--ladder", "600,1000,2100", "--ls-iters", "40",
+               "--cold-iters", "120", "--fw-rounds", "1", "--fw-iters", "50", "--gens", "2",
+               "--seconds", "60", "--polish-iters", "0", "--out", str(base), "--seed", "3"])
+    recs = [json.loads(line) for line in out.splitlines() if line.startswith("{")]
+    assert [r["level"] for r in recs if "level" in r] == [600, 1000, 2100]
+    carries = [r for r in recs if "carry" in r]
+    assert [c["carry"] for c in carries] == ["600 -> 1000", "1000 -> 2100"]
+    assert all(c["form"] == "nearest-neighbour resample" and c["fresh"] == 12 for c in carries)
+    stats = [r["stat"] for r in recs if "stat" in r]
+    assert stats[0]["cold"] > 0
+    final = recs[-1]
+    v = O.verify_file(str(base) + ".txt")
+    assert v["rc"] == 0 and v["n"] == final["n"]
+    assert abs(final["best_F"] - final["host_recheck"]) <= 1e-12 * final["best_F"]
