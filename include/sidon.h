@@ -38,7 +38,8 @@ typedef enum {
 } sidon_status;
 
 /* Reads and caches the attributes of `device` (-1 = current device) that the
- * launches need (LDS per workgroup, CU count).  Allocates nothing; called
+ * launches need (LDS per workgroup, CU count, the device address of the
+ * no-alloc polish's static scratch array).  Allocates nothing; called
  * implicitly on first use.  No reference counterpart. */
 int sidon_init(int device);
 /* Human-readable text of the last error on this thread. */
@@ -114,6 +115,19 @@ int sidon_polish_f64(double* pop, double* mom, double* vel, int n, int K, int it
                      double pStart, double pEnd, double lrStart, double lrEnd, double beta1,
                      double beta2, int mode, void* workspace, size_t workspace_bytes,
                      hipStream_t stream);
+/* sidon_polish_f64 with no workspace argument and no allocation: the
+ * workspace is a static device array of 32 MiB (33554432 bytes) that each GPU
+ * the library runs on holds until the process ends, resolved once per device
+ * by sidon_init.  Same kernels and results as sidon_polish_f64.  Returns
+ * SIDON_ERR_ARG when sidon_polish_f64_workspace(n, K) exceeds 32 MiB (K <= 15
+ * at n = 8400).  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (two streams, two threads) race; serialise them.  No counterpart
+ * in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). */
+int sidon_polish_f64_noalloc(double* pop, double* mom, double* vel, int n, int K, int iters,
+                             double pStart, double pEnd, double lrStart, double lrEnd,
+                             double beta1, double beta2, int mode, hipStream_t stream);
 /* Uneven lengths in one persistent launch: candidate p occupies
  * pop[p*stride .. p*stride + ns[p]) (ns on the device, ns[p] <= nmax <= stride);
  * `order` (may be NULL) lists the processing order, longest first gives the

@@ -32,6 +32,7 @@ EXPORTS = (
     "sidon_lds_limit", "sidon_eval_f64", "sidon_eval_f64_ragged", "sidon_eval_f64_d",
     "sidon_eval_exact", "sidon_local_search", "sidon_local_search_f64",
     "sidon_local_search_ragged", "sidon_polish_f64_workspace", "sidon_polish_f64",
+    "sidon_polish_f64_noalloc",
     "sidon_set_ls_block", "sidon_get_ls_block", "sidon_lp_grad",
     "sidon_lp_grad_f64", "sidon_frank_wolfe", "sidon_refine", "sidon_variation",
     "sidon_variation_ragged", "sidon_support_sig", "sidon_select", "sidon_advance_gen", "sidon_order_by_length",
@@ -106,6 +107,8 @@ def _declare(L):
         "sidon_polish_f64_workspace": ([_i, _i], C.c_size_t),
         "sidon_polish_f64": ([_vp, _vp, _vp, _i, _i, _i, _d, _d, _d, _d, _d, _d, _i, _vp,
                               C.c_size_t, _vp], _i),
+        "sidon_polish_f64_noalloc": ([_vp, _vp, _vp, _i, _i, _i, _d, _d, _d, _d, _d, _d, _i,
+                                      _vp], _i),
         "sidon_policy_create": ([_i, _i, _i, _u64, _vp], _i),
         "sidon_policy_destroy": ([_vp], None),
         "sidon_policy_adopt": ([_vp, _d], _i),
@@ -348,6 +351,21 @@ def polish_f64(pop: torch.Tensor, iters: int, pStart: float, pEnd: float, lrStar
     _check(lib().sidon_polish_f64(_ptr(pop), _ptr(mom), _ptr(vel), n, K, iters, pStart, pEnd,
                                   lrStart, lrEnd, beta1, beta2, mode, _ptr(workspace),
                                   workspace.numel(), _stream(stream)), "sidon_polish_f64")
+
+
+def polish_f64_noalloc(pop: torch.Tensor, iters: int, pStart: float, pEnd: float,
+                       lrStart: float, lrEnd: float, beta1: float = 0.9, beta2: float = 0.999,
+                       mode: int = 0, mom=None, vel=None, stream=None) -> None:
+    """sidon_polish_f64_noalloc: the fp64 polish on the library's static
+    scratch array (no workspace, no allocation; K x n must fit 32 MiB of
+    workspace)."""
+    _need_gpu()
+    if pop.dtype != torch.float64:
+        raise ValueError("polish_f64_noalloc needs float64 rows")
+    K, n = pop.shape
+    _check(lib().sidon_polish_f64_noalloc(_ptr(pop), _ptr(mom), _ptr(vel), n, K, iters, pStart,
+                                          pEnd, lrStart, lrEnd, beta1, beta2, mode,
+                                          _stream(stream)), "sidon_polish_f64_noalloc")
 
 
 def support_sig(pop: torch.Tensor, tol: float = 0.02, stream=None):

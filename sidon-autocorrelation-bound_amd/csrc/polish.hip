@@ -26,6 +26,7 @@
 // the same order, so all sums are deterministic and the arithmetic is the
 // fused kernel's step in fp64.
 #include <cmath>
+#include <cstdio>
 
 #include "common.hpp"
 #include "runtime.hpp"
@@ -592,4 +593,51 @@ extern "C" int sidon_polish_f64(double* pop, double* mom, double* vel, int n, in
   }
   return polish_run(pop, mom, vel, n, K, iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2, mode,
                     workspace, stream);
+}
+
+// ------------------------------------------------------------ no-alloc --
+// sidon_polish_f64_noalloc takes its workspace from `scratch`, a file-scope
+// device array: every GPU the library runs on holds its own copy until the
+// process ends, and every call on that GPU shares it (calls that overlap on
+// one GPU race).  Host code never passes the array's name to a kernel (on
+// the host it is a stand-in, not a device address): sidon_init resolves each
+// device's address once with hipGetSymbolAddress, and the entry point uses
+// that address.  The kernels rewrite every part of the workspace they read
+// (the setup kernel the heights, moments and flags; each step its b / c,
+// partials and sums), so what a previous call left there does no harm.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace sidon {
+int noalloc_scratch_address(void** out) {
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) {
+    set_last_error("hipGetSymbolAddress(scratch)", e);
+    return SIDON_ERR_HIP;
+  }
+  *out = p;
+  return SIDON_OK;
+}
+}  // namespace sidon
+
+extern "C" int sidon_polish_f64_noalloc(double* pop, double* mom, double* vel, int n, int K,
+                                        int iters, double pStart, double pEnd, double lrStart,
+                                        double lrEnd, double beta1, double beta2, int mode,
+                                        hipStream_t stream) {
+  if (int rc = check_polish_args(pop, n, K, iters, pStart, pEnd, lrStart, lrEnd, mode)) return rc;
+  if (K == 0 || iters == 0) return SIDON_OK;
+  if (ws_bytes(n, K) > kNoAllocScratchBytes) {
+    char msg[200];
+    std::snprintf(msg, sizeof msg,
+                  "polish (no-alloc): n = %d, K = %d need %zu bytes of scratch, above the "
+                  "%zu-byte static array (use sidon_polish_f64 with a workspace)",
+                  n, K, ws_bytes(n, K), kNoAllocScratchBytes);
+    set_last_error_msg(msg);
+    return SIDON_ERR_ARG;
+  }
+  int rc = SIDON_OK;
+  DeviceState* ds = device_state(&rc);
+  if (!ds) return rc;
+  return polish_run(pop, mom, vel, n, K, iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2, mode,
+                    ds->scratch, stream);
 }

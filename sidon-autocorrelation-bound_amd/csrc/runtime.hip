@@ -36,8 +36,11 @@ static int init_current(int dev) {
     lds_optin = 0;
   (void)hipGetLastError();
   SIDON_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  void* scratch = nullptr;
+  if (int rc = noalloc_scratch_address(&scratch)) return rc;
   s.lds_max = lds_optin > lds_block ? lds_optin : lds_block;
   s.cu_count = cus;
+  s.scratch = scratch;
   s.device.store(dev, std::memory_order_release);
   return SIDON_OK;
 }

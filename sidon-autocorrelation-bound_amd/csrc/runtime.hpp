@@ -18,7 +18,15 @@ struct DeviceState {
   std::atomic<int> device{-1};
   int lds_max = 0;        // max dynamic LDS bytes per workgroup
   int cu_count = 0;
+  void* scratch = nullptr;  // this device's address of the no-alloc polish scratch
 };
+
+// Bytes of the static device array behind sidon_polish_f64_noalloc.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+
+// Device address of that array on the current device (hipGetSymbolAddress;
+// polish.hip, where the array lives).
+int noalloc_scratch_address(void** out);
 
 // Returns the state of the current device (initialising it on first use), or
 // nullptr with *rc set on failure.
