@@ -52,6 +52,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0,
                     help="workgroup size of the local search at this n (0: the library's rule)")
+    ap.add_argument("--fw-only", action="store_true",
+                    help="time the Frank-Wolfe / Chambolle-Pock kernel alone (config 5's "
+                         "levels) and print its JSON")
+    ap.add_argument("--polish-only", action="store_true",
+                    help="time the whole-device fp64 polish alone and print its JSON")
     ap.add_argument("--f-evals-only", action="store_true",
                     help="run only the F-evaluation leg and print its rows as one JSON line")
     ap.add_argument("--profile-copy-mib", type=int, default=0,
@@ -179,6 +184,65 @@ def f_evals(S, W, torch, with_cpu: bool, seed: int):
     return out
 
 
+def fw_legs(S, W, torch, seed: int, rounds: int = 1, cp_iters: int = 200, P: int = 256):
+    """Frank-Wolfe rounds solved by Chambolle-Pock over P structured-prior
+    candidates at config 5's levels n = 1050, 2100, 4200, 8400 (a bounded
+    sample: `rounds` rounds of `cp_iters` iterations, not the 50 x 2000 of a
+    full level), HIP events around the launch.  Matrix work the kernel ran is
+    its block-step count (4 x 2048 FLOP each); dense-equivalent work is 4 n^2
+    per CP iteration plus 2 n^2 (nT + 1) per round."""
+    out = []
+    stream = torch.cuda.current_stream()
+    for n in (1050, 2100, 4200, 8400):
+        start = torch.as_tensor(W.structured_prior(P, n, seed=seed + n), device="cuda")
+        pop = start.clone()
+        S.frank_wolfe(pop, 1, 10)                      # warm-up
+        pop.copy_(start)
+        torch.cuda.synchronize()
+        S.fw_block_steps(reset=True)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        S.frank_wolfe(pop, rounds, cp_iters)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        s = ev0.elapsed_time(ev1) / 1e3
+        steps = S.fw_block_steps(reset=True)
+        dense = P * rounds * (4.0 * n * n * cp_iters + 2.0 * n * n * 13)
+        out.append({"n": n, "P": P, "rounds": rounds, "cp_iters": cp_iters, "ms": s * 1e3,
+                    "cp_iterations_per_s": P * rounds * cp_iters / s,
+                    "executed_tflops": steps * 8192.0 / s / 1e12,
+                    "frac_fp32_peak": steps * 8192.0 / s / 1e12 / PEAK_FP32,
+                    "dense_equivalent_tflops": dense / s / 1e12,
+                    "block_size": 1024 if n >= 2048 else (512 if n >= 512 else 256)})
+    return out
+
+
+def polish_leg(S, W, torch, seed: int, n: int = 8400, K: int = 8, iters: int = 100):
+    """The whole-device fp64 polish (sidon_polish_f64) of K candidates at n over
+    `iters` steps of the anneal, HIP events around the launch sequence.  Dense
+    matrix work: n^2 / 2 fp64 multiply-adds for the symmetric autoconvolution
+    plus n^2 for the correlation, per candidate-step."""
+    start = torch.as_tensor(W.structured_prior(K, n, seed=seed).astype("float64"), device="cuda")
+    ws = torch.empty(S.lib().sidon_polish_f64_workspace(n, K), dtype=torch.uint8, device="cuda")
+    sched = (200.0, 6000.0, 0.01, 0.0002)
+    pop = start.clone()
+    S.polish_f64(pop, 4, *sched, workspace=ws)            # warm-up
+    pop.copy_(start)
+    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    S.polish_f64(pop, iters, *sched, workspace=ws)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    s = ev0.elapsed_time(ev1) / 1e3
+    dense = 2.0 * 1.5 * n * n * K * iters
+    return {"n": n, "K": K, "iters": iters, "ms": s * 1e3,
+            "candidate_steps_per_s": K * iters / s, "dense_fp64_tflops": dense / s / 1e12,
+            "frac_fp64_peak": dense / s / 1e12 / PEAK_FP64,
+            "schedule": "p 200 -> 6000, lr 0.01 -> 0.0002 (the driver's first polish pass)"}
+
+
 def hostf_evals(vecs, threads: int, seconds: float = 1.5) -> float:
     """hostF-order fp64 F-evaluations per second on `threads` host threads, one
     vector per task (the oracle's C restatement; ctypes releases the GIL)."""
@@ -209,6 +273,18 @@ def main() -> int:
     S.init(0)
     host = {"nproc": os.cpu_count(),
             "threads": int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)}
+    if args.profile_copy_mib and (args.fw_only or args.polish_only or args.f_evals_only):
+        src = torch.ones(args.profile_copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    if args.fw_only:
+        print(json.dumps({"frank_wolfe": fw_legs(S, W, torch, args.seed), "host": host}))
+        return 0
+    if args.polish_only:
+        print(json.dumps({"polish": polish_leg(S, W, torch, args.seed), "host": host}))
+        return 0
     if args.f_evals_only:
         print(json.dumps({"f_evals": f_evals(S, W, torch, not args.no_cpu, args.seed),
                           "host": host}))

@@ -16,13 +16,16 @@
 //                segments at once (one MFMA tile = 16 output segments);
 //      corr:  G[16I + t] = sum_e sum_s c[16e + t + s] * a[16(e-I) + s]
 //             -> Hankel block C_e[t][s] = c[16e + t + s] times 16 segments;
-//  * the vector `a` lives in LDS in an overlapped segment layout: 20
-//    elements per 16-element segment (the last four heights of the previous
-//    segment, then its own sixteen), 16 zero segments on each side.  The
-//    MFMA B-operand reads (one 16-byte read per lane, stride 16 heights
-//    across lanes) hit disjoint banks, the four Toeplitz A-operand heights
-//    of a lane are always four adjacent elements, and every out-of-range
-//    index reads an exact zero;
+//  * the vector `a` lives in LDS in an overlapped segment layout: SS (20 or
+//    24) elements per 16-element segment (the last four heights of the
+//    previous segment, then its own sixteen, then SS - 20 unused slots), 16
+//    zero segments on each side.  The four Toeplitz A-operand heights of a
+//    lane are always four adjacent elements and every out-of-range index
+//    reads an exact zero.  The MFMA B operand is one 16-byte read per lane
+//    from 16 consecutive segments; gfx950 serves such a read 16 lanes a pass
+//    over 64 four-byte banks, and measured on the device a 20-element stride
+//    costs a second pass (2-way conflict) where a 24-element stride costs
+//    none, so kernels take SS = 24 wherever the LDS budget allows;
 //  * the work of each product is cut into W equal contiguous ranges of
 //    (tile, block-offset) units, one per wave ("stream-K").  A tile is never
 //    split among more than two waves, and two partial sums added onto an
@@ -37,6 +40,7 @@ namespace sidon {
 constexpr int kWave = 64;
 constexpr int kSeg = 16;          // MFMA tile edge / segment length
 constexpr int kSegStride = 20;    // padded elements per segment in LDS (16-byte aligned)
+constexpr int kSegStrideWide = 24;  // the same with 4 unused slots: no LDS bank conflicts
 constexpr int kPadSeg = 16;       // zero segments on each side of `a`
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -125,20 +129,23 @@ __device__ __forceinline__ T block_max(T v, T* slots) {
 // four elements pad_addr(x)-3 .. pad_addr(x); valid for x in
 // [-256, 16*(N16+16)).  Writes go through pad_put, which also refreshes the
 // copy held by the next segment.
+template <int SS = kSegStride>
 __host__ __device__ __forceinline__ int pad_addr(int x) {
   const int y = x + kPadSeg * kSeg;
-  return (y >> 4) * kSegStride + 4 + (y & 15);
+  return (y >> 4) * SS + 4 + (y & 15);
 }
-template <typename T>
+template <int SS = kSegStride, typename T>
 __device__ __forceinline__ void pad_put(T* s, int x, T v) {
-  const int p = pad_addr(x);
+  const int p = pad_addr<SS>(x);
   s[p] = v;
-  if (((x + kPadSeg * kSeg) & 15) >= 12) s[p + 4] = v;  // copy at the head of the next segment
+  // copy at the head of the next segment
+  if (((x + kPadSeg * kSeg) & 15) >= 12) s[p + (SS - kSeg)] = v;
 }
 __host__ __device__ __forceinline__ int nseg16(int n) { return (n + kSeg - 1) / kSeg; }
 // floats (elements) of the padded vector buffer
+template <int SS = kSegStride>
 __host__ __device__ __forceinline__ int pad_len(int n) {
-  return (nseg16(n) + 2 * kPadSeg) * kSegStride;
+  return (nseg16(n) + 2 * kPadSeg) * SS;
 }
 // elements of the weight / autoconvolution buffer (normal layout, zero tail)
 __host__ __device__ __forceinline__ int cbuf_len(int n) { return kSeg * (2 * nseg16(n) + 16); }
@@ -290,15 +297,15 @@ __device__ __forceinline__ typename Acc4<T>::type run_steps(int cnt, const Ld& l
 //   B operand:                      sB[16(K0 + t - d) + s]      (column = t)
 // LDS addresses are kept as integer element indices so the reads stay
 // ds_read.
-template <typename T, int D = Pipe<T>::kDepth>
+template <typename T, int D = Pipe<T>::kDepth, int SS = kSegStride>
 __device__ __forceinline__ typename Acc4<T>::type conv_tile(const T* __restrict__ sA,
                                                            const T* __restrict__ sB, int K0,
                                                            int da, int db) {
-  constexpr int S = kSegStride;
+  constexpr int S = SS;
   const int lane = opaque((int)threadIdx.x) & (kWave - 1);
   const int t = lane & 15, kk = lane >> 4;
-  const int pa = pad_addr(16 * da + t - 4 * kk);
-  const int pb = pad_addr(16 * (K0 + t - da)) + 4 * kk;
+  const int pa = pad_addr<SS>(16 * da + t - 4 * kk);
+  const int pb = pad_addr<SS>(16 * (K0 + t - da)) + 4 * kk;
   return run_steps<T, D>(db - da, [&](int i, T (&x)[4], T (&y)[4]) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) x[q] = sA[pa + i * S - q];
@@ -310,15 +317,15 @@ __device__ __forceinline__ typename Acc4<T>::type conv_tile(const T* __restrict_
 // sum_s c[16e + row + s] * a[16(e - I0 - t) + s]  = G[16(I0 + t) + row].
 //   A operand: C_e[t][s] = c[16e + t + s]       (four consecutive elements)
 //   B operand: sA[16(e - I0 - t) + s]           (column = t)
-template <typename T, int D = Pipe<T>::kDepth>
+template <typename T, int D = Pipe<T>::kDepth, int SS = kSegStride>
 __device__ __forceinline__ typename Acc4<T>::type corr_tile(const T* __restrict__ c,
                                                            const T* __restrict__ sA, int I0,
                                                            int ea, int eb) {
-  constexpr int S = kSegStride;
+  constexpr int S = SS;
   const int lane = opaque((int)threadIdx.x) & (kWave - 1);
   const int t = lane & 15, kk = lane >> 4;
   const int pc = 16 * ea + t + 4 * kk;
-  const int pb = pad_addr(16 * (ea - I0 - t)) + 4 * kk;
+  const int pb = pad_addr<SS>(16 * (ea - I0 - t)) + 4 * kk;
   return run_steps<T, D>(eb - ea, [&](int i, T (&x)[4], T (&y)[4]) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) x[q] = c[pc + 16 * i + q];
@@ -370,13 +377,13 @@ __device__ __forceinline__ int next_step(const unsigned* segmask, int e, int eb,
 }
 
 // Hankel tile over the needed steps of [ea, eb) only (see corr_tile).
-template <typename T, int D = Pipe<T>::kDepth>
+template <typename T, int D = Pipe<T>::kDepth, int SS = kSegStride>
 __device__ __forceinline__ typename Acc4<T>::type corr_tile_sparse(const T* __restrict__ c,
                                                                   const T* __restrict__ sA,
                                                                   int I0, int ea, int eb,
                                                                   const unsigned* segmask,
                                                                   int& steps) {
-  constexpr int S = kSegStride;
+  constexpr int S = SS;
   const int lane = opaque((int)threadIdx.x) & (kWave - 1);
   const int t = lane & 15, kk = lane >> 4;
   typename Acc4<T>::type acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
@@ -384,7 +391,7 @@ __device__ __forceinline__ typename Acc4<T>::type corr_tile_sparse(const T* __re
   while (e < eb) {
     const int e1 = next_step(segmask, e + 1, eb, false);
     const int pc = 16 * e + t + 4 * kk;
-    const int pb = pad_addr(16 * (e - I0 - t)) + 4 * kk;
+    const int pb = pad_addr<SS>(16 * (e - I0 - t)) + 4 * kk;
     run_steps_into<T, D>(e1 - e, [&](int i, T (&x)[4], T (&y)[4]) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) x[q] = c[pc + 16 * i + q];
@@ -416,7 +423,7 @@ __device__ __forceinline__ void put4(T* out, int col, const A4& acc, bool whole)
 // sB are padded vectors of length n.  `out` must be exactly zero before the
 // call (two commuting partial sums per element at most).  Every wave calls
 // this with its wave index; returns without a barrier.
-template <typename T>
+template <typename T, int SS = kSegStride>
 __device__ void conv_mfma(const T* __restrict__ sA, const T* __restrict__ sB, T* out, int n,
                           int wave, int nwaves) {
   const int N16 = nseg16(n);
@@ -440,7 +447,8 @@ __device__ void conv_mfma(const T* __restrict__ sA, const T* __restrict__ sB, T*
     const int ua = sh.u0 > base ? sh.u0 : base;
     const int ub = sh.u1 < base + cost ? sh.u1 : base + cost;
     const int K0 = 16 * j;
-    const auto acc = conv_tile<T>(sA, sB, K0, dlo + (ua - base), dlo + (ub - base));
+    const auto acc =
+        conv_tile<T, Pipe<T>::kDepth, SS>(sA, sB, K0, dlo + (ua - base), dlo + (ub - base));
     put4(out, K0 + t, acc, (ua == base) && (ub == base + cost));
     base += cost;
     ++j;
@@ -473,16 +481,16 @@ __device__ __forceinline__ int autoconv_cost(int j, int N16) {
 // zone d >= 8j, column t keeps its B operand only while d < dp(K0 + t) =
 // 8j + ceil(t/2); a lane past that reads four zeros from the left padding
 // instead, so zone steps run in the same pipelined loop.
-template <typename T, int D = Pipe<T>::kDepth>
+template <typename T, int D = Pipe<T>::kDepth, int SS = kSegStride>
 __device__ __forceinline__ typename Acc4<T>::type autoconv_tile(const T* __restrict__ sA, int K0,
                                                                int da, int db) {
-  constexpr int S = kSegStride;
+  constexpr int S = SS;
   const int lane = opaque((int)threadIdx.x) & (kWave - 1);
   const int t = lane & 15, kk = lane >> 4;
-  const int pa = pad_addr(16 * da + t - 4 * kk);
-  const int pb = pad_addr(16 * (K0 + t - da)) + 4 * kk;
+  const int pa = pad_addr<SS>(16 * da + t - 4 * kk);
+  const int pb = pad_addr<SS>(16 * (K0 + t - da)) + 4 * kk;
   const int live = (K0 >> 1) + ((t + 1) >> 1) - da;  // steps i < live use this column
-  const int zero = pad_addr(-kPadSeg * kSeg);       // first padding segment: always zero
+  const int zero = pad_addr<SS>(-kPadSeg * kSeg);       // first padding segment: always zero
   return run_steps<T, D>(db - da, [&](int i, T (&x)[4], T (&y)[4]) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) x[q] = sA[pa + i * S - q];
@@ -581,26 +589,27 @@ __device__ __forceinline__ void autoconv_items(int n, unsigned* items, int* scra
 // The loop runs its full 15 trips with the surplus terms masked, so the reads
 // of all trips are issued together; every read stays inside the padded
 // vector (i >= -15, k - i <= k/2 + 15).
-template <typename T>
+template <typename T, int SS = kSegStride>
 __device__ __forceinline__ T autoconv_finish(const T* __restrict__ sA, T h, int k) {
   const int K = k >> 4, row = k & 15;
   const int dp = (K + 1) >> 1;
   const int i0 = 16 * (dp - 1) + row + 1;
   const int cnt = ((k + 1) >> 1) - i0;
-  // a_{i0+u}: from pad_addr(i0), one segment boundary at most (skip the 4
-  // copy slots there); a_{k-i0-u} = a_{16(K-dp)+15-u}: one segment, descending
-  const int xs = pad_addr(i0), r1 = (i0 + kPadSeg * kSeg) & 15;
-  const int ys = pad_addr(16 * (K - dp) + 15);
+  // a_{i0+u}: from pad_addr(i0), one segment boundary at most (skip the SS - 16
+  // copy / padding slots there); a_{k-i0-u} = a_{16(K-dp)+15-u}: one segment,
+  // descending
+  const int xs = pad_addr<SS>(i0), r1 = (i0 + kPadSeg * kSeg) & 15;
+  const int ys = pad_addr<SS>(16 * (K - dp) + 15);
   T r = h;
 #pragma unroll
   for (int u = 0; u < 15; ++u) {
-    const T x = sA[xs + u + (((r1 + u) >> 4) << 2)];
+    const T x = sA[xs + u + ((r1 + u) >> 4) * (SS - kSeg)];
     const T y = sA[ys - u];
     r += u < cnt ? x * y : T(0);
   }
   T b = r + r;
   if (!(k & 1)) {
-    const T x = sA[pad_addr(k >> 1)];
+    const T x = sA[pad_addr<SS>(k >> 1)];
     b += x * x;
   }
   return b;
@@ -612,17 +621,17 @@ __device__ __forceinline__ T autoconv_finish(const T* __restrict__ sA, T h, int 
 // output 0, so the eighteen heights a_{i0 .. i0+17} and the fifteen a_{16(K-dp)
 // + 15 - u} are read once for all four outputs (the same elements, the same
 // products in the same order as autoconv_finish).
-template <typename T>
+template <typename T, int SS = kSegStride>
 __device__ __forceinline__ void autoconv_finish_lane(const T* __restrict__ sA,
                                                      typename Acc4<T>::type& acc, int K, int kk) {
   static_assert(Acc4<T>::kRowMul == 4 && Acc4<T>::kRowStride == 1, "fp32 accumulator layout");
   const int dp = (K + 1) >> 1;
   const int i0 = 16 * (dp - 1) + 4 * kk + 1;
-  const int xs = pad_addr(i0), r1 = (i0 + kPadSeg * kSeg) & 15;
-  const int ys = pad_addr(16 * (K - dp) + 15);
+  const int xs = pad_addr<SS>(i0), r1 = (i0 + kPadSeg * kSeg) & 15;
+  const int ys = pad_addr<SS>(16 * (K - dp) + 15);
   T xv[18], yv[15];
 #pragma unroll
-  for (int v = 0; v < 18; ++v) xv[v] = sA[xs + v + (((r1 + v) >> 4) << 2)];
+  for (int v = 0; v < 18; ++v) xv[v] = sA[xs + v + ((r1 + v) >> 4) * (SS - kSeg)];
 #pragma unroll
   for (int u = 0; u < 15; ++u) yv[u] = sA[ys - u];
 #pragma unroll
@@ -634,7 +643,7 @@ __device__ __forceinline__ void autoconv_finish_lane(const T* __restrict__ sA,
     for (int u = 0; u < 15; ++u) s += u < cnt ? xv[r + u] * yv[u] : T(0);
     T b = s + s;
     if (!(k & 1)) {
-      const T x = sA[pad_addr(k >> 1)];
+      const T x = sA[pad_addr<SS>(k >> 1)];
       b += x * x;
     }
     acc[r] = b;
@@ -660,7 +669,7 @@ __device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
 // *bmax to the tile's largest b_k, so the finishing work overlaps the other
 // waves' matrix work.  Every wave of the workgroup calls this; returns
 // without a barrier.
-template <typename T, int D = Pipe<T>::kDepth>
+template <typename T, int D = Pipe<T>::kDepth, int SS = kSegStride>
 __device__ void autoconv_mfma(const T* __restrict__ sA, T* out, int n, int* ctr,
                               const unsigned* items, unsigned* steps = nullptr,
                               T* bmax = nullptr) {
@@ -679,19 +688,19 @@ __device__ void autoconv_mfma(const T* __restrict__ sA, T* out, int n, int* ctr,
       autoconv_drange(j, N16, dlo, dhi);
       const int cost = dhi >= dlo ? dhi + 1 - dlo : 0;
       const int K0 = 16 * j;
-      auto acc = autoconv_tile<T, D>(sA, K0, dlo + u0, dlo + u1);
+      auto acc = autoconv_tile<T, D, SS>(sA, K0, dlo + u0, dlo + u1);
       if (bmax) {  // whole tiles: this wave owns the outputs, so it finishes them
         const int kk = (opaque((int)threadIdx.x) & (kWave - 1)) >> 4;
         T m = T(0);
         if constexpr (Acc4<T>::kRowStride == 1) {
-          autoconv_finish_lane<T>(sA, acc, K0 + t, kk);
+          autoconv_finish_lane<T, SS>(sA, acc, K0 + t, kk);
 #pragma unroll
           for (int r = 0; r < 4; ++r) m = acc[r] > m ? acc[r] : m;
         } else {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int k = 16 * (K0 + t) + Acc4<T>::kRowMul * kk + Acc4<T>::kRowStride * r;
-            acc[r] = autoconv_finish(sA, acc[r], k);
+            acc[r] = autoconv_finish<T, SS>(sA, acc[r], k);
             m = acc[r] > m ? acc[r] : m;
           }
         }
@@ -720,7 +729,7 @@ __device__ __forceinline__ void corr_items(int n, unsigned* items, int* scratch,
 // >= 2n-1 up to cbuf_len(n); a padded.  `out` must be exactly zero before
 // the call.  With `segmask` (the weight segments' non-zero bits) steps that
 // read only zero weights are skipped.
-template <typename T, int D = Pipe<T>::kDepth>
+template <typename T, int D = Pipe<T>::kDepth, int SS = kSegStride>
 __device__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* out, int n,
                           int* ctr, const unsigned* items, const unsigned* segmask,
                           unsigned* steps = nullptr) {
@@ -738,9 +747,9 @@ __device__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* 
       const int I0 = 16 * j;
       typename Acc4<T>::type acc;
       if (segmask) {
-        acc = corr_tile_sparse<T, D>(c, sA, I0, I0 + u0, I0 + u1, segmask, done);
+        acc = corr_tile_sparse<T, D, SS>(c, sA, I0, I0 + u0, I0 + u1, segmask, done);
       } else {
-        acc = corr_tile<T, D>(c, sA, I0, I0 + u0, I0 + u1);
+        acc = corr_tile<T, D, SS>(c, sA, I0, I0 + u0, I0 + u1);
         done += u1 - u0;
       }
       put4(out, I0 + t, acc, u0 == 0 && u1 == cost);

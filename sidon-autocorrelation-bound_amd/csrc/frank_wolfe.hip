@@ -44,9 +44,9 @@ __global__ void k_take_fw_block_steps(int reset) {
       reset ? atomicExch(&g_fw_block_steps, 0ull) : atomicAdd(&g_fw_block_steps, 0ull);
 }
 
-template <int BD>
+template <int BD, int SS = kSegStride>
 __host__ __device__ inline long long fw_lds_bytes(int n) {
-  const long long f = 2LL * pad_len(n) + cbuf_len(n) + 2 * (BD / kWave);
+  const long long f = 2LL * pad_len<SS>(n) + cbuf_len(n) + 2 * (BD / kWave);
   return f * 4 + 4LL * kMaxConvTiles + 4LL * 2 * segmask_words(n);
 }
 
@@ -58,7 +58,7 @@ struct FwArgs {
 };
 
 // GT: correlation tiles owned per wave (ceil(corr_tiles / waves)).
-template <int BD, int GT>
+template <int BD, int GT, int SS>
 __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
   constexpr int NW = BD / kWave;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -67,15 +67,15 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
   const int n = A.n;
   float* a = A.pop + (size_t)cand * n;
   float* sA = reinterpret_cast<float*>(smem);
-  float* sH = sA + pad_len(n);     // gbar during PDHG, h during the line search
-  float* sL = sH + pad_len(n);     // lambda / autoconvolution
+  float* sH = sA + pad_len<SS>(n);     // gbar during PDHG, h during the line search
+  float* sL = sH + pad_len<SS>(n);     // lambda / autoconvolution
   float* slots = sL + cbuf_len(n);
   int* owner = reinterpret_cast<int*>(slots + 2 * NW);  // Toeplitz tile -> wave
   const int SW = segmask_words(n);
   unsigned* lmask = reinterpret_cast<unsigned*>(owner + kMaxConvTiles);  // 2 x SW: non-zero lambda
   const int tid = threadIdx.x, wave = wave_id();
   const int lane = tid & (kWave - 1), t = lane & 15, kk = lane >> 4;
-  const int LA = pad_len(n), LC = cbuf_len(n);
+  const int LA = pad_len<SS>(n), LC = cbuf_len(n);
   const int N16 = nseg16(n), NTc = conv_tiles(n), NTg = corr_tiles(n);
   const int nb = 2 * n - 1;
   int U = 0;  // block steps of one full autoconvolution
@@ -100,12 +100,12 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
   }
   __syncthreads();
   float sl = 0.0f;
-  for (int i = tid; i < n; i += BD) { const float x = a[i]; pad_put(sA, i, x); sl += x; }
+  for (int i = tid; i < n; i += BD) { const float x = a[i]; pad_put<SS>(sA, i, x); sl += x; }
   const float s = block_sum<float, NW>(sl, slots);
   if (!(s > 0.0f)) return;
   {
     const float inv = 1.0f / s;
-    for (int i = tid; i < n; i += BD) pad_put(sA, i, sA[pad_addr(i)] * inv);
+    for (int i = tid; i < n; i += BD) pad_put<SS>(sA, i, sA[pad_addr<SS>(i)] * inv);
   }
   __syncthreads();
 
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
   float g[GT][4];
 
   for (int r = 0; r < A.rounds; ++r) {
-    conv_mfma<float>(sA, sA, sL, n, wave, NW);
+    conv_mfma<float, SS>(sA, sA, sL, n, wave, NW);
     if (wave == 0) ran += U;
     __syncthreads();
     float loc = 0.0f;
@@ -128,10 +128,10 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int i = 16 * (16 * j + t) + 4 * kk + q;
-        g[u][q] = (j < NTg && i < n) ? sA[pad_addr(i)] : 0.0f;
+        g[u][q] = (j < NTg && i < n) ? sA[pad_addr<SS>(i)] : 0.0f;
       }
     }
-    for (int i = tid; i < n; i += BD) pad_put(sH, i, sA[pad_addr(i)]);
+    for (int i = tid; i < n; i += BD) pad_put<SS>(sH, i, sA[pad_addr<SS>(i)]);
     for (int k = tid; k < LC; k += BD) sL[k] = 0.0f;
     for (int w = tid; w < 2 * SW; w += BD) lmask[w] = 0u;
     __syncthreads();
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
         if (owner[j] != wave) continue;
         int dlo, dhi;
         conv_drange(j, N16, dlo, dhi);
-        const f32x4 acc = conv_tile<float>(sA, sH, 16 * j, dlo, dhi + 1);
+        const f32x4 acc = conv_tile<float, Pipe<float>::kDepth, SS>(sA, sH, 16 * j, dlo, dhi + 1);
         ran += dhi + 1 - dlo;
         const int k0 = 16 * (16 * j + t) + 4 * kk;
         bool nz = false;
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
         if (j >= NTg) continue;
         const int I0 = 16 * j;
         int steps = 0;
-        const f32x4 c = corr_tile_sparse<float>(sL, sA, I0, I0, I0 + N16 + 15, mcur, steps);
+        const f32x4 c = corr_tile_sparse<float, Pipe<float>::kDepth, SS>(sL, sA, I0, I0, I0 + N16 + 15, mcur, steps);
         ran += steps;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
           if (i < n) {
             float gn = g[u][q] + tau * (1.0f - c[q]);
             gn = gn > 0.0f ? gn : 0.0f;
-            pad_put(sH, i, 2.0f * gn - g[u][q]);
+            pad_put<SS>(sH, i, 2.0f * gn - g[u][q]);
             g[u][q] = gn;
           }
         }
@@ -213,12 +213,12 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int i = 16 * (16 * j + t) + 4 * kk + q;
-          if (i < n) pad_put(sH, i, (1.0f - tt) * sA[pad_addr(i)] + tt * g[u][q]);
+          if (i < n) pad_put<SS>(sH, i, (1.0f - tt) * sA[pad_addr<SS>(i)] + tt * g[u][q]);
         }
       }
       for (int k = tid; k < LC; k += BD) sL[k] = 0.0f;
       __syncthreads();
-      conv_mfma<float>(sH, sH, sL, n, wave, NW);
+      conv_mfma<float, SS>(sH, sH, sL, n, wave, NW);
       if (wave == 0) ran += U;
       __syncthreads();
       float lc = 0.0f;
@@ -234,30 +234,30 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int i = 16 * (16 * j + t) + 4 * kk + q;
-        if (i < n) pad_put(sA, i, (1.0f - bestT) * sA[pad_addr(i)] + bestT * g[u][q]);
+        if (i < n) pad_put<SS>(sA, i, (1.0f - bestT) * sA[pad_addr<SS>(i)] + bestT * g[u][q]);
       }
     }
     for (int k = tid; k < LC; k += BD) sL[k] = 0.0f;
     __syncthreads();
   }
   __syncthreads();
-  for (int i = tid; i < n; i += BD) a[i] = sA[pad_addr(i)];
+  for (int i = tid; i < n; i += BD) a[i] = sA[pad_addr<SS>(i)];
   if (lane == 0 && ran) atomicAdd(&g_fw_block_steps, ran);
 }
 
-template <int BD, int GT>
-int launch_fw(const FwArgs& A, int P, hipStream_t stream) {
+template <int BD, int GT, int SS>
+int launch_fw_layout(const FwArgs& A, int P, hipStream_t stream) {
   int rc = SIDON_OK;
   DeviceState* ds = device_state(&rc);
   if (!ds) return rc;
   if (corr_tiles(A.n) > GT * (BD / kWave) || conv_tiles(A.n) > kMaxConvTiles)
     return SIDON_ERR_UNSUPPORTED;
-  const long long lds = fw_lds_bytes<BD>(A.n);
+  const long long lds = fw_lds_bytes<BD, SS>(A.n);
   if (lds > ds->lds_max) {
     set_last_error_msg("frank_wolfe: n needs more LDS than a workgroup may use");
     return SIDON_ERR_LDS;
   }
-  auto kfn = k_frank_wolfe<BD, GT>;
+  auto kfn = k_frank_wolfe<BD, GT, SS>;
   SIDON_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kfn),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(kfn, dim3(P), dim3(BD), (size_t)lds, stream, A);
@@ -265,17 +265,36 @@ int launch_fw(const FwArgs& A, int P, hipStream_t stream) {
   return SIDON_OK;
 }
 
+// The wide (bank-conflict-free) segment layout where both padded vectors fit
+// the LDS with it (n up to about 7900), else the narrow one.
+template <int BD, int GT>
+int launch_fw(const FwArgs& A, int P, hipStream_t stream) {
+  int rc = SIDON_OK;
+  DeviceState* ds = device_state(&rc);
+  if (!ds) return rc;
+  if (fw_lds_bytes<BD, kSegStrideWide>(A.n) <= ds->lds_max)
+    return launch_fw_layout<BD, GT, kSegStrideWide>(A, P, stream);
+  return launch_fw_layout<BD, GT, kSegStride>(A, P, stream);
+}
+
 }  // namespace
 }  // namespace sidon
 
 using namespace sidon;
 
+// the layout a 160 KiB workgroup takes
+template <int BD>
+long long fw_lds_choice(int n) {
+  const long long wide = fw_lds_bytes<BD, kSegStrideWide>(n);
+  return wide <= 160LL * 1024 ? wide : fw_lds_bytes<BD>(n);
+}
+
 extern "C" long long sidon_fw_lds_bytes_impl(int n, int block) {
   if (n < 1) return -1;
   switch (block) {
-    case 256: return fw_lds_bytes<256>(n);
-    case 512: return fw_lds_bytes<512>(n);
-    case 1024: return fw_lds_bytes<1024>(n);
+    case 256: return fw_lds_choice<256>(n);
+    case 512: return fw_lds_choice<512>(n);
+    case 1024: return fw_lds_choice<1024>(n);
     default: return -1;
   }
 }

@@ -50,18 +50,18 @@ struct LsArgs {
 };
 
 // Bytes of dynamic LDS for one candidate of length n.
-template <typename T>
+template <typename T, int SS = kSegStride>
 __host__ __device__ inline long long ls_lds_bytes(int n, int block) {
   // + reduction slots (2 per wave), the running max of b, three counters
   // (work items of each product, matrix steps run), the work items of both
   // products and the weight segments' bits (32-bit each; T is at least as
   // wide)
-  const long long elems = (long long)pad_len(n) + cbuf_len(n) + gbuf_len(n) + 2 * (block / kWave) +
+  const long long elems = (long long)pad_len<SS>(n) + cbuf_len(n) + gbuf_len(n) + 2 * (block / kWave) +
                           1 + 3 + 2 * conv_tiles(n) + 2 * corr_tiles(n) + segmask_words(n);
   return elems * (long long)sizeof(T);
 }
 
-template <typename T, int BD, int QMAX>
+template <typename T, int BD, int QMAX, int SS>
 __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) {
   constexpr int NW = BD / kWave;
   // operand steps in flight (fp64 operands take twice the registers)
@@ -73,7 +73,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
   T* __restrict__ a = A.pop + (size_t)cand * (size_t)A.stride;
 
   T* sA = reinterpret_cast<T*>(smem);
-  T* sC = sA + pad_len(n);
+  T* sC = sA + pad_len<SS>(n);
   T* sG = sC + cbuf_len(n);
   T* slots = sG + gbuf_len(n);  // 2 * NW
   T* bmax = slots + 2 * NW;     // max_k b_k of the step (raised by the autoconvolution)
@@ -83,7 +83,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
   unsigned* gitems = citems + 2 * conv_tiles(n);
   unsigned* segmask = gitems + 2 * corr_tiles(n);
   const int tid = threadIdx.x;
-  const int LA = pad_len(n), LC = cbuf_len(n), LG = gbuf_len(n);
+  const int LA = pad_len<SS>(n), LC = cbuf_len(n), LG = gbuf_len(n);
   const int SW = segmask_words(n);
 
   if (tid == 0) { ctr[0] = 0; ctr[1] = 0; *nsteps = 0u; *bmax = T(0); }
@@ -94,7 +94,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
   for (int i = tid; i < LG; i += BD) sG[i] = T(0);
   for (int w = tid; w < SW; w += BD) segmask[w] = 0u;
   __syncthreads();
-  for (int i = tid; i < n; i += BD) pad_put(sA, i, a[i]);
+  for (int i = tid; i < n; i += BD) pad_put<SS>(sA, i, a[i]);
 
   T m[QMAX], v[QMAX];
 #pragma unroll
@@ -113,7 +113,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     const T pexp = A.pStart * dexp(lnPr * frac);
     const T lr = A.lrStart * dexp(lnLr * frac);
 
-    autoconv_mfma<T, D>(sA, sC, n, ctr, citems, nsteps, bmax);  // sC[k] = b_k, *bmax = M
+    autoconv_mfma<T, D, SS>(sA, sC, n, ctr, citems, nsteps, bmax);  // sC[k] = b_k, *bmax = M
     __syncthreads();
 
     if (tid == 0) ctr[0] = 0;  // every wave is past the autoconvolution
@@ -149,7 +149,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     b2t *= beta2;
     const T bc1 = T(1) / (T(1) - b1t), bc2 = T(1) / (T(1) - b2t);
 
-    corr_mfma<T, D>(sC, sA, sG, n, ctr + 1, gitems, segmask, nsteps);
+    corr_mfma<T, D, SS>(sC, sA, sG, n, ctr + 1, gitems, segmask, nsteps);
     const int ut = opaque(tid);  // element addresses below are built per step, not hoisted
     __syncthreads();
 
@@ -163,7 +163,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     for (int q = 0; q < QMAX; ++q) {
       const int i = ut + BD * q;
       if (i < n) {
-        const T ai = sA[pad_addr(i)];
+        const T ai = sA[pad_addr<SS>(i)];
         const T gi = dquot(sG[i], R) - M;
         T xx;
         if (mode == 0) {
@@ -191,14 +191,14 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) {
       const int i = ut + BD * q;
-      if (i < n) pad_put(sA, i, reset ? T(1) / nT : sG[i] * inv);
+      if (i < n) pad_put<SS>(sA, i, reset ? T(1) / nT : sG[i] * inv);
     }
     // sC needs no clearing: the autoconvolution stores every tile whole
     for (int i = tid; i < LG; i += BD) sG[i] = T(0);
     __syncthreads();
   }
 
-  for (int i = tid; i < n; i += BD) a[i] = sA[pad_addr(i)];
+  for (int i = tid; i < n; i += BD) a[i] = sA[pad_addr<SS>(i)];
   __syncthreads();  // every wave has added its matrix steps
   if (tid == 0) atomicAdd(&g_block_steps, (unsigned long long)*nsteps);
   if (A.mom) {
@@ -216,11 +216,11 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
 
 // One candidate per workgroup (grid = candidates), or workgroups pulling
 // candidates from a device queue (persistent).
-template <typename T, int BD, int QMAX>
+template <typename T, int BD, int QMAX, int SS>
 __device__ __forceinline__ void ls_body(const LsArgs<T>& A, unsigned char* smem, int& s_item) {
   if (!A.queue) {
     const int cand = A.order ? A.order[blockIdx.x] : (int)blockIdx.x;
-    ls_candidate<T, BD, QMAX>(A, cand, smem);
+    ls_candidate<T, BD, QMAX, SS>(A, cand, smem);
     return;
   }
   for (;;) {
@@ -230,26 +230,26 @@ __device__ __forceinline__ void ls_body(const LsArgs<T>& A, unsigned char* smem,
     __syncthreads();
     if (item >= A.P) break;
     const int cand = A.order ? A.order[item] : item;
-    ls_candidate<T, BD, QMAX>(A, cand, smem);
+    ls_candidate<T, BD, QMAX, SS>(A, cand, smem);
   }
 }
 
-template <typename T, int BD, int QMAX>
+template <typename T, int BD, int QMAX, int SS>
 __global__ __launch_bounds__(BD) void k_local_search(LsArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_item;
-  ls_body<T, BD, QMAX>(A, smem, s_item);
+  ls_body<T, BD, QMAX, SS>(A, smem, s_item);
 }
 
 // The same kernel held to 128 registers a lane (four waves a SIMD): two
 // 512-thread candidates then share a CU where their LDS allows it (n <= 4096),
 // and one candidate's elementwise phases overlap the other's matrix work.
-template <typename T, int BD, int QMAX>
+template <typename T, int BD, int QMAX, int SS>
 __global__ __launch_bounds__(BD) __attribute__((amdgpu_waves_per_eu(4, 4))) void
 k_local_search_pair(LsArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_item;
-  ls_body<T, BD, QMAX>(A, smem, s_item);
+  ls_body<T, BD, QMAX, SS>(A, smem, s_item);
 }
 
 // ---------------------------------------------------------------- lp_grad --
@@ -262,7 +262,7 @@ struct GradArgs {
   T* grad;
 };
 
-template <typename T, int BD>
+template <typename T, int BD, int SS>
 __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   constexpr int NW = BD / kWave;
   constexpr int D = (BD >= 1024 || sizeof(T) == 8) ? 2 : 4;
@@ -270,7 +270,7 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   const int n = A.n, cand = blockIdx.x;
   const T* a = A.a + (size_t)cand * n;
   T* sA = reinterpret_cast<T*>(smem);
-  T* sC = sA + pad_len(n);
+  T* sC = sA + pad_len<SS>(n);
   T* sG = sC + cbuf_len(n);
   T* slots = sG + gbuf_len(n);
   T* bmax = slots + 2 * NW;
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   unsigned* gitems = citems + 2 * conv_tiles(n);
   unsigned* segmask = gitems + 2 * corr_tiles(n);
   const int tid = threadIdx.x;
-  const int LA = pad_len(n), LC = cbuf_len(n), LG = gbuf_len(n);
+  const int LA = pad_len<SS>(n), LC = cbuf_len(n), LG = gbuf_len(n);
   if (tid == 0) { ctr[0] = 0; ctr[1] = 0; *bmax = T(0); }
   for (int w = tid; w < segmask_words(n); w += BD) segmask[w] = 0u;
   autoconv_items(n, citems, reinterpret_cast<int*>(sC), tid, BD);
@@ -289,10 +289,10 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   for (int i = tid; i < LG; i += BD) sG[i] = T(0);
   __syncthreads();
   T sl = T(0);
-  for (int i = tid; i < n; i += BD) { const T x = a[i]; pad_put(sA, i, x); sl += x; }
+  for (int i = tid; i < n; i += BD) { const T x = a[i]; pad_put<SS>(sA, i, x); sl += x; }
   const T S = block_sum<T, NW>(sl, slots);  // barrier also publishes sA
   const int nb = 2 * n - 1;
-  autoconv_mfma<T, D>(sA, sC, n, ctr, citems, nullptr, bmax);  // sC[k] = b_k, *bmax = M
+  autoconv_mfma<T, D, SS>(sA, sC, n, ctr, citems, nullptr, bmax);  // sC[k] = b_k, *bmax = M
   __syncthreads();
   const T M = *bmax;
   const T pm1 = A.p - T(1);
@@ -308,7 +308,7 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   const T Rp = dexp(log(R) / A.p);
   if (A.J && tid == 0) A.J[cand] = T(2) * T(n) * M * Rp / (S * S);
   if (!A.grad) return;
-  corr_mfma<T, D>(sC, sA, sG, n, ctr + 1, gitems, segmask);
+  corr_mfma<T, D, SS>(sC, sA, sG, n, ctr + 1, gitems, segmask);
   __syncthreads();
   const T scale = T(4) * T(n) * Rp / (S * S);
   const T MS = M / S;
@@ -335,20 +335,20 @@ int pick_block(int nmax) {
   return default_block(nmax);
 }
 
-template <typename T, int BD, int QMAX, bool PAIR = false>
-int launch_ls(const LsArgs<T>& A, int nmax, hipStream_t stream) {
+template <typename T, int BD, int QMAX, bool PAIR, int SS>
+int launch_ls_layout(const LsArgs<T>& A, int nmax, hipStream_t stream) {
   if (nmax > BD * QMAX) return SIDON_ERR_UNSUPPORTED;
   int rc = SIDON_OK;
   DeviceState* ds = device_state(&rc);
   if (!ds) return rc;
-  const long long lds = ls_lds_bytes<T>(nmax, BD);
+  const long long lds = ls_lds_bytes<T, SS>(nmax, BD);
   if (lds > ds->lds_max) {
     set_last_error_msg("local search: n needs more LDS than a workgroup may use");
     return SIDON_ERR_LDS;
   }
   auto kfn = [] {
-    if constexpr (PAIR) return k_local_search_pair<T, BD, QMAX>;
-    else return k_local_search<T, BD, QMAX>;
+    if constexpr (PAIR) return k_local_search_pair<T, BD, QMAX, SS>;
+    else return k_local_search<T, BD, QMAX, SS>;
   }();
   SIDON_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kfn),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -366,6 +366,19 @@ int launch_ls(const LsArgs<T>& A, int nmax, hipStream_t stream) {
   hipLaunchKernelGGL(kfn, dim3(grid), dim3(BD), (size_t)lds, stream, A);
   SIDON_HIP_TRY(hipGetLastError());
   return SIDON_OK;
+}
+
+// The wide (bank-conflict-free) segment layout where it fits the LDS, else
+// the narrow one.
+template <typename T, int BD, int QMAX, bool PAIR = false>
+int launch_ls(const LsArgs<T>& A, int nmax, hipStream_t stream) {
+  int rc = SIDON_OK;
+  DeviceState* ds = device_state(&rc);
+  if (!ds) return rc;
+  if constexpr (sizeof(T) == 4)
+    if (ls_lds_bytes<T, kSegStrideWide>(nmax, BD) <= ds->lds_max)
+      return launch_ls_layout<T, BD, QMAX, PAIR, kSegStrideWide>(A, nmax, stream);
+  return launch_ls_layout<T, BD, QMAX, PAIR, kSegStride>(A, nmax, stream);
 }
 
 // `queue` non-null: persistent launch pulling candidates from that counter.
@@ -398,22 +411,33 @@ int check_schedule(int iters, T pS, T pE, T lS, T lE) {
   return SIDON_OK;
 }
 
-template <typename T, int BD>
-int launch_grad(const GradArgs<T>& A, hipStream_t stream) {
+template <typename T, int BD, int SS>
+int launch_grad_layout(const GradArgs<T>& A, hipStream_t stream) {
   int rc = SIDON_OK;
   DeviceState* ds = device_state(&rc);
   if (!ds) return rc;
-  const long long lds = ls_lds_bytes<T>(A.n, BD);
+  const long long lds = ls_lds_bytes<T, SS>(A.n, BD);
   if (lds > ds->lds_max) {
     set_last_error_msg("lp_grad: n needs more LDS than a workgroup may use");
     return SIDON_ERR_LDS;
   }
-  auto kfn = k_lp_grad<T, BD>;
+  auto kfn = k_lp_grad<T, BD, SS>;
   SIDON_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kfn),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(kfn, dim3(A.B), dim3(BD), (size_t)lds, stream, A);
   SIDON_HIP_TRY(hipGetLastError());
   return SIDON_OK;
+}
+
+template <typename T, int BD>
+int launch_grad(const GradArgs<T>& A, hipStream_t stream) {
+  int rc = SIDON_OK;
+  DeviceState* ds = device_state(&rc);
+  if (!ds) return rc;
+  if constexpr (sizeof(T) == 4)
+    if (ls_lds_bytes<T, kSegStrideWide>(A.n, BD) <= ds->lds_max)
+      return launch_grad_layout<T, BD, kSegStrideWide>(A, stream);
+  return launch_grad_layout<T, BD, kSegStride>(A, stream);
 }
 
 }  // namespace
@@ -494,9 +518,11 @@ extern "C" int sidon_lp_grad_f64(const double* a, int n, int B, double p, double
 extern "C" long long sidon_ls_lds_bytes_impl(int kernel, int n, int block) {
   if (n < 1) return -1;
   switch (kernel) {
-    case 0: return ls_lds_bytes<float>(n, block);
+    case 0: case 4: {  // the layout a 160 KiB workgroup takes: wide where it fits
+      const long long wide = ls_lds_bytes<float, kSegStrideWide>(n, block);
+      return wide <= 160LL * 1024 ? wide : ls_lds_bytes<float>(n, block);
+    }
     case 1: return ls_lds_bytes<double>(n, block);
-    case 4: return ls_lds_bytes<float>(n, block);
     default: return -1;
   }
 }

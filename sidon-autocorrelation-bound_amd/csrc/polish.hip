@@ -160,15 +160,27 @@ constexpr int kWinShift = 15 * kSegStride;
 __host__ __device__ constexpr int win_len(int len) { return (len + 2) * kSegStride; }
 __device__ __forceinline__ int wpos(int x) { return pad_addr(x) - kWinShift; }
 
-// Matrix B-operand windows (a lane reads four heights inside one segment, so
-// no copies are needed): segment x/16 of the window at 18 (x/16) doubles.
-// Sixteen lanes one segment apart then cover all 64 LDS banks once per
-// 16-byte read (a 20-double stride would put lanes t and t + 8 on the same
-// banks).  LEN segments, then one zero segment (the masked reads of the
+// Matrix B-operand windows (a lane reads two pairs of heights inside one
+// segment, so no copies are needed): segment x/16 of the window at 20 (x/16)
+// doubles.  The 16 reduction indices of a block step are dealt to lane group
+// kk and MFMA q as s = 2 kk + 8 (q / 2) + q % 2 (see polish_conv_tile), so a
+// lane's B operand is two 16-byte reads at doubles 2 kk and 2 kk + 8 of its
+// segment.  gfx950 serves a 16-byte LDS read 16 lanes a pass over 64
+// four-byte banks; by the device's bank-conflict counters on probe patterns,
+// this order with a 20-double stride keeps each pass on distinct banks, where
+// four contiguous doubles per lane at an 18- or 20-double stride cost a
+// second pass.  LEN segments, then one zero segment (the masked reads of the
 // symmetric tile land there).
-constexpr int kBStride = 18;
+constexpr int kBStride = 20;
 __host__ __device__ constexpr int bwin_len(int len) { return (len + 1) * kBStride; }
 __device__ __forceinline__ int bpos(int x) { return (x >> 4) * kBStride + (x & 15); }
+
+// Two pairs of doubles: o = p[0], p[1], p[8], p[9] (two 16-byte LDS reads).
+__device__ __forceinline__ void load_pairs(const double* p, double (&o)[4]) {
+  const double2 v0 = *reinterpret_cast<const double2*>(p);
+  const double2 v1 = *reinterpret_cast<const double2*>(p + 8);
+  o[0] = v0.x; o[1] = v0.y; o[2] = v1.x; o[3] = v1.y;
+}
 
 template <int NT, int LEN>
 __device__ __forceinline__ void fill_bwindow(double* win, const double* __restrict__ xs, int n,
@@ -250,20 +262,25 @@ __global__ void k_polish_setup(const double* __restrict__ pop, int n, Ws w) {
 // [da, db), from windows: wa starts at segment ga, wb at segment gb, and wb's
 // zero segment follows its LENB segments.  Column t takes block d only while
 // d < ceil((K0 + t) / 2); past that its B reads come from the zero segment.
+// Reduction index s of lane group kk and MFMA q: s = 2 kk + 8 (q / 2) + q % 2
+// (pairs of adjacent heights; see the B-window layout above).
 template <int D, int LENB>
 __device__ __forceinline__ f64x4 polish_conv_tile(const double* wa, int ga, const double* wb,
                                                   int gb, int K0, int da, int db) {
   constexpr int S = kSegStride;
   const int lane = opaque((int)threadIdx.x) & (kWave - 1);
   const int t = lane & 15, kk = lane >> 4;
-  const int pa = wpos(16 * (da - ga) + t - 4 * kk);         // a_{16d + t - s}
-  const int pb = bpos(16 * (K0 + t - da - gb)) + 4 * kk;    // a_{16(K0 + t - d) + s}
+  const int pa0 = wpos(16 * (da - ga) + t - 2 * kk);        // a_{16d + t - s}, s = 2kk (+1)
+  const int pa1 = wpos(16 * (da - ga) + t - 2 * kk - 8);    // s = 2kk + 8 (+1)
+  const int pb = bpos(16 * (K0 + t - da - gb)) + 2 * kk;    // a_{16(K0 + t - d) + s}
   const int live = ((K0 + t + 1) >> 1) - da;
   constexpr int zero = LENB * kBStride;
   return run_steps<double, D>(db - da, [&](int i, double (&x)[4], double (&y)[4]) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) x[q] = wa[pa + i * S - q];
-    load4(wb + (i < live ? pb - i * kBStride : zero), y);
+    x[0] = wa[pa0 + i * S];
+    x[1] = wa[pa0 + i * S - 1];
+    x[2] = wa[pa1 + i * S];
+    x[3] = wa[pa1 + i * S - 1];
+    load_pairs(wb + (i < live ? pb - i * kBStride : zero), y);
   });
 }
 
@@ -395,12 +412,14 @@ __device__ __forceinline__ f64x4 polish_corr_tile(const double* wc, int cb, cons
                                                   int ga, int I0, int ea, int eb) {
   const int lane = opaque((int)threadIdx.x) & (kWave - 1);
   const int t = lane & 15, kk = lane >> 4;
-  const int pc = 16 * ea + t + 4 * kk - cb;                 // c_{16e + t + s}
-  const int pb = bpos(16 * (ea - I0 - t - ga)) + 4 * kk;    // a_{16(e - I0 - t) + s}
+  const int pc = 16 * ea + t + 2 * kk - cb;                 // c_{16e + t + s}
+  const int pb = bpos(16 * (ea - I0 - t - ga)) + 2 * kk;    // a_{16(e - I0 - t) + s}
   return run_steps<double, D>(eb - ea, [&](int i, double (&x)[4], double (&y)[4]) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) x[q] = wc[pc + 16 * i + q];
-    load4(wa + pb + i * kBStride, y);
+    x[0] = wc[pc + 16 * i];
+    x[1] = wc[pc + 16 * i + 1];
+    x[2] = wc[pc + 16 * i + 8];
+    x[3] = wc[pc + 16 * i + 9];
+    load_pairs(wa + pb + i * kBStride, y);
   });
 }
 
