@@ -131,16 +131,27 @@ int sidon_polish_f64_noalloc(double* pop, double* mom, double* vel, int n, int K
 /* Uneven lengths in one persistent launch: candidate p occupies
  * pop[p*stride .. p*stride + ns[p]) (ns on the device, ns[p] <= nmax <= stride);
  * `order` (may be NULL) lists the processing order, longest first gives the
- * best balance.  Workgroups pull candidates from the work-queue counter
- * `queue`: one caller-owned device int, set to zero on `stream` before the
- * launch (no memset node).  Launches that may overlap need distinct queues.
- * As many workgroups as fit the device at once (occupancy x CUs, at most P).
- * No reference counterpart (the reference runs one n per launch). */
+ * best balance.  As many workgroups as fit the device at once (occupancy x
+ * CUs, at most P) pull tickets from a work queue.  With split = 1 and an
+ * order, the longest candidates (n >= 4096) may each be shared by two
+ * workgroups that exchange b and the new heights through global memory
+ * twice a step: as many as a makespan model of the launch says shortens it
+ * (none when every workgroup has a candidate of the longest length to run);
+ * the result is bit-identical to one workgroup's.  `workspace` (caller-owned, device,
+ * sidon_local_search_ragged_workspace(P, stride) bytes) holds the queue
+ * counter, the pairs' progress flags and exchange buffers; the launch clears
+ * what must start at zero with a kernel on `stream` (no memset node).
+ * Launches that may overlap need distinct workspaces.  After the launch the
+ * int at byte 4 of the workspace counts pair waits that gave up (a partner
+ * workgroup that never ran within 20 s; 0 in any normal run) and the int at
+ * byte 8 holds the number of candidates split.  No reference counterpart
+ * (the reference runs one n per launch). */
+size_t sidon_local_search_ragged_workspace(int P, int stride);
 int sidon_local_search_ragged(float* pop, int stride, const int* ns, int P, int nmax,
                               const int* order, const int* skip, const int* mode, int iters,
                               float pStart, float pEnd, float lrStart, float lrEnd,
-                              float beta1, float beta2, int modeDefault, int* queue,
-                              hipStream_t stream);
+                              float beta1, float beta2, int modeDefault, int split,
+                              void* workspace, size_t workspace_bytes, hipStream_t stream);
 /* Choose the workgroup size the fp32 local search uses at length n
  * (256, 512 or 1024; 0 restores the built-in rule).  Set by the autotuner. */
 int sidon_set_ls_block(int n, int block);
@@ -235,9 +246,11 @@ int sidon_order_by_length(const int* ns, int P, int* order, hipStream_t stream);
 /* Device buffers (caller-owned) and parameters of a generation of the island
  * search.  Populations are [P x stride] fp32 with per-slot lengths ns (<= nmax);
  * a generation reads population A (ns_a) and writes population B (ns_b), the
- * next one the other way round.  gen points to a device uint64 counter, queue
- * to the device int the local search's work queue counts in (one per
- * generation object that may run concurrently with another). */
+ * next one the other way round.  gen points to a device uint64 counter;
+ * ls_workspace is the local search's workspace
+ * (sidon_local_search_ragged_workspace(P, stride) bytes; one per generation
+ * object that may run concurrently with another); split as in
+ * sidon_local_search_ragged. */
 typedef struct {
   float* pop_a; float* pop_b;
   int* ns_a; int* ns_b;
@@ -245,8 +258,9 @@ typedef struct {
   int* p1; int* p2; int* op; float* strength;
   int* order;
   unsigned long long* gen;
-  int* queue;
-  int P, stride, nmax, islands, elite, migrate_every, iters;
+  void* ls_workspace;
+  unsigned long long ls_workspace_bytes;
+  int P, stride, nmax, islands, elite, migrate_every, iters, split;
   float pStart, pEnd, lrStart, lrEnd, sigma;
   double activeTol;
   unsigned long long seed;

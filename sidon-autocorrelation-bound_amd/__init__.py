@@ -31,7 +31,8 @@ EXPORTS = (
     "sidon_init", "sidon_last_error", "sidon_status_string", "sidon_lds_bytes",
     "sidon_lds_limit", "sidon_eval_f64", "sidon_eval_f64_ragged", "sidon_eval_f64_d",
     "sidon_eval_exact", "sidon_local_search", "sidon_local_search_f64",
-    "sidon_local_search_ragged", "sidon_polish_f64_workspace", "sidon_polish_f64",
+    "sidon_local_search_ragged", "sidon_local_search_ragged_workspace",
+    "sidon_polish_f64_workspace", "sidon_polish_f64",
     "sidon_polish_f64_noalloc",
     "sidon_set_ls_block", "sidon_get_ls_block", "sidon_lp_grad",
     "sidon_lp_grad_f64", "sidon_frank_wolfe", "sidon_refine", "sidon_variation",
@@ -50,8 +51,9 @@ class GenerationDesc(C.Structure):
     """sidon_generation_desc (include/sidon.h)."""
     _fields_ = [(name, C.c_void_p) for name in (
         "pop_a", "pop_b", "ns_a", "ns_b", "fit", "nact", "p1", "p2", "op", "strength", "order",
-        "gen", "queue")] + [(name, C.c_int) for name in (
-            "P", "stride", "nmax", "islands", "elite", "migrate_every", "iters")] + [
+        "gen", "ls_workspace")] + [("ls_workspace_bytes", C.c_uint64)] + [
+            (name, C.c_int) for name in (
+            "P", "stride", "nmax", "islands", "elite", "migrate_every", "iters", "split")] + [
         (name, C.c_float) for name in ("pStart", "pEnd", "lrStart", "lrEnd", "sigma")] + [
         ("activeTol", C.c_double), ("seed", C.c_uint64)]
 
@@ -95,7 +97,8 @@ def _declare(L):
         "sidon_local_search_f64": ([_vp, _vp, _vp, _vp, _vp, _i, _i, _i] + [_d] * 6 + [_i, _vp],
                                    _i),
         "sidon_local_search_ragged": ([_vp, _i, _vp, _i, _i, _vp, _vp, _vp, _i] + [_f] * 6 +
-                                      [_i, _vp, _vp], _i),
+                                      [_i, _i, _vp, C.c_size_t, _vp], _i),
+        "sidon_local_search_ragged_workspace": ([_i, _i], C.c_size_t),
         "sidon_set_ls_block": ([_i, _i], _i),
         "sidon_get_ls_block": ([_i], _i),
         "sidon_lp_grad": ([_vp, _i, _i, _f, _vp, _vp, _vp], _i),
@@ -263,20 +266,38 @@ def local_search_ragged(pop: torch.Tensor, ns: torch.Tensor, nmax: int, iters: i
                         p_start: float, p_end: float, lr_start: float, lr_end: float,
                         beta1: float = 0.9, beta2: float = 0.999, mode: int = 0,
                         order: torch.Tensor | None = None, skip: torch.Tensor | None = None,
-                        mode_arr: torch.Tensor | None = None, queue: torch.Tensor | None = None,
-                        stream=None) -> None:
-    """Persistent uneven-length local search.  `queue` is the int32 device
-    counter the workgroups pull candidates from (a fresh one when not given;
-    launches that may overlap need distinct counters)."""
+                        mode_arr: torch.Tensor | None = None, split: int = 1,
+                        workspace: torch.Tensor | None = None, stream=None) -> None:
+    """Persistent uneven-length local search.  `workspace` (uint8 device
+    tensor of ragged_workspace_bytes(P, stride)) holds the work queue and the
+    buffers of candidates split over two workgroups (split = 1); a fresh one
+    when not given.  Launches that may overlap need distinct workspaces."""
     _need_gpu()
     P, stride = pop.shape
-    if queue is None:
-        queue = torch.zeros(1, dtype=torch.int32, device=pop.device)
+    if workspace is None:
+        workspace = torch.empty(ragged_workspace_bytes(P, stride), dtype=torch.uint8,
+                                device=pop.device)
     _check(lib().sidon_local_search_ragged(_ptr(pop), stride, _ptr(ns), P, nmax, _ptr(order),
                                            _ptr(skip), _ptr(mode_arr), iters, p_start, p_end,
-                                           lr_start, lr_end, beta1, beta2, mode, _ptr(queue),
-                                           _stream(stream)),
+                                           lr_start, lr_end, beta1, beta2, mode, split,
+                                           _ptr(workspace), workspace.numel(), _stream(stream)),
            "sidon_local_search_ragged")
+
+
+def ragged_workspace_bytes(P: int, stride: int) -> int:
+    return lib().sidon_local_search_ragged_workspace(P, stride)
+
+
+def pair_timeouts(workspace: torch.Tensor) -> int:
+    """Pair waits of the last ragged launch on `workspace` that gave up (0 in
+    a normal run): the int at byte 4."""
+    return int(workspace[4:8].cpu().view(torch.int32).item())
+
+
+def split_count(workspace: torch.Tensor) -> int:
+    """Candidates the last ragged launch on `workspace` split over two
+    workgroups: the int at byte 8."""
+    return int(workspace[8:12].cpu().view(torch.int32).item())
 
 
 def set_ls_block(n: int, block: int) -> None:
@@ -531,7 +552,7 @@ class Generation:
                  p_start: float = 4.0, p_end: float = 1e4, lr_start: float = 0.085,
                  lr_end: float = 0.0015, sigma: float = 0.05, elite: int = 2,
                  migrate_every: int = 12, seed: int = 20260905, active_tol: float = 1e-4,
-                 graph: bool = True, stream=None):
+                 graph: bool = True, split: int = 1, stream=None):
         _need_gpu()
         P, stride = pop.shape
         dev = pop.device
@@ -548,14 +569,16 @@ class Generation:
         self.strength = torch.zeros(P, dtype=torch.float32, device=dev)
         self.order = torch.zeros(P, **i32)
         self.gen = torch.zeros(1, dtype=torch.int64, device=dev)
-        self.queue = torch.zeros(1, **i32)
+        self.ls_workspace = torch.empty(ragged_workspace_bytes(P, stride), dtype=torch.uint8,
+                                        device=dev)
         nmax = int(self.ns_a.max().item())
         d = GenerationDesc()
         for name in ("pop_a", "pop_b", "ns_a", "ns_b", "fit", "nact", "p1", "p2", "op",
-                     "strength", "order", "gen", "queue"):
+                     "strength", "order", "gen", "ls_workspace"):
             setattr(d, name, _ptr(getattr(self, name)))
+        d.ls_workspace_bytes = self.ls_workspace.numel()
         d.P, d.stride, d.nmax, d.islands, d.elite = P, stride, nmax, islands, elite
-        d.migrate_every, d.iters = migrate_every, iters
+        d.migrate_every, d.iters, d.split = migrate_every, iters, split
         d.pStart, d.pEnd, d.lrStart, d.lrEnd, d.sigma = p_start, p_end, lr_start, lr_end, sigma
         d.activeTol, d.seed = active_tol, seed
         self.desc = d

@@ -669,10 +669,19 @@ __device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
 // *bmax to the tile's largest b_k, so the finishing work overlaps the other
 // waves' matrix work.  Every wave of the workgroup calls this; returns
 // without a barrier.
+// With `half` >= 0 the workgroup is one of a pair sharing the candidate: it
+// computes only the items it owns (conv_item_owner) and passes the others.
+__device__ __forceinline__ int conv_item_owner(int rank) {
+  // items are ranked by decreasing cost: dealt 0 1 1 0 0 1 1 0 ... the two
+  // halves' totals differ by at most the largest tile
+  return ((rank & 3) == 1 || (rank & 3) == 2) ? 1 : 0;
+}
+__device__ __forceinline__ int corr_tile_owner(int j) { return j & 1; }
+
 template <typename T, int D = Pipe<T>::kDepth, int SS = kSegStride>
 __device__ void autoconv_mfma(const T* __restrict__ sA, T* out, int n, int* ctr,
                               const unsigned* items, unsigned* steps = nullptr,
-                              T* bmax = nullptr) {
+                              T* bmax = nullptr, int half = -1) {
   const int N16 = nseg16(n);
   const int NT = conv_tiles(n);
   const int t = opaque((int)threadIdx.x) & 15;
@@ -682,8 +691,9 @@ __device__ void autoconv_mfma(const T* __restrict__ sA, T* out, int n, int* ctr,
     const int next = take_ticket(ctr);
     int j, u0, u1;
     unpack_item(__builtin_amdgcn_readfirstlane(items[it]), j, u0, u1);  // wave-uniform
-    done += u1 - u0;
-    if (it < NT || u0 < u1) {  // heads always store, so every tile is written
+    const bool mine = half < 0 || conv_item_owner(it) == half;
+    if (mine) done += u1 - u0;
+    if (mine && (it < NT || u0 < u1)) {  // heads always store, so every tile is written
       int dlo, dhi;
       autoconv_drange(j, N16, dlo, dhi);
       const int cost = dhi >= dlo ? dhi + 1 - dlo : 0;
@@ -732,7 +742,7 @@ __device__ __forceinline__ void corr_items(int n, unsigned* items, int* scratch,
 template <typename T, int D = Pipe<T>::kDepth, int SS = kSegStride>
 __device__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* out, int n,
                           int* ctr, const unsigned* items, const unsigned* segmask,
-                          unsigned* steps = nullptr) {
+                          unsigned* steps = nullptr, int half = -1) {
   const int N16 = nseg16(n);
   const int NT = corr_tiles(n);
   const int cost = corr_cost(N16);
@@ -743,7 +753,7 @@ __device__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* 
     const int next = take_ticket(ctr);
     int j, u0, u1;
     unpack_item(__builtin_amdgcn_readfirstlane(items[it]), j, u0, u1);  // wave-uniform
-    if (u0 < u1) {
+    if (u0 < u1 && (half < 0 || corr_tile_owner(j) == half)) {
       const int I0 = 16 * j;
       typename Acc4<T>::type acc;
       if (segmask) {

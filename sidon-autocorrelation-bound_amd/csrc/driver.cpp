@@ -246,6 +246,7 @@ struct Args {
   float sigma = 0.05f, t_max = 0.30f;
   std::vector<int> lengths{600, 1200, 2100, 4096, 8400};
   bool tune = true;
+  int split = 1;  // --no-split: every candidate on one workgroup
 };
 
 // Parses "64,128,..." into positive integers; false on anything else.
@@ -407,7 +408,9 @@ int uneven(const Args& A) {
   d.strength = dalloc<float>(P);
   d.order = dalloc<int>(P);
   d.gen = dalloc<unsigned long long>(1);
-  d.queue = dalloc<int>(1);
+  d.ls_workspace_bytes = sidon_local_search_ragged_workspace(P, stride);
+  d.ls_workspace = dalloc<unsigned char>(d.ls_workspace_bytes);
+  d.split = A.split;
   d.P = P; d.stride = stride; d.nmax = nmax; d.islands = A.islands; d.elite = A.elite;
   d.migrate_every = 12; d.iters = A.iters;
   d.pStart = 400.0f; d.pEnd = 6e4f; d.lrStart = 0.012f; d.lrEnd = 0.0003f;  // warm tier, main.cu:629
@@ -439,13 +442,18 @@ int uneven(const Args& A) {
   std::vector<double> fit(P);
   CHECK_HIP(hipMemcpy(fit.data(), d.fit, P * sizeof(double), hipMemcpyDeviceToHost));
   const double best = *std::min_element(fit.begin(), fit.end());
+  int timeouts = 0;  // pair waits that gave up (byte 4 of the local-search workspace)
+  CHECK_HIP(hipMemcpy(&timeouts, static_cast<char*>(d.ls_workspace) + 4, sizeof timeouts,
+                      hipMemcpyDeviceToHost));
   std::printf("{\"workload\": \"uneven islands: %d candidates, n in {600,1200,2100,4096,8400} "
               "(initial draw), %d islands, %d Adam steps + mutation sigma=%.2f + tournament per "
               "generation, HIP graph\", \"generations\": %d, \"ms_per_generation\": %.3f, "
               "\"candidate_steps_per_s\": %.1f, \"local_search_executed_tflops\": %.2f, "
+              "\"executed_frac_fp32_peak\": %.4f, \"split\": %d, \"pair_timeouts\": %d, "
               "\"best_F\": %.12f}\n",
               P, A.islands, A.iters, A.sigma, A.gens, ms / A.gens,
-              (double)steps * A.gens / (ms / 1e3), flop * A.gens / (ms / 1e3) / 1e12, best);
+              (double)steps * A.gens / (ms / 1e3), flop * A.gens / (ms / 1e3) / 1e12,
+              flop * A.gens / (ms / 1e3) / 1e12 / 157.3, A.split, timeouts, best);
   CHECK_SIDON(sidon_generation_destroy(g));
   return 0;
 }
@@ -849,7 +857,9 @@ int search_graph(const Args& A) {
   d.p1 = dalloc<int>(P); d.p2 = dalloc<int>(P); d.op = dalloc<int>(P);
   d.strength = dalloc<float>(P); d.order = dalloc<int>(P);
   d.gen = dalloc<unsigned long long>(1);
-  d.queue = dalloc<int>(1);
+  d.ls_workspace_bytes = sidon_local_search_ragged_workspace(P, nmax);
+  d.ls_workspace = dalloc<unsigned char>(d.ls_workspace_bytes);
+  d.split = A.split;
   d.P = P; d.stride = nmax; d.islands = A.islands; d.elite = A.elite; d.migrate_every = 12;
   d.iters = A.iters; d.sigma = A.sigma; d.activeTol = 1e-4; d.seed = A.seed;
   const auto t0 = Clock::now();
@@ -1010,7 +1020,8 @@ void usage() {
       "  --search              the search (the default)\n"
       "  --uneven              island generations of uneven n as HIP graphs\n"
       "                        (defaults --pop 512 --ls-iters 200 --gens 3;\n"
-      "                        --lengths 600,1200,2100,4096,8400 --sigma 0.05 --no-tune)\n"
+      "                        --lengths 600,1200,2100,4096,8400 --sigma 0.05 --no-tune\n"
+      "                        --no-split: no candidate shared by two workgroups)\n"
       "  --fw-ladder           Frank-Wolfe rounds + exact n -> 2n splits per level\n"
       "                        (defaults --ladder 1050,2100,4200,8400 --fw-rounds 50\n"
       "                        --fw-iters 2000)\n"
@@ -1111,6 +1122,7 @@ int main(int argc, char** argv) {
     else if (s == "--sigma") A.sigma = (float)std::atof(next());
     else if (s == "--lengths") list(A.lengths);
     else if (s == "--no-tune") A.tune = false;
+    else if (s == "--no-split") A.split = 0;
     else if (s == "-h" || s == "--help") { usage(); return 0; }
     else {
       std::fprintf(stderr, "unknown option %s\n", s.c_str());

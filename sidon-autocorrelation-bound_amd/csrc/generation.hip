@@ -10,8 +10,8 @@
 // Two graphs are captured (population A -> B and B -> A) and replayed in turn,
 // so a run of G generations is G graph launches.  Every kernel in the
 // sequence reads its generation number from device memory and clears its
-// work-queue counter with a kernel, so replays need no host updates and no
-// memset nodes.
+// work-queue counter and pair flags with a kernel, so replays need no host
+// updates and no memset nodes.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -40,7 +40,8 @@ int enqueue_generation(const sidon_generation_desc& d, int parity, hipStream_t s
   if ((rc = sidon_order_by_length(ns_src, d.P, d.order, s))) return rc;
   if ((rc = sidon_local_search_ragged(src, d.stride, ns_src, d.P, d.nmax, d.order, nullptr,
                                       nullptr, d.iters, d.pStart, d.pEnd, d.lrStart, d.lrEnd,
-                                      0.9f, 0.999f, 0, d.queue, s)))
+                                      0.9f, 0.999f, 0, d.split, d.ls_workspace,
+                                      (size_t)d.ls_workspace_bytes, s)))
     return rc;
   if ((rc = sidon_eval_f64_ragged(src, d.stride, ns_src, d.P, d.nmax, d.fit, d.nact,
                                   d.activeTol, s)))
@@ -56,9 +57,10 @@ int enqueue_generation(const sidon_generation_desc& d, int parity, hipStream_t s
 
 bool desc_ok(const sidon_generation_desc& d) {
   return d.pop_a && d.pop_b && d.ns_a && d.ns_b && d.fit && d.nact && d.p1 && d.p2 && d.op &&
-         d.strength && d.order && d.gen && d.queue && d.P > 0 && d.islands > 0 && d.P % d.islands == 0 &&
+         d.strength && d.order && d.gen && d.ls_workspace && d.P > 0 && d.islands > 0 && d.P % d.islands == 0 &&
          d.nmax > 0 && d.stride >= d.nmax && d.iters >= 0 && d.elite >= 0 &&
-         d.elite <= d.P / d.islands;
+         d.elite <= d.P / d.islands && (d.split == 0 || d.split == 1) &&
+         d.ls_workspace_bytes >= sidon_local_search_ragged_workspace(d.P, d.stride);
 }
 
 }  // namespace

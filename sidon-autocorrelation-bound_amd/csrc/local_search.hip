@@ -47,7 +47,70 @@ struct LsArgs {
   int* queue;           // work-queue counter (nullptr: one candidate per block)
   int iters;
   T pStart, pEnd, lrStart, lrEnd, beta1, beta2;
+  // persistent launches: candidates that may be split over two workgroups
+  // (0: none; 1: the longest, see split_count) and the pair workspace
+  int split;
+  int* pair_flags;      // 16 ints per candidate: [0], [1] each half's progress
+  int* pair_err;        // set when a half stopped waiting for its partner
+  int* split_out;       // the number of candidates split (written by workgroup 0)
+  T* xchg;              // per candidate xstride elements: the b_k of each half's tiles
+  long long xstride;
 };
+
+// ------------------------------------------------------- paired halves --
+// A candidate of a persistent launch may be shared by two workgroups (halves
+// 0 and 1), each holding all of a in LDS: each computes the autoconvolution
+// tiles it owns (conv_item_owner), the halves exchange those b_k through
+// global memory, both form all weights and R (the same arithmetic in the
+// same order, so the same bits), each computes the correlation tiles it owns
+// (corr_tile_owner) and updates their heights, and the halves exchange the
+// new heights through the candidate's row.  S is summed over all heights in
+// the one-workgroup order, so a paired candidate ends bit-identical to the
+// same candidate run by one workgroup.  The two workgroups may sit on
+// different XCDs, whose L2 caches are not coherent with each other, so every
+// exchanged value is written and read with device-coherent (agent-scope
+// relaxed atomic, sc1) accesses: an exchange is those stores, a wait for
+// their completion in every wave, a barrier, a flag raised to a step-numbered
+// value (coherent store), and on the other side a poll of the flag (coherent
+// load), a barrier and coherent loads of the data.  No cache is flushed or
+// invalidated, so the other workgroups of the XCD are not disturbed.  A wait
+// gives up after kPairWaitTicks of the 100 MHz wall clock (a partner that
+// never runs), counts that in *pair_err and stops waiting, so every wave
+// reaches the end.
+constexpr unsigned long long kPairWaitTicks = 2000000000ull;  // 20 s
+constexpr int kPairFinal = 1 << 30;
+
+template <typename T>
+__device__ __forceinline__ void coherent_store(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T coherent_load(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void pair_release(int* flag, int value) {
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wave's coherent stores are done
+  __syncthreads();
+  if (threadIdx.x == 0) coherent_store(flag, value);
+}
+// Returns when the partner's flag has reached `value` (its coherent stores
+// before that are then visible to coherent loads); `broken` (LDS) turns on
+// after a timeout and makes later waits return at once.
+__device__ __forceinline__ void pair_acquire(const int* flag, int value, int* err, int* broken) {
+  if (threadIdx.x == 0 && !*broken) {
+    const unsigned long long t0 = wall_clock64();
+    while (coherent_load(flag) < value) {
+      if (wall_clock64() - t0 > kPairWaitTicks) {
+        *broken = 1;
+        atomicAdd(err, 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __syncthreads();
+}
 
 // Bytes of dynamic LDS for one candidate of length n.
 template <typename T, int SS = kSegStride>
@@ -56,13 +119,15 @@ __host__ __device__ inline long long ls_lds_bytes(int n, int block) {
   // (work items of each product, matrix steps run), the work items of both
   // products and the weight segments' bits (32-bit each; T is at least as
   // wide)
+  // (conv_tiles(n) more: the item rank of each autoconvolution tile)
   const long long elems = (long long)pad_len<SS>(n) + cbuf_len(n) + gbuf_len(n) + 2 * (block / kWave) +
-                          1 + 3 + 2 * conv_tiles(n) + 2 * corr_tiles(n) + segmask_words(n);
+                          1 + 3 + 3 * conv_tiles(n) + 2 * corr_tiles(n) + segmask_words(n);
   return elems * (long long)sizeof(T);
 }
 
 template <typename T, int BD, int QMAX, int SS>
-__device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) {
+__device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned char* smem,
+                             int* broken) {
   constexpr int NW = BD / kWave;
   // operand steps in flight (fp64 operands take twice the registers)
   constexpr int D = sizeof(T) == 8 ? 2 : 4;
@@ -82,19 +147,28 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
   unsigned* citems = nsteps + 1;
   unsigned* gitems = citems + 2 * conv_tiles(n);
   unsigned* segmask = gitems + 2 * corr_tiles(n);
+  int* crank = reinterpret_cast<int*>(segmask + segmask_words(n));  // conv tile -> item rank
   const int tid = threadIdx.x;
   const int LA = pad_len<SS>(n), LC = cbuf_len(n), LG = gbuf_len(n);
   const int SW = segmask_words(n);
+  const int NTc = conv_tiles(n);
+  // pair state (half >= 0): own progress flag, the partner's, exchanged b_k
+  int* myflag = half >= 0 ? A.pair_flags + 16 * (size_t)cand + half : nullptr;
+  const int* peerflag = half >= 0 ? A.pair_flags + 16 * (size_t)cand + (1 - half) : nullptr;
+  T* xb = half >= 0 ? A.xchg + (size_t)cand * (size_t)A.xstride : nullptr;
 
   if (tid == 0) { ctr[0] = 0; ctr[1] = 0; *nsteps = 0u; *bmax = T(0); }
   autoconv_items(n, citems, reinterpret_cast<int*>(sC), tid, BD);  // sC is scratch until zeroed
   corr_items(n, gitems, reinterpret_cast<int*>(sC), tid, BD);
+  for (int r = tid; r < NTc; r += BD) crank[citems[r] >> (2 * kItemBits)] = r;
   for (int i = tid; i < LA; i += BD) sA[i] = T(0);
   for (int i = tid; i < LC; i += BD) sC[i] = T(0);
   for (int i = tid; i < LG; i += BD) sG[i] = T(0);
   for (int w = tid; w < SW; w += BD) segmask[w] = 0u;
   __syncthreads();
   for (int i = tid; i < n; i += BD) pad_put<SS>(sA, i, a[i]);
+  if (half >= 0)  // both halves have read the row before either writes heights into it
+    pair_release(myflag, 1), pair_acquire(peerflag, 1, A.pair_err, broken);
 
   T m[QMAX], v[QMAX];
 #pragma unroll
@@ -113,8 +187,22 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     const T pexp = A.pStart * dexp(lnPr * frac);
     const T lr = A.lrStart * dexp(lnLr * frac);
 
-    autoconv_mfma<T, D, SS>(sA, sC, n, ctr, citems, nsteps, bmax);  // sC[k] = b_k, *bmax = M
+    // sC[k] = b_k, *bmax = M (this half's tiles only when paired)
+    autoconv_mfma<T, D, SS>(sA, sC, n, ctr, citems, nsteps, bmax, half);
     __syncthreads();
+    if (half >= 0) {  // exchange b_k of the owned tiles and the tile maxima
+      for (int k = tid; k < 256 * NTc; k += BD)
+        if (conv_item_owner(crank[k >> 8]) == half) coherent_store(xb + k, sC[k]);
+      if (tid == 0) coherent_store(xb + 256 * NTc + half, *bmax);
+      pair_release(myflag, 2 + 2 * it);
+      pair_acquire(peerflag, 2 + 2 * it, A.pair_err, broken);
+      for (int k = tid; k < 256 * NTc; k += BD)
+        if (conv_item_owner(crank[k >> 8]) != half) sC[k] = coherent_load(xb + k);
+      const T other = coherent_load(xb + 256 * NTc + 1 - half);
+      __syncthreads();  // every thread has read *bmax
+      if (tid == 0 && other > *bmax) *bmax = other;
+      __syncthreads();
+    }
 
     if (tid == 0) ctr[0] = 0;  // every wave is past the autoconvolution
     const int st = opaque(tid);  // per-step copy: nothing derived from it is hoisted
@@ -149,7 +237,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     b2t *= beta2;
     const T bc1 = T(1) / (T(1) - b1t), bc2 = T(1) / (T(1) - b2t);
 
-    corr_mfma<T, D, SS>(sC, sA, sG, n, ctr + 1, gitems, segmask, nsteps);
+    corr_mfma<T, D, SS>(sC, sA, sG, n, ctr + 1, gitems, segmask, nsteps, half);
     const int ut = opaque(tid);  // element addresses below are built per step, not hoisted
     __syncthreads();
 
@@ -162,7 +250,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) {
       const int i = ut + BD * q;
-      if (i < n) {
+      if (i < n && (half < 0 || corr_tile_owner(i >> 8) == half)) {
         const T ai = sA[pad_addr<SS>(i)];
         const T gi = dquot(sG[i], R) - M;
         T xx;
@@ -182,6 +270,20 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
         }
         sG[i] = xx;  // the owner thread parks the new height in place of G_i
         sloc += xx;
+        if (half >= 0) coherent_store(a + i, xx);
+      }
+    }
+    if (half >= 0) {  // exchange the new heights; S over all of them, in the same order
+      pair_release(myflag, 3 + 2 * it);
+      pair_acquire(peerflag, 3 + 2 * it, A.pair_err, broken);
+      sloc = T(0);
+#pragma unroll
+      for (int q = 0; q < QMAX; ++q) {
+        const int i = ut + BD * q;
+        if (i < n) {
+          if (corr_tile_owner(i >> 8) != half) sG[i] = coherent_load(a + i);
+          sloc += sG[i];
+        }
       }
     }
     T S = block_sum<T, NW>(sloc, slots);
@@ -198,47 +300,119 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, unsigned char* smem) 
     __syncthreads();
   }
 
-  for (int i = tid; i < n; i += BD) a[i] = sA[pad_addr<SS>(i)];
+  if (half >= 0)  // the partner has read this half's last heights from the row
+    pair_release(myflag, kPairFinal), pair_acquire(peerflag, kPairFinal, A.pair_err, broken);
+  for (int i = tid; i < n; i += BD)
+    if (half < 0 || corr_tile_owner(i >> 8) == half) a[i] = sA[pad_addr<SS>(i)];
   __syncthreads();  // every wave has added its matrix steps
   if (tid == 0) atomicAdd(&g_block_steps, (unsigned long long)*nsteps);
   if (A.mom) {
     T* mo = A.mom + (size_t)cand * (size_t)A.stride;
 #pragma unroll
-    for (int q = 0; q < QMAX; ++q) { const int i = tid + BD * q; if (i < n) mo[i] = m[q]; }
+    for (int q = 0; q < QMAX; ++q) {
+      const int i = tid + BD * q;
+      if (i < n && (half < 0 || corr_tile_owner(i >> 8) == half)) mo[i] = m[q];
+    }
   }
   if (A.vel) {
     T* vo = A.vel + (size_t)cand * (size_t)A.stride;
 #pragma unroll
-    for (int q = 0; q < QMAX; ++q) { const int i = tid + BD * q; if (i < n) vo[i] = v[q]; }
+    for (int q = 0; q < QMAX; ++q) {
+      const int i = tid + BD * q;
+      if (i < n && (half < 0 || corr_tile_owner(i >> 8) == half)) vo[i] = v[q];
+    }
   }
   __syncthreads();  // LDS is reused by the next candidate of a persistent block
 }
 
+// How many of the longest candidates a persistent launch splits over two
+// workgroups.  With costs t_i (a candidate's time on one workgroup, modelled
+// as n (n + kLinCost), fitted to measured per-length times of the
+// 1024-thread kernel: the O(n) phases and the few work items of short
+// candidates weigh more than n^2 says) sorted longest first, W = sum t_i and
+// G workgroups, splitting the s longest gives a makespan of about
+//   max(c t_0 / 2, t_s, (W + (c - 1)(t_0 + ... + t_{s-1})) / G),
+// c the measured cost of a pair relative to one workgroup (the exchanges,
+// and half as many work items per workgroup); s minimises it (s = 0 unless a
+// split helps), among candidates of at least kSplitMinN.  Uniform lengths
+// with at least one candidate per workgroup split none; a few long
+// candidates among many short ones (uneven islands) split the long ones.
+// `order` lists the candidates longest first, so they are its first
+// entries.  Every workgroup computes the same count (thread 0, serially).
+constexpr int kSplitMinN = 4096;
+constexpr double kLinCost = 11000.0;
+constexpr double kPairCost = 1.5;
+template <typename T>
+__device__ int split_count(const LsArgs<T>& A, int* s_out) {
+  if (!A.split || !A.order || !A.ns) return 0;
+  if (threadIdx.x == 0) {
+    double W = 0.0;
+    for (int p = 0; p < A.P; ++p) {
+      const double n = (double)A.ns[p];
+      W += n * (n + kLinCost);
+    }
+    const double G = (double)gridDim.x;
+    const double n0 = (double)A.ns[A.order[0]];
+    const double t0 = n0 * (n0 + kLinCost);
+    double best = t0 > W / G ? t0 : W / G;
+    int bs = 0;
+    double pre = 0.0;
+    for (int s = 1; s <= A.P; ++s) {
+      const double ns1 = (double)A.ns[A.order[s - 1]];
+      if (ns1 < kSplitMinN) break;
+      pre += ns1 * (ns1 + kLinCost);
+      const double next = s < A.P ? (double)A.ns[A.order[s]] : 0.0;
+      double m = kPairCost * t0 / 2.0;
+      const double tn = next * (next + kLinCost);
+      if (tn > m) m = tn;
+      const double avg = (W + (kPairCost - 1.0) * pre) / G;
+      if (avg > m) m = avg;
+      if (m < best) { best = m; bs = s; }
+    }
+    *s_out = bs;
+  }
+  __syncthreads();
+  const int r = *s_out;
+  __syncthreads();
+  return r;
+}
+
 // One candidate per workgroup (grid = candidates), or workgroups pulling
-// candidates from a device queue (persistent).
+// tickets from a device queue (persistent): ticket t < 2 s is half t % 2 of
+// candidate order[t / 2] (s = split_count), later tickets whole candidates.
 template <typename T, int BD, int QMAX, int SS>
-__device__ __forceinline__ void ls_body(const LsArgs<T>& A, unsigned char* smem, int& s_item) {
+__device__ __forceinline__ void ls_body(const LsArgs<T>& A, unsigned char* smem, int& s_item,
+                                        int* broken) {
+  if (threadIdx.x == 0) *broken = 0;
   if (!A.queue) {
     const int cand = A.order ? A.order[blockIdx.x] : (int)blockIdx.x;
-    ls_candidate<T, BD, QMAX, SS>(A, cand, smem);
+    ls_candidate<T, BD, QMAX, SS>(A, cand, -1, smem, broken);
     return;
   }
+  const int split = split_count<T>(A, &s_item);
+  if (A.split_out && blockIdx.x == 0 && threadIdx.x == 0) *A.split_out = split;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(A.queue, 1);
     __syncthreads();
     const int item = s_item;
     __syncthreads();
-    if (item >= A.P) break;
-    const int cand = A.order ? A.order[item] : item;
-    ls_candidate<T, BD, QMAX, SS>(A, cand, smem);
+    if (item >= A.P + split) break;
+    int cand, half = -1;
+    if (item < 2 * split) {
+      cand = A.order[item >> 1];
+      half = item & 1;
+    } else {
+      cand = A.order ? A.order[item - split] : item - split;
+    }
+    ls_candidate<T, BD, QMAX, SS>(A, cand, half, smem, broken);
   }
 }
 
 template <typename T, int BD, int QMAX, int SS>
 __global__ __launch_bounds__(BD) void k_local_search(LsArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int s_item;
-  ls_body<T, BD, QMAX, SS>(A, smem, s_item);
+  __shared__ int s_item, s_broken;
+  ls_body<T, BD, QMAX, SS>(A, smem, s_item, &s_broken);
 }
 
 // The same kernel held to 128 registers a lane (four waves a SIMD): two
@@ -248,8 +422,8 @@ template <typename T, int BD, int QMAX, int SS>
 __global__ __launch_bounds__(BD) __attribute__((amdgpu_waves_per_eu(4, 4))) void
 k_local_search_pair(LsArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int s_item;
-  ls_body<T, BD, QMAX, SS>(A, smem, s_item);
+  __shared__ int s_item, s_broken;
+  ls_body<T, BD, QMAX, SS>(A, smem, s_item, &s_broken);
 }
 
 // ---------------------------------------------------------------- lp_grad --
@@ -361,7 +535,8 @@ int launch_ls_layout(const LsArgs<T>& A, int nmax, hipStream_t stream) {
         &per_cu, reinterpret_cast<const void*>(kfn), BD, (size_t)lds));
     if (per_cu < 1) per_cu = 1;
     const long long resident = (long long)per_cu * ds->cu_count;
-    grid = resident < A.P ? (int)resident : A.P;
+    const long long tickets = A.split && A.order ? 2LL * A.P : A.P;  // at most P + P halves
+    grid = resident < tickets ? (int)resident : (int)tickets;
   }
   hipLaunchKernelGGL(kfn, dim3(grid), dim3(BD), (size_t)lds, stream, A);
   SIDON_HIP_TRY(hipGetLastError());
@@ -381,13 +556,41 @@ int launch_ls(const LsArgs<T>& A, int nmax, hipStream_t stream) {
   return launch_ls_layout<T, BD, QMAX, PAIR, kSegStride>(A, nmax, stream);
 }
 
-// `queue` non-null: persistent launch pulling candidates from that counter.
+// Workspace of a persistent launch (bytes): the queue counter and the pair
+// timeout count, then 16 progress ints per candidate, then per candidate the
+// b_k exchange of a split pair (256 per autoconvolution tile, and the two
+// halves' maxima).
+constexpr size_t kWsHeader = 256;
+inline long long xchg_stride(int stride) { return 256LL * conv_tiles(stride) + 64; }
+inline size_t ragged_ws_bytes(int P, int stride) {
+  return kWsHeader + (size_t)P * 64 + (size_t)P * (size_t)xchg_stride(stride) * sizeof(float);
+}
+
+// Clears the queue counter, the timeout count and the candidates' progress
+// flags (a kernel, not a memset node, so the launch can be captured).
+__global__ void k_clear_ints(int* p, long long count) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+// `ws` non-null: persistent launch pulling tickets from the workspace queue.
 template <typename T>
-int dispatch_ls(LsArgs<T>& A, int nmax, int block, int* queue, hipStream_t stream) {
-  if (queue) {
-    if (int rc = reset_counter(queue, stream)) return rc;
+int dispatch_ls(LsArgs<T>& A, int nmax, int block, void* ws, hipStream_t stream) {
+  A.queue = nullptr;
+  if (ws) {
+    int* wi = static_cast<int*>(ws);
+    const long long ints = (long long)(kWsHeader / sizeof(int)) + 16LL * A.P;
+    const int grid = (int)((ints + 255) / 256 < 1024 ? (ints + 255) / 256 : 1024);
+    hipLaunchKernelGGL(k_clear_ints, dim3(grid), dim3(256), 0, stream, wi, ints);
+    SIDON_HIP_TRY(hipGetLastError());
+    A.queue = wi;
+    A.pair_err = wi + 1;
+    A.split_out = wi + 2;
+    A.pair_flags = wi + kWsHeader / sizeof(int);
+    A.xchg = reinterpret_cast<T*>(static_cast<char*>(ws) + kWsHeader + (size_t)A.P * 64);
+    A.xstride = xchg_stride((int)A.stride);
   }
-  A.queue = queue;
   switch (block) {
     case 256: return launch_ls<T, 256, 16>(A, nmax, stream);
     case 512:
@@ -486,17 +689,30 @@ extern "C" int sidon_local_search_f64(double* pop, double* mom, double* vel, con
   return dispatch_ls<double>(A, n, block, nullptr, stream);
 }
 
+extern "C" size_t sidon_local_search_ragged_workspace(int P, int stride) {
+  if (P < 0 || stride < 1) return 0;
+  return ragged_ws_bytes(P, stride);
+}
+
 extern "C" int sidon_local_search_ragged(float* pop, int stride, const int* ns, int P, int nmax,
                                          const int* order, const int* skip, const int* mode,
                                          int iters, float pStart, float pEnd, float lrStart,
                                          float lrEnd, float beta1, float beta2, int modeDefault,
-                                         int* queue, hipStream_t stream) {
-  if (!pop || !ns || !queue || nmax < 1 || stride < nmax || P < 0) return SIDON_ERR_ARG;
+                                         int split, void* workspace, size_t workspace_bytes,
+                                         hipStream_t stream) {
+  if (!pop || !ns || !workspace || nmax < 1 || stride < nmax || P < 0) return SIDON_ERR_ARG;
+  if (split != 0 && split != 1) return SIDON_ERR_ARG;
   if (int rc = check_schedule<float>(iters, pStart, pEnd, lrStart, lrEnd)) return rc;
+  if (workspace_bytes < ragged_ws_bytes(P, stride)) {
+    set_last_error_msg("local search (ragged): workspace smaller than "
+                       "sidon_local_search_ragged_workspace(P, stride)");
+    return SIDON_ERR_ARG;
+  }
   if (P == 0 || iters == 0) return SIDON_OK;
   LsArgs<float> A{pop, stride, ns, nmax, P, nullptr, nullptr, skip, mode, modeDefault, order,
-                  nullptr, iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2};
-  return dispatch_ls<float>(A, nmax, pick_block(nmax), queue, stream);
+                  nullptr, iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2,
+                  split, nullptr, nullptr, nullptr, nullptr, 0};
+  return dispatch_ls<float>(A, nmax, pick_block(nmax), workspace, stream);
 }
 
 extern "C" int sidon_lp_grad(const float* a, int n, int B, float p, float* J, float* grad,

@@ -72,7 +72,8 @@ class Eager:
         self.p1, self.p2, self.op = (torch.zeros(P, **i32) for _ in range(3))
         self.st = torch.zeros(P, dtype=torch.float32, device="cuda")
         self.gen = torch.zeros(1, dtype=torch.int64, device="cuda")
-        self.queue = torch.zeros(1, **i32)
+        self.ws = torch.empty(S.ragged_workspace_bytes(P, int(ns.max())), dtype=torch.uint8,
+                              device="cuda")
         self.parity = 0
 
     def generation(self, record: bool):
@@ -84,8 +85,10 @@ class Eager:
             rec["before_ls"] = src.cpu().numpy()
             rec["gen"] = int(self.gen.item())
         S.order_by_length(ns_src, self.order)
-        S.local_search_ragged(src, ns_src, self.nmax, ITERS, order=self.order, queue=self.queue,
+        S.local_search_ragged(src, ns_src, self.nmax, ITERS, order=self.order, workspace=self.ws,
                               **SCHED)
+        if record:
+            rec["pair_timeouts"] = S.pair_timeouts(self.ws)
         if record:
             rec["after_ls"] = src.cpu().numpy()
         fit, nact = S.eval_f64_ragged(src, ns_src, self.nmax)
@@ -133,7 +136,10 @@ def runs(S, start):
 
 
 def test_config3_graph_equals_eager(runs):
-    eager, graph, _ = runs
+    """(Generation 0 splits its longest candidates over two workgroups when
+    the makespan model says so; no pair wait gives up.)"""
+    eager, graph, recs = runs
+    assert all(r["pair_timeouts"] == 0 for r in recs.values() if r)
     gp, gn = graph.population()
     ep, en = eager.population()
     assert graph.gen.item() == eager.gen.item() == GENS

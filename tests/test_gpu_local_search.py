@@ -390,12 +390,45 @@ def test_concurrent_ragged_launches_match_serial(S, W):
         serial.append(t)
     torch.cuda.synchronize()
     streams = [torch.cuda.Stream(), torch.cuda.Stream()]
-    queues = [torch.zeros(1, dtype=torch.int32, device="cuda") for _ in range(2)]
+    wss = [torch.empty(S.ragged_workspace_bytes(ns.size, nmax), dtype=torch.uint8,
+                       device="cuda") for _ in range(2)]
     conc = [dev(pops[0]), dev(pops[1])]
     torch.cuda.synchronize()
     for k in range(2):
         with torch.cuda.stream(streams[k]):
-            S.local_search_ragged(conc[k], nsd, nmax, queue=queues[k], stream=streams[k], **kw)
+            S.local_search_ragged(conc[k], nsd, nmax, workspace=wss[k], stream=streams[k], **kw)
     torch.cuda.synchronize()
     for k in range(2):
         assert torch.equal(conc[k], serial[k]), k
+
+
+def test_split_pairs_equal_one_workgroup_bitwise(S, W):
+    """Uneven islands in one persistent launch with the long candidates split
+    over two workgroups (split = 1) end bit-identical to the same launch with
+    every candidate on one workgroup (split = 0), and no pair wait gives up."""
+    print("[ls] split pairs vs one workgroup", flush=True)
+    rng = np.random.default_rng(12)
+    ns = rng.choice(np.array([600, 1200, 2100, 4096, 8400], np.int32), size=96).astype(np.int32)
+    ns[:3] = 8400
+    nmax = 8400
+    pop = np.zeros((ns.size, nmax), np.float32)
+    for p, n in enumerate(ns):
+        pop[p, :n] = W.structured_prior(1, int(n), seed=900 + p)[0]
+    nsd = dev(ns, torch.int32)
+    order = torch.empty(ns.size, dtype=torch.int32, device="cuda")
+    S.order_by_length(nsd, order)
+    kw = dict(iters=60, p_start=4.0, p_end=1e4, lr_start=0.085, lr_end=0.0015, order=order)
+    out = {}
+    for split in (0, 1):
+        t = dev(pop)
+        ws = torch.empty(S.ragged_workspace_bytes(ns.size, nmax), dtype=torch.uint8,
+                         device="cuda")
+        S.local_search_ragged(t, nsd, nmax, split=split, workspace=ws, **kw)
+        torch.cuda.synchronize()
+        assert S.pair_timeouts(ws) == 0
+        # the makespan model splits all 24 n = 8400 candidates (96 candidates,
+        # 192 workgroups) when allowed to
+        assert S.split_count(ws) == (24 if split else 0)
+        out[split] = t
+    assert torch.equal(out[0], out[1])
+    assert not torch.equal(out[1], dev(pop))

@@ -48,9 +48,18 @@ def test_argument_validation_without_device():
     assert L.sidon_select(fake, None, None, 0, 10, 3, 1, 0, 0.05, 1, fake, fake, fake, fake,
                           fake, None) == S.SIDON_ERR_ARG  # P not divisible by NI
     assert L.sidon_set_ls_block(4096, 300) == S.SIDON_ERR_ARG
-    # the persistent launch needs a caller-owned work-queue counter
+    # the persistent launch needs a caller-owned workspace of the stated size
     assert L.sidon_local_search_ragged(fake, 64, fake, 4, 64, None, None, None, 10, 4.0, 8.0,
-                                       0.1, 0.01, 0.9, 0.999, 0, None, None) == S.SIDON_ERR_ARG
+                                       0.1, 0.01, 0.9, 0.999, 0, 1, None, 0,
+                                       None) == S.SIDON_ERR_ARG
+    need = L.sidon_local_search_ragged_workspace(4, 64)
+    assert need > 0
+    assert L.sidon_local_search_ragged(fake, 64, fake, 4, 64, None, None, None, 10, 4.0, 8.0,
+                                       0.1, 0.01, 0.9, 0.999, 0, 1, fake, need - 1,
+                                       None) == S.SIDON_ERR_ARG
+    assert L.sidon_local_search_ragged(fake, 64, fake, 4, 64, None, None, None, 10, 4.0, 8.0,
+                                       0.1, 0.01, 0.9, 0.999, 0, 2, fake, need,
+                                       None) == S.SIDON_ERR_ARG
     assert S.lib().sidon_status_string(S.SIDON_ERR_LDS).startswith(b"length needs more LDS")
 
 
@@ -72,6 +81,7 @@ def test_generation_desc_matches_header():
     body = re.search(r"typedef struct \{(.*?)\} sidon_generation_desc;", text, re.S).group(1)
     body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
     kinds = {"float*": ctypes.c_void_p, "int*": ctypes.c_void_p, "double*": ctypes.c_void_p,
+             "void*": ctypes.c_void_p,
              "unsigned long long*": ctypes.c_void_p, "int": ctypes.c_int,
              "float": ctypes.c_float, "double": ctypes.c_double,
              "unsigned long long": ctypes.c_uint64}
@@ -80,7 +90,7 @@ def test_generation_desc_matches_header():
         decl = decl.strip()
         if not decl:
             continue
-        m = re.match(r"((?:unsigned long long|int|float|double))\s*(\*?)\s*(.*)", decl)
+        m = re.match(r"((?:unsigned long long|int|float|double|void))\s*(\*?)\s*(.*)", decl)
         base, star, names = m.group(1), m.group(2), m.group(3)
         for part in names.split(","):
             part = part.strip()
