@@ -42,6 +42,27 @@ def test_plain_config_vectors_bitwise(S, O, W):
     np.testing.assert_array_equal(na.cpu().numpy(), cnt)
 
 
+def test_plain_config_unrounded_fp64(S, O, W):
+    """Config 1 as stated: 64 x 512 fp64 vectors (a_i = 0 w.p. 0.33 else
+    U(0,1)), not rounded to fp32, through the fp64-input evaluation
+    (kEval<double,double>).  fp64 products are not exact, and the kernel adds
+    each with one fused rounding where hostF rounds the product first, so F
+    agrees with hostF and with the long-double verifier to 1e-12, not bit
+    for bit (measured: within a few ulp)."""
+    print("[eval] plain config 64 x 512 fp64 unrounded", flush=True)
+    a = W.plain_vectors(64, 512)
+    assert not np.array_equal(a, a.astype(np.float32).astype(np.float64))
+    F, na = S.eval_f64(dev(a, torch.float64), active_tol=1e-9)
+    F = F.cpu().numpy()
+    worst = 0.0
+    for p in range(64):
+        h = O.F_hostorder(a[p])
+        ld = float(O.verify_ld(a[p])["F"])
+        worst = max(worst, abs(F[p] - h) / h, abs(F[p] - ld) / ld)
+    print(f"   worst relative difference {worst:.2e}")
+    assert worst < 1e-12
+
+
 @pytest.mark.parametrize("n,tol", [(1000, 0.3), (2048, 0.05), (2100, 0.3), (4096, 0.02),
                                    (8400, 0.3), (8400, 1e-3), (2100, 0.0), (1000, 1.5)])
 def test_active_count_many_tiles(S, O, W, n, tol):
