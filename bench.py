@@ -52,6 +52,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0,
                     help="workgroup size of the local search at this n (0: the library's rule)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner and launch with the built-in configuration")
+    ap.add_argument("--tune-only", default="",
+                    help="comma-separated lengths: run the autotuner at each (P candidates "
+                         "of that length, 50-step trials) and print its tables")
     ap.add_argument("--fw-only", action="store_true",
                     help="time the Frank-Wolfe / Chambolle-Pock kernel alone (config 5's "
                          "levels) and print its JSON")
@@ -259,6 +264,24 @@ def hostf_evals(vecs, threads: int, seconds: float = 1.5) -> float:
     return done / (time.perf_counter() - t0)
 
 
+def tune_table(S, args):
+    """The autotuner's trials at each length: P structured-prior candidates
+    of that length, 50 projected-Adam steps per trial, three timed trials
+    after a warm-up; the built-in rule and the tuner's choice beside them."""
+    out = {}
+    for n in (int(x) for x in args.tune_only.split(",")):
+        builtin = S.get_ls_tile(n)
+        tile, ms, table = S.tune_local_search(n, P=args.P, iters=50, repeats=3,
+                                              seed=args.seed + 1)
+        S.set_ls_tile(n, 0)
+        name = "{}/{}/{}".format
+        out[n] = {"builtin": name(*builtin), "choice": name(*tile), "ms": ms,
+                  "builtin_ms": table.get(builtin),
+                  "trials_ms": {name(*k): v for k, v in sorted(table.items())}}
+        print(f"[tune] n={n}: {out[n]['choice']} {ms:.3f} ms", file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -279,6 +302,9 @@ def main() -> int:
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.tune_only:
+        print(json.dumps({"autotune": tune_table(S, args), "host": host}))
+        return 0
     if args.fw_only:
         print(json.dumps({"frank_wolfe": fw_legs(S, W, torch, args.seed), "host": host}))
         return 0
@@ -291,8 +317,17 @@ def main() -> int:
         return 0
     n, P, iters = args.n, args.P, args.iters
     sched = (4.0, 1e4, 0.085, 0.0015)
+    tuning = None
     if args.block:
         S.set_ls_block(n, args.block)
+    elif not args.no_tune:
+        # the autotuner picks waves per candidate, operand depth and candidates
+        # a CU at n on the workload's own population size (20-step trials)
+        tile, tms, table = S.tune_local_search(n, P=P, iters=20, repeats=2, seed=args.seed + 1)
+        tuning = {"choice": {"block": tile[0], "depth": tile[1], "per_cu": tile[2]},
+                  "ms_per_trial": tms, "trial_iters": 20,
+                  "trials": [{"block": b, "depth": d, "per_cu": c, "ms": m}
+                             for (b, d, c), m in sorted(table.items())]}
 
     if args.profile_copy_mib:
         src = torch.ones(args.profile_copy_mib << 18, dtype=torch.float32, device="cuda")
@@ -395,7 +430,9 @@ def main() -> int:
                                f"{iters} projected-Adam steps per launch, p 4->1e4, "
                                f"lr 0.085->0.0015, one persistent launch",
                    "P": P, "n": n, "iters_per_step": iters, "seed": args.seed,
-                   "block": S.get_ls_block(n)},
+                   "block": S.get_ls_block(n),
+                   "tile": dict(zip(("block", "depth", "per_cu"), S.get_ls_tile(n)))},
+        "autotune": tuning,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
                      "traffic": traffic_from_profiles(n, P, iters),

@@ -152,8 +152,19 @@ int sidon_local_search_ragged(float* pop, int stride, const int* ns, int P, int 
                               float pStart, float pEnd, float lrStart, float lrEnd,
                               float beta1, float beta2, int modeDefault, int split,
                               void* workspace, size_t workspace_bytes, hipStream_t stream);
-/* Choose the workgroup size the fp32 local search uses at length n
- * (256, 512 or 1024; 0 restores the built-in rule).  Set by the autotuner. */
+/* Choose the launch configuration the fp32 local search uses at length n:
+ * workgroup size `block` (256, 512 or 1024 threads: 4, 8 or 16 waves per
+ * candidate), `depth` operand steps in flight in the matrix-core tile loops
+ * (2 or 4) and `per_cu` candidates a CU (2 only with block 512 and
+ * n <= 4096: two workgroups held to 128 registers a lane share a CU; else 1).
+ * block 0 restores the built-in rule (512/4/2 for 512 <= n <= 4096,
+ * 1024/4/1 above, 256/4/1 below).  Set by the autotuner.  The rule is
+ * process-wide host state, read when a launch is enqueued (a captured graph
+ * keeps the configuration it was captured with). */
+int sidon_set_ls_tile(int n, int block, int depth, int per_cu);
+int sidon_get_ls_tile(int n, int* block, int* depth, int* per_cu);
+/* The workgroup-size-only forms: set_ls_block keeps the built-in depth and
+ * sharing for that size. */
 int sidon_set_ls_block(int n, int block);
 int sidon_get_ls_block(int n);
 
@@ -311,14 +322,24 @@ int sidon_fw_block_steps(unsigned long long* steps, int reset);
 
 /* ---- autotuner ---------------------------------------------------------- */
 
-/* Times the fused local search at length n for each workgroup size (4, 8 or
- * 16 waves per candidate) on `scratch` (P x n, overwritten; reset from
- * `reference` before every trial when non-NULL), `repeats` timed runs of
- * `iters` steps after one warm-up, and keeps the fastest as the rule for n
- * (sidon_get_ls_block).  No reference counterpart (tile sizes are
- * compile-time constants there, src/kernels.cuh:30-33). */
+/* The configurations the autotuner tries, i = 0 .. SIDON_LS_TILES-1:
+ * 256/2/1, 256/4/1, 512/2/1, 512/4/1, 512/2/2, 512/4/2, 1024/2/1, 1024/4/1
+ * (block/depth/per_cu as in sidon_set_ls_tile). */
+#define SIDON_LS_TILES 8
+int sidon_ls_tile_config(int i, int* block, int* depth, int* per_cu);
+
+/* Times the fused local search at length n in every configuration above on
+ * `scratch` (P x n, overwritten; reset from `reference` before every trial
+ * when non-NULL), `repeats` timed runs of `iters` steps after one warm-up,
+ * and keeps the fastest as the rule for n (sidon_get_ls_tile); the built-in
+ * configuration stays unless another beats it by more than 1%.  Reports the
+ * choice and its time, and, when `trial_ms` is non-NULL, the time of every
+ * configuration (SIDON_LS_TILES floats, -1 where it does not fit n).  No
+ * reference counterpart (tile sizes are compile-time constants there,
+ * src/kernels.cuh:30-33). */
 int sidon_tune_local_search(float* scratch, const float* reference, int n, int P, int iters,
-                            int repeats, int* best_block, float* best_ms, hipStream_t stream);
+                            int repeats, int* best_block, int* best_depth, int* best_per_cu,
+                            float* best_ms, float* trial_ms, hipStream_t stream);
 
 /* ---- island search policy (host only) ---------------------------------- */
 

@@ -34,7 +34,8 @@ EXPORTS = (
     "sidon_local_search_ragged", "sidon_local_search_ragged_workspace",
     "sidon_polish_f64_workspace", "sidon_polish_f64",
     "sidon_polish_f64_noalloc",
-    "sidon_set_ls_block", "sidon_get_ls_block", "sidon_lp_grad",
+    "sidon_set_ls_block", "sidon_get_ls_block", "sidon_set_ls_tile", "sidon_get_ls_tile",
+    "sidon_ls_tile_config", "sidon_lp_grad",
     "sidon_lp_grad_f64", "sidon_frank_wolfe", "sidon_refine", "sidon_variation",
     "sidon_variation_ragged", "sidon_support_sig", "sidon_select", "sidon_advance_gen", "sidon_order_by_length",
     "sidon_generation_create", "sidon_generation_run", "sidon_generation_current",
@@ -101,6 +102,9 @@ def _declare(L):
         "sidon_local_search_ragged_workspace": ([_i, _i], C.c_size_t),
         "sidon_set_ls_block": ([_i, _i], _i),
         "sidon_get_ls_block": ([_i], _i),
+        "sidon_set_ls_tile": ([_i, _i, _i, _i], _i),
+        "sidon_get_ls_tile": ([_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)], _i),
+        "sidon_ls_tile_config": ([_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)], _i),
         "sidon_lp_grad": ([_vp, _i, _i, _f, _vp, _vp, _vp], _i),
         "sidon_lp_grad_f64": ([_vp, _i, _i, _d, _vp, _vp, _vp], _i),
         "sidon_frank_wolfe": ([_vp, _vp, _i, _i, _i, _i, _i, _f, _vp], _i),
@@ -132,8 +136,8 @@ def _declare(L):
         "sidon_generation_current": ([_vp, C.POINTER(_i)], _i),
         "sidon_generation_enqueue": ([C.POINTER(GenerationDesc), _i, _vp], _i),
         "sidon_generation_destroy": ([_vp], _i),
-        "sidon_tune_local_search": ([_vp, _vp, _i, _i, _i, _i, C.POINTER(_i), C.POINTER(_f), _vp],
-                                    _i),
+        "sidon_tune_local_search": ([_vp, _vp, _i, _i, _i, _i, C.POINTER(_i), C.POINTER(_i),
+                                     C.POINTER(_i), C.POINTER(_f), C.POINTER(_f), _vp], _i),
         "sidon_measure_fp32_ceilings": ([_vp, _i, C.POINTER(_d), C.POINTER(_d), _vp], _i),
         "sidon_ls_block_steps": ([C.POINTER(C.c_ulonglong), _i], _i),
         "sidon_fw_block_steps": ([C.POINTER(C.c_ulonglong), _i], _i),
@@ -306,6 +310,30 @@ def set_ls_block(n: int, block: int) -> None:
 
 def get_ls_block(n: int) -> int:
     return lib().sidon_get_ls_block(n)
+
+
+LS_TILES = 8
+
+
+def set_ls_tile(n: int, block: int, depth: int = 4, per_cu: int = 1) -> None:
+    """Launch configuration of the fp32 local search at length n: workgroup
+    size, operand steps in flight (2 or 4), candidates a CU (2 only with 512
+    threads and n <= 4096).  block 0 restores the built-in rule."""
+    _check(lib().sidon_set_ls_tile(n, block, depth, per_cu), "sidon_set_ls_tile")
+
+
+def get_ls_tile(n: int) -> tuple[int, int, int]:
+    b, d, c = C.c_int(), C.c_int(), C.c_int()
+    _check(lib().sidon_get_ls_tile(n, C.byref(b), C.byref(d), C.byref(c)), "sidon_get_ls_tile")
+    return b.value, d.value, c.value
+
+
+def ls_tile_config(i: int) -> tuple[int, int, int]:
+    """The autotuner's i-th configuration (block, depth, per_cu)."""
+    b, d, c = C.c_int(), C.c_int(), C.c_int()
+    _check(lib().sidon_ls_tile_config(i, C.byref(b), C.byref(d), C.byref(c)),
+           "sidon_ls_tile_config")
+    return b.value, d.value, c.value
 
 
 def lp_grad(a: torch.Tensor, p: float, want_J: bool = True, want_grad: bool = True,
@@ -527,18 +555,23 @@ def fw_block_steps(reset: bool = False) -> int:
 
 def tune_local_search(n: int, P: int = 64, iters: int = 20, repeats: int = 2, seed: int = 1,
                       stream=None):
-    """Autotune the waves per candidate (workgroup size) of the fused local
-    search at length n on a structured-prior sample; the choice is kept by the
-    library for n.  Returns (block, ms per trial)."""
+    """Autotune the launch configuration of the fused local search at length n
+    (workgroup size, operand steps in flight, candidates a CU) on a
+    structured-prior sample; the choice is kept by the library for n.  Returns
+    ((block, depth, per_cu), ms of the choice, {configuration: ms} of every
+    trial that fits n)."""
     _need_gpu()
     from .workloads import structured_prior
     ref = torch.as_tensor(structured_prior(P, n, seed=seed), device="cuda")
     scratch = torch.empty_like(ref)
-    blk, ms = C.c_int(), C.c_float()
+    blk, dep, per, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
+    trials = (C.c_float * LS_TILES)()
     _check(lib().sidon_tune_local_search(_ptr(scratch), _ptr(ref), n, P, iters, repeats,
-                                         C.byref(blk), C.byref(ms), _stream(stream)),
+                                         C.byref(blk), C.byref(dep), C.byref(per), C.byref(ms),
+                                         trials, _stream(stream)),
            "sidon_tune_local_search")
-    return blk.value, ms.value
+    table = {ls_tile_config(i): float(trials[i]) for i in range(LS_TILES) if trials[i] >= 0}
+    return (blk.value, dep.value, per.value), ms.value, table
 
 
 class Generation:

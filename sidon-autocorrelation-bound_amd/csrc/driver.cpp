@@ -382,16 +382,19 @@ int uneven(const Args& A) {
 
   hipStream_t s;
   CHECK_HIP(hipStreamCreate(&s));  // blocking: orders with the null-stream copies below
-  if (A.tune) {  // autotune waves per candidate at the longest length (it sets the launch)
+  if (A.tune) {  // autotune the launch configuration at the longest length (it sets the launch)
     float* scratch = dalloc<float>((size_t)64 * nmax);
     float* ref = dalloc<float>((size_t)64 * nmax);
     std::vector<float> r((size_t)64 * nmax);
     for (int p = 0; p < 64; ++p) structured_prior(rng, nmax, &r[(size_t)p * nmax]);
     CHECK_HIP(hipMemcpy(ref, r.data(), r.size() * sizeof(float), hipMemcpyHostToDevice));
-    int blk = 0;
+    int blk = 0, depth = 0, per_cu = 0;
     float ms = 0.0f;
-    CHECK_SIDON(sidon_tune_local_search(scratch, ref, nmax, 64, 10, 2, &blk, &ms, s));
-    std::printf("{\"autotune\": {\"n\": %d, \"block\": %d, \"ms\": %.3f}}\n", nmax, blk, ms);
+    CHECK_SIDON(sidon_tune_local_search(scratch, ref, nmax, 64, 10, 2, &blk, &depth, &per_cu, &ms,
+                                        nullptr, s));
+    std::printf("{\"autotune\": {\"n\": %d, \"block\": %d, \"depth\": %d, \"per_cu\": %d, "
+                "\"ms\": %.3f}}\n",
+                nmax, blk, depth, per_cu, ms);
     CHECK_HIP(hipFree(scratch));
     CHECK_HIP(hipFree(ref));
   }

@@ -137,24 +137,43 @@ extern "C" int sidon_generation_destroy(sidon_generation* g) {
 }
 
 // ------------------------------------------------------------ autotuner --
-// Times the fused local search at length n for every workgroup size on the
+// The launch configurations the tuner tries, in the order of its trial table:
+// waves per candidate (workgroup size), operand steps in flight, candidates a
+// CU.  The built-in rule at every length is one of them.
+static const int kTiles[SIDON_LS_TILES][3] = {
+    {256, 2, 1}, {256, 4, 1}, {512, 2, 1}, {512, 4, 1},
+    {512, 2, 2}, {512, 4, 2}, {1024, 2, 1}, {1024, 4, 1}};
+
+extern "C" int sidon_ls_tile_config(int i, int* block, int* depth, int* per_cu) {
+  if (i < 0 || i >= SIDON_LS_TILES) return SIDON_ERR_ARG;
+  if (block) *block = kTiles[i][0];
+  if (depth) *depth = kTiles[i][1];
+  if (per_cu) *per_cu = kTiles[i][2];
+  return SIDON_OK;
+}
+
+// Times the fused local search at length n in every configuration on the
 // caller's scratch population (P x n, modified in place; restored from
 // `reference` before each trial when given) and keeps the fastest as the rule
-// for n.  Block size sets the waves per candidate (4, 8 or 16).
+// for n.  The built-in configuration stays unless another beats it by more
+// than 1% (two runs of one configuration differ by a few tenths of a percent).
 extern "C" int sidon_tune_local_search(float* scratch, const float* reference, int n, int P,
-                                       int iters, int repeats, int* best_block, float* best_ms,
+                                       int iters, int repeats, int* best_block, int* best_depth,
+                                       int* best_per_cu, float* best_ms, float* trial_ms,
                                        hipStream_t stream) {
   if (!scratch || n < 1 || P < 1 || iters < 1 || repeats < 1) return SIDON_ERR_ARG;
-  static const int blocks[3] = {256, 512, 1024};
+  (void)sidon_set_ls_tile(n, 0, 0, 0);
+  int dblock = 0, ddepth = 0, dper = 0;
+  (void)sidon_get_ls_tile(n, &dblock, &ddepth, &dper);
   hipEvent_t e0, e1;
   SIDON_HIP_TRY(hipEventCreate(&e0));
   SIDON_HIP_TRY(hipEventCreate(&e1));
-  int best = 0;
-  float bms = 0.0f;
+  float ms_of[SIDON_LS_TILES];
   int rc = SIDON_OK;
   const size_t bytes = (size_t)P * (size_t)n * sizeof(float);
-  for (int b : blocks) {
-    if ((rc = sidon_set_ls_block(n, b))) break;
+  for (int c = 0; c < SIDON_LS_TILES && !rc; ++c) {
+    ms_of[c] = -1.0f;
+    if ((rc = sidon_set_ls_tile(n, kTiles[c][0], kTiles[c][1], kTiles[c][2]))) break;
     float tot = 0.0f;
     bool ok = true;
     for (int r = 0; r <= repeats && ok; ++r) {  // r = 0 warms up
@@ -174,17 +193,26 @@ extern "C" int sidon_tune_local_search(float* scratch, const float* reference, i
       (void)hipEventElapsedTime(&ms, e0, e1);
       if (r > 0) tot += ms;
     }
-    if (rc) break;
-    if (!ok) continue;
-    const float ms = tot / (float)repeats;
-    if (!best || ms < bms) { best = b; bms = ms; }
+    if (!rc && ok) ms_of[c] = tot / (float)repeats;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (rc) { (void)sidon_set_ls_block(n, 0); return rc; }
-  if (!best) { (void)sidon_set_ls_block(n, 0); return SIDON_ERR_LDS; }
-  (void)sidon_set_ls_block(n, best);
-  if (best_block) *best_block = best;
-  if (best_ms) *best_ms = bms;
+  (void)sidon_set_ls_tile(n, 0, 0, 0);
+  if (rc) return rc;
+  int dflt = -1, best = -1;
+  for (int c = 0; c < SIDON_LS_TILES; ++c) {
+    if (kTiles[c][0] == dblock && kTiles[c][1] == ddepth && kTiles[c][2] == dper) dflt = c;
+    if (ms_of[c] >= 0.0f && (best < 0 || ms_of[c] < ms_of[best])) best = c;
+  }
+  if (best < 0) return SIDON_ERR_LDS;
+  if (dflt >= 0 && ms_of[dflt] >= 0.0f && ms_of[best] > 0.99f * ms_of[dflt]) best = dflt;
+  if (best != dflt)
+    (void)sidon_set_ls_tile(n, kTiles[best][0], kTiles[best][1], kTiles[best][2]);
+  if (best_block) *best_block = kTiles[best][0];
+  if (best_depth) *best_depth = kTiles[best][1];
+  if (best_per_cu) *best_per_cu = kTiles[best][2];
+  if (best_ms) *best_ms = ms_of[best];
+  if (trial_ms)
+    for (int c = 0; c < SIDON_LS_TILES; ++c) trial_ms[c] = ms_of[c];
   return SIDON_OK;
 }

@@ -125,12 +125,11 @@ __host__ __device__ inline long long ls_lds_bytes(int n, int block) {
   return elems * (long long)sizeof(T);
 }
 
-template <typename T, int BD, int QMAX, int SS>
+template <typename T, int BD, int QMAX, int SS, int D>
 __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned char* smem,
                              int* broken) {
   constexpr int NW = BD / kWave;
-  // operand steps in flight (fp64 operands take twice the registers)
-  constexpr int D = sizeof(T) == 8 ? 2 : 4;
+  // D: operand steps in flight in the tile loops (a tuning choice)
   const int n = A.ns ? A.ns[cand] : A.n;
   if (A.skip && A.skip[cand]) return;
   if (n <= 0) return;
@@ -380,13 +379,13 @@ __device__ int split_count(const LsArgs<T>& A, int* s_out) {
 // One candidate per workgroup (grid = candidates), or workgroups pulling
 // tickets from a device queue (persistent): ticket t < 2 s is half t % 2 of
 // candidate order[t / 2] (s = split_count), later tickets whole candidates.
-template <typename T, int BD, int QMAX, int SS>
+template <typename T, int BD, int QMAX, int SS, int D>
 __device__ __forceinline__ void ls_body(const LsArgs<T>& A, unsigned char* smem, int& s_item,
                                         int* broken) {
   if (threadIdx.x == 0) *broken = 0;
   if (!A.queue) {
     const int cand = A.order ? A.order[blockIdx.x] : (int)blockIdx.x;
-    ls_candidate<T, BD, QMAX, SS>(A, cand, -1, smem, broken);
+    ls_candidate<T, BD, QMAX, SS, D>(A, cand, -1, smem, broken);
     return;
   }
   const int split = split_count<T>(A, &s_item);
@@ -404,26 +403,26 @@ __device__ __forceinline__ void ls_body(const LsArgs<T>& A, unsigned char* smem,
     } else {
       cand = A.order ? A.order[item - split] : item - split;
     }
-    ls_candidate<T, BD, QMAX, SS>(A, cand, half, smem, broken);
+    ls_candidate<T, BD, QMAX, SS, D>(A, cand, half, smem, broken);
   }
 }
 
-template <typename T, int BD, int QMAX, int SS>
+template <typename T, int BD, int QMAX, int SS, int D>
 __global__ __launch_bounds__(BD) void k_local_search(LsArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_item, s_broken;
-  ls_body<T, BD, QMAX, SS>(A, smem, s_item, &s_broken);
+  ls_body<T, BD, QMAX, SS, D>(A, smem, s_item, &s_broken);
 }
 
 // The same kernel held to 128 registers a lane (four waves a SIMD): two
 // 512-thread candidates then share a CU where their LDS allows it (n <= 4096),
 // and one candidate's elementwise phases overlap the other's matrix work.
-template <typename T, int BD, int QMAX, int SS>
+template <typename T, int BD, int QMAX, int SS, int D>
 __global__ __launch_bounds__(BD) __attribute__((amdgpu_waves_per_eu(4, 4))) void
 k_local_search_pair(LsArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_item, s_broken;
-  ls_body<T, BD, QMAX, SS>(A, smem, s_item, &s_broken);
+  ls_body<T, BD, QMAX, SS, D>(A, smem, s_item, &s_broken);
 }
 
 // ---------------------------------------------------------------- lp_grad --
@@ -493,23 +492,42 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
 // ------------------------------------------------------------- host side --
 namespace {
 
-// Per-length block-size override set by the autotuner (n -> block).
-struct BlockRule { int n, block; };
-BlockRule g_rules[64];
+// Launch configuration of the fp32 local search at one length: waves per
+// candidate (workgroup size), operand steps in flight in the tile loops, and
+// candidates a CU (2: the 512-thread workgroup held to 128 registers a lane,
+// so two share a CU where their LDS allows; 1: one candidate a CU).
+struct LsTile { int block, depth, per_cu; };
+
+// Per-length overrides set by the autotuner (n -> configuration).
+struct TileRule { int n; LsTile tile; };
+TileRule g_rules[64];
 int g_nrules = 0;
 
-// 512 threads from n = 512 up to 4096, where the register-capped kernel fits
-// two candidates a CU (measured faster than 256 and 1024 at n = 600 ... 4096);
-// 1024 above, where one candidate fills a CU's LDS.
-int default_block(int n) { return n > 4096 ? 1024 : (n >= 512 ? 512 : 256); }
-
-int pick_block(int nmax) {
-  for (int r = 0; r < g_nrules; ++r)
-    if (g_rules[r].n == nmax) return g_rules[r].block;
-  return default_block(nmax);
+// The built-in rule: 512 threads from n = 512 up to 4096, two candidates a
+// CU (measured faster than 256 and 1024 at n = 600 ... 4096); 1024 above,
+// where one candidate fills a CU's LDS; four operand steps in flight.
+LsTile default_tile(int n) {
+  if (n > 4096) return LsTile{1024, 4, 1};
+  if (n >= 512) return LsTile{512, 4, 2};
+  return LsTile{256, 4, 1};
 }
 
-template <typename T, int BD, int QMAX, bool PAIR, int SS>
+LsTile pick_tile(int nmax) {
+  for (int r = 0; r < g_nrules; ++r)
+    if (g_rules[r].n == nmax) return g_rules[r].tile;
+  return default_tile(nmax);
+}
+
+bool tile_valid(const LsTile& c) {
+  if (c.depth != 2 && c.depth != 4) return false;
+  switch (c.block) {
+    case 256: case 1024: return c.per_cu == 1;
+    case 512: return c.per_cu == 1 || c.per_cu == 2;
+    default: return false;
+  }
+}
+
+template <typename T, int BD, int QMAX, bool PAIR, int SS, int D>
 int launch_ls_layout(const LsArgs<T>& A, int nmax, hipStream_t stream) {
   if (nmax > BD * QMAX) return SIDON_ERR_UNSUPPORTED;
   int rc = SIDON_OK;
@@ -521,8 +539,8 @@ int launch_ls_layout(const LsArgs<T>& A, int nmax, hipStream_t stream) {
     return SIDON_ERR_LDS;
   }
   auto kfn = [] {
-    if constexpr (PAIR) return k_local_search_pair<T, BD, QMAX, SS>;
-    else return k_local_search<T, BD, QMAX, SS>;
+    if constexpr (PAIR) return k_local_search_pair<T, BD, QMAX, SS, D>;
+    else return k_local_search<T, BD, QMAX, SS, D>;
   }();
   SIDON_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kfn),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -545,15 +563,23 @@ int launch_ls_layout(const LsArgs<T>& A, int nmax, hipStream_t stream) {
 
 // The wide (bank-conflict-free) segment layout where it fits the LDS, else
 // the narrow one.
-template <typename T, int BD, int QMAX, bool PAIR = false>
+template <typename T, int BD, int QMAX, bool PAIR, int D>
 int launch_ls(const LsArgs<T>& A, int nmax, hipStream_t stream) {
   int rc = SIDON_OK;
   DeviceState* ds = device_state(&rc);
   if (!ds) return rc;
   if constexpr (sizeof(T) == 4)
     if (ls_lds_bytes<T, kSegStrideWide>(nmax, BD) <= ds->lds_max)
-      return launch_ls_layout<T, BD, QMAX, PAIR, kSegStrideWide>(A, nmax, stream);
-  return launch_ls_layout<T, BD, QMAX, PAIR, kSegStride>(A, nmax, stream);
+      return launch_ls_layout<T, BD, QMAX, PAIR, kSegStrideWide, D>(A, nmax, stream);
+  return launch_ls_layout<T, BD, QMAX, PAIR, kSegStride, D>(A, nmax, stream);
+}
+
+// fp32 configurations by operand depth
+template <int BD, int QMAX, bool PAIR>
+int launch_ls_depth(const LsArgs<float>& A, int nmax, int depth, hipStream_t stream) {
+  if (depth == 2) return launch_ls<float, BD, QMAX, PAIR, 2>(A, nmax, stream);
+  if (depth == 4) return launch_ls<float, BD, QMAX, PAIR, 4>(A, nmax, stream);
+  return SIDON_ERR_UNSUPPORTED;
 }
 
 // Workspace of a persistent launch (bytes): the queue counter and the pair
@@ -576,7 +602,7 @@ __global__ void k_clear_ints(int* p, long long count) {
 
 // `ws` non-null: persistent launch pulling tickets from the workspace queue.
 template <typename T>
-int dispatch_ls(LsArgs<T>& A, int nmax, int block, void* ws, hipStream_t stream) {
+int dispatch_ls(LsArgs<T>& A, int nmax, const LsTile& tile, void* ws, hipStream_t stream) {
   A.queue = nullptr;
   if (ws) {
     int* wi = static_cast<int*>(ws);
@@ -591,18 +617,24 @@ int dispatch_ls(LsArgs<T>& A, int nmax, int block, void* ws, hipStream_t stream)
     A.xchg = reinterpret_cast<T*>(static_cast<char*>(ws) + kWsHeader + (size_t)A.P * 64);
     A.xstride = xchg_stride((int)A.stride);
   }
-  switch (block) {
-    case 256: return launch_ls<T, 256, 16>(A, nmax, stream);
-    case 512:
-      if constexpr (sizeof(T) == 4) {
-        if (nmax <= 512 * 8) return launch_ls<T, 512, 8, true>(A, nmax, stream);
-        return launch_ls<T, 512, 20>(A, nmax, stream);
-      }
-      return launch_ls<T, 512, 12>(A, nmax, stream);
-    case 1024:
-      if constexpr (sizeof(T) == 4) return launch_ls<T, 1024, 10>(A, nmax, stream);
-      return SIDON_ERR_UNSUPPORTED;
-    default: return SIDON_ERR_UNSUPPORTED;
+  if constexpr (sizeof(T) == 8) {  // fp64: fixed configurations, two steps in flight
+    switch (tile.block) {
+      case 256: return launch_ls<T, 256, 16, false, 2>(A, nmax, stream);
+      case 512: return launch_ls<T, 512, 12, false, 2>(A, nmax, stream);
+      default: return SIDON_ERR_UNSUPPORTED;
+    }
+  } else {
+    switch (tile.block) {
+      case 256: return launch_ls_depth<256, 16, false>(A, nmax, tile.depth, stream);
+      case 512:
+        if (tile.per_cu == 2) {
+          if (nmax > 512 * 8) return SIDON_ERR_UNSUPPORTED;
+          return launch_ls_depth<512, 8, true>(A, nmax, tile.depth, stream);
+        }
+        return launch_ls_depth<512, 20, false>(A, nmax, tile.depth, stream);
+      case 1024: return launch_ls_depth<1024, 10, false>(A, nmax, tile.depth, stream);
+      default: return SIDON_ERR_UNSUPPORTED;
+    }
   }
 }
 
@@ -648,21 +680,39 @@ int launch_grad(const GradArgs<T>& A, hipStream_t stream) {
 
 using namespace sidon;
 
-extern "C" int sidon_set_ls_block(int n, int block) {
-  if (block != 0 && block != 256 && block != 512 && block != 1024) return SIDON_ERR_ARG;
+extern "C" int sidon_set_ls_tile(int n, int block, int depth, int per_cu) {
+  const bool clear = block == 0;
+  const LsTile tile{block, depth, per_cu};
+  if (!clear && !tile_valid(tile)) return SIDON_ERR_ARG;
   for (int r = 0; r < g_nrules; ++r)
     if (g_rules[r].n == n) {
-      if (block == 0) { g_rules[r] = g_rules[--g_nrules]; return SIDON_OK; }
-      g_rules[r].block = block;
+      if (clear) { g_rules[r] = g_rules[--g_nrules]; return SIDON_OK; }
+      g_rules[r].tile = tile;
       return SIDON_OK;
     }
-  if (block == 0) return SIDON_OK;
+  if (clear) return SIDON_OK;
   if (g_nrules >= 64) return SIDON_ERR_ARG;
-  g_rules[g_nrules++] = BlockRule{n, block};
+  g_rules[g_nrules++] = TileRule{n, tile};
   return SIDON_OK;
 }
 
-extern "C" int sidon_get_ls_block(int n) { return pick_block(n); }
+extern "C" int sidon_get_ls_tile(int n, int* block, int* depth, int* per_cu) {
+  const LsTile t = pick_tile(n);
+  if (block) *block = t.block;
+  if (depth) *depth = t.depth;
+  if (per_cu) *per_cu = t.per_cu;
+  return SIDON_OK;
+}
+
+extern "C" int sidon_set_ls_block(int n, int block) {
+  if (block == 0) return sidon_set_ls_tile(n, 0, 0, 0);
+  if (block != 256 && block != 512 && block != 1024) return SIDON_ERR_ARG;
+  // the built-in depth and sharing for that workgroup size
+  const int per_cu = block == 512 && n <= 4096 ? 2 : 1;
+  return sidon_set_ls_tile(n, block, 4, per_cu);
+}
+
+extern "C" int sidon_get_ls_block(int n) { return pick_tile(n).block; }
 
 extern "C" int sidon_local_search(float* pop, float* mom, float* vel, const int* skip,
                                   const int* mode, int n, int P, int iters, float pStart,
@@ -673,7 +723,7 @@ extern "C" int sidon_local_search(float* pop, float* mom, float* vel, const int*
   if (P == 0 || iters == 0) return SIDON_OK;
   LsArgs<float> A{pop, n, nullptr, n, P, mom, vel, skip, mode, modeDefault, nullptr, nullptr,
                   iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2};
-  return dispatch_ls<float>(A, n, pick_block(n), nullptr, stream);
+  return dispatch_ls<float>(A, n, pick_tile(n), nullptr, stream);
 }
 
 extern "C" int sidon_local_search_f64(double* pop, double* mom, double* vel, const int* skip,
@@ -685,8 +735,8 @@ extern "C" int sidon_local_search_f64(double* pop, double* mom, double* vel, con
   if (P == 0 || iters == 0) return SIDON_OK;
   LsArgs<double> A{pop, n, nullptr, n, P, mom, vel, skip, mode, modeDefault, nullptr, nullptr,
                    iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2};
-  const int block = n >= 1024 ? 512 : 256;
-  return dispatch_ls<double>(A, n, block, nullptr, stream);
+  const LsTile tile{n >= 1024 ? 512 : 256, 2, 1};
+  return dispatch_ls<double>(A, n, tile, nullptr, stream);
 }
 
 extern "C" size_t sidon_local_search_ragged_workspace(int P, int stride) {
@@ -712,7 +762,7 @@ extern "C" int sidon_local_search_ragged(float* pop, int stride, const int* ns, 
   LsArgs<float> A{pop, stride, ns, nmax, P, nullptr, nullptr, skip, mode, modeDefault, order,
                   nullptr, iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2,
                   split, nullptr, nullptr, nullptr, nullptr, 0};
-  return dispatch_ls<float>(A, nmax, pick_block(nmax), workspace, stream);
+  return dispatch_ls<float>(A, nmax, pick_tile(nmax), workspace, stream);
 }
 
 extern "C" int sidon_lp_grad(const float* a, int n, int B, float p, float* J, float* grad,

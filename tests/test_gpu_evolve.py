@@ -220,9 +220,19 @@ def test_generation_graph_matches_eager(S, W):
     g.close()
 
 
-def test_autotuner_picks_a_block(S):
+def test_autotuner_picks_a_configuration(S):
+    """The tuner times every configuration that fits n, keeps the built-in one
+    unless another beats it by more than 1%, and leaves its choice as n's rule."""
     print("[evolve] autotuner", flush=True)
-    blk, ms = S.tune_local_search(600, P=16, iters=5, repeats=1)
-    assert blk in (256, 512, 1024) and ms > 0
-    assert S.get_ls_block(600) == blk
-    S.set_ls_block(600, 0)
+    builtin = S.get_ls_tile(600)
+    tile, ms, table = S.tune_local_search(600, P=16, iters=5, repeats=1)
+    try:
+        assert ms > 0 and tile in table and table[tile] == ms
+        assert len(table) == S.LS_TILES        # every configuration fits n = 600
+        assert min(table.values()) <= ms
+        if tile != builtin:
+            assert ms < 0.99 * table[builtin]
+        assert S.get_ls_tile(600) == tile
+    finally:
+        S.set_ls_tile(600, 0)
+    assert S.get_ls_tile(600) == builtin

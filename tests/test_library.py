@@ -48,6 +48,9 @@ def test_argument_validation_without_device():
     assert L.sidon_select(fake, None, None, 0, 10, 3, 1, 0, 0.05, 1, fake, fake, fake, fake,
                           fake, None) == S.SIDON_ERR_ARG  # P not divisible by NI
     assert L.sidon_set_ls_block(4096, 300) == S.SIDON_ERR_ARG
+    assert L.sidon_set_ls_tile(4096, 512, 3, 1) == S.SIDON_ERR_ARG    # depth 2 or 4
+    assert L.sidon_set_ls_tile(4096, 1024, 4, 2) == S.SIDON_ERR_ARG   # two a CU: 512 only
+    assert L.sidon_ls_tile_config(S.LS_TILES, None, None, None) == S.SIDON_ERR_ARG
     # the persistent launch needs a caller-owned workspace of the stated size
     assert L.sidon_local_search_ragged(fake, 64, fake, 4, 64, None, None, None, 10, 4.0, 8.0,
                                        0.1, 0.01, 0.9, 0.999, 0, 1, None, 0,
@@ -101,3 +104,29 @@ def test_generation_desc_matches_header():
             want.append((name, kinds[base + ("*" if ptr else "")]))
     got = [(f[0], f[1]) for f in S.GenerationDesc._fields_]
     assert got == want
+
+
+def test_tile_rules_are_host_state():
+    """The launch-configuration rules live on the host: the built-in rule, an
+    override per length, and its removal (no GPU call)."""
+    S = importlib.import_module("sidon-autocorrelation-bound_amd")
+    assert S.get_ls_tile(300) == (256, 4, 1)
+    assert S.get_ls_tile(2100) == (512, 4, 2)
+    assert S.get_ls_tile(8400) == (1024, 4, 1)
+    S.set_ls_tile(2100, 1024, 2, 1)
+    try:
+        assert S.get_ls_tile(2100) == (1024, 2, 1)
+        assert S.get_ls_block(2100) == 1024
+        assert S.get_ls_tile(2101) == (512, 4, 2)
+    finally:
+        S.set_ls_tile(2100, 0)
+    assert S.get_ls_tile(2100) == (512, 4, 2)
+    S.set_ls_block(600, 512)                  # keeps the built-in depth and sharing
+    try:
+        assert S.get_ls_tile(600) == (512, 4, 2)
+    finally:
+        S.set_ls_block(600, 0)
+    tiles = [S.ls_tile_config(i) for i in range(S.LS_TILES)]
+    assert len(set(tiles)) == S.LS_TILES
+    for n in (300, 600, 2100, 4096, 8400):      # the built-in rule is a trial
+        assert S.get_ls_tile(n) in tiles
