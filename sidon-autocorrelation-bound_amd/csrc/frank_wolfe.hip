@@ -84,18 +84,24 @@ __global__ __launch_bounds__(BD) void k_frank_wolfe(FwArgs A) {
 
   for (int i = tid; i < LA; i += BD) { sA[i] = 0.0f; sH[i] = 0.0f; }
   for (int i = tid; i < LC; i += BD) sL[i] = 0.0f;
-  if (tid == 0) {  // longest-processing-time deal of Toeplitz tiles to waves
-    int load[NW];
+  if (tid == 0) {
+    // longest-processing-time deal of Toeplitz tiles: each to the SIMD with
+    // the least work (waves w, w + 4, w + 8, ... of a workgroup share SIMD
+    // w % 4 and its matrix core), then to that SIMD's least-loaded wave
+    int load[NW], sload[4] = {0, 0, 0, 0};
     for (int w = 0; w < NW; ++w) load[w] = 0;
     for (int j = 0; j < NTc; ++j) owner[j] = -1;
     for (int r = 0; r < NTc; ++r) {
       int best = -1, bc = -1;
       for (int j = 0; j < NTc; ++j)
         if (owner[j] < 0 && conv_cost(j, N16) > bc) { bc = conv_cost(j, N16); best = j; }
-      int wmin = 0;
-      for (int w = 1; w < NW; ++w) if (load[w] < load[wmin]) wmin = w;
+      int smin = 0;
+      for (int q = 1; q < 4; ++q) if (sload[q] < sload[smin]) smin = q;
+      int wmin = smin;
+      for (int w = smin + 4; w < NW; w += 4) if (load[w] < load[wmin]) wmin = w;
       owner[best] = wmin;
       load[wmin] += bc;
+      sload[smin] += bc;
     }
   }
   __syncthreads();
