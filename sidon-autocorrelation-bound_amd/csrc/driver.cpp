@@ -430,18 +430,45 @@ int uneven(const Args& A) {
   hipEvent_t e0, e1;
   CHECK_HIP(hipEventCreate(&e0));
   CHECK_HIP(hipEventCreate(&e1));
-  unsigned long long block_steps = 0;
+  unsigned long long block_steps = 0, total_steps = 0;
   CHECK_SIDON(sidon_ls_block_steps(&block_steps, 1));
-  CHECK_HIP(hipEventRecord(e0, s));
-  CHECK_SIDON(sidon_generation_run(g, A.gens, s));
-  CHECK_HIP(hipEventRecord(e1, s));
-  CHECK_HIP(hipEventSynchronize(e1));
-  CHECK_HIP(hipStreamSynchronize(s));
+  // one generation at a time: its time (HIP events around the graph replay), the matrix
+  // work its local search executed, and the lengths of the population it searched
   float ms = 0.0f;
-  CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+  std::string per_gen;
+  std::vector<int> cur(P);
+  for (int gi = 0; gi < A.gens; ++gi) {
+    int parity = 0;
+    CHECK_SIDON(sidon_generation_current(g, &parity));
+    CHECK_HIP(hipMemcpy(cur.data(), parity ? d.ns_b : d.ns_a, P * sizeof(int),
+                        hipMemcpyDeviceToHost));
+    CHECK_HIP(hipEventRecord(e0, s));
+    CHECK_SIDON(sidon_generation_run(g, 1, s));
+    CHECK_HIP(hipEventRecord(e1, s));
+    CHECK_HIP(hipEventSynchronize(e1));
+    float gms = 0.0f;
+    CHECK_HIP(hipEventElapsedTime(&gms, e0, e1));
+    CHECK_SIDON(sidon_ls_block_steps(&block_steps, 1));
+    total_steps += block_steps;
+    ms += gms;
+    int split_n = 0;  // candidates this generation's local search split (byte 8 of its workspace)
+    CHECK_HIP(hipMemcpy(&split_n, static_cast<char*>(d.ls_workspace) + 8, sizeof split_n,
+                        hipMemcpyDeviceToHost));
+    char buf[512];
+    int off = std::snprintf(buf, sizeof buf, "%s{\"ms\": %.3f, \"executed_frac_fp32_peak\": %.4f, "
+                            "\"split\": %d, \"lengths\": {", gi ? ", " : "", gms,
+                            8192.0 * (double)block_steps / (gms / 1e3) / 1e12 / 157.3, split_n);
+    for (size_t li = 0; li < A.lengths.size(); ++li) {
+      const int L = A.lengths[li];
+      const int c = (int)std::count(cur.begin(), cur.end(), L);
+      off += std::snprintf(buf + off, sizeof buf - off, "%s\"%d\": %d", li ? ", " : "", L, c);
+    }
+    std::snprintf(buf + off, sizeof buf - off, "}}");
+    per_gen += buf;
+  }
+  CHECK_HIP(hipStreamSynchronize(s));
   // matrix work the local searches executed (block steps of 4 x 2048 FLOP)
-  CHECK_SIDON(sidon_ls_block_steps(&block_steps, 1));
-  const double flop = 8192.0 * (double)block_steps / A.gens;
+  const double flop = 8192.0 * (double)total_steps / A.gens;
   std::vector<double> fit(P);
   CHECK_HIP(hipMemcpy(fit.data(), d.fit, P * sizeof(double), hipMemcpyDeviceToHost));
   const double best = *std::min_element(fit.begin(), fit.end());
@@ -453,10 +480,10 @@ int uneven(const Args& A) {
               "generation, HIP graph\", \"generations\": %d, \"ms_per_generation\": %.3f, "
               "\"candidate_steps_per_s\": %.1f, \"local_search_executed_tflops\": %.2f, "
               "\"executed_frac_fp32_peak\": %.4f, \"split\": %d, \"pair_timeouts\": %d, "
-              "\"best_F\": %.12f}\n",
+              "\"best_F\": %.12f, \"per_generation\": [%s]}\n",
               P, A.islands, A.iters, A.sigma, A.gens, ms / A.gens,
               (double)steps * A.gens / (ms / 1e3), flop * A.gens / (ms / 1e3) / 1e12,
-              flop * A.gens / (ms / 1e3) / 1e12 / 157.3, A.split, timeouts, best);
+              flop * A.gens / (ms / 1e3) / 1e12 / 157.3, A.split, timeouts, best, per_gen.c_str());
   CHECK_SIDON(sidon_generation_destroy(g));
   return 0;
 }
