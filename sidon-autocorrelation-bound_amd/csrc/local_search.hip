@@ -325,48 +325,65 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
 }
 
 // How many of the longest candidates a persistent launch splits over two
-// workgroups.  With costs t_i (a candidate's time on one workgroup, modelled
-// as n (n + kLinCost), fitted to measured per-length times of the
+// workgroups.  A candidate's time on one workgroup is modelled as
+// t = n (n + kLinCost) (fitted to measured per-length times of the
 // 1024-thread kernel: the O(n) phases and the few work items of short
-// candidates weigh more than n^2 says) sorted longest first, W = sum t_i and
-// G workgroups, splitting the s longest gives a makespan of about
-//   max(c t_0 / 2, t_s, (W + (c - 1)(t_0 + ... + t_{s-1})) / G),
-// c the measured cost of a pair relative to one workgroup (the exchanges,
-// and half as many work items per workgroup); s minimises it (s = 0 unless a
-// split helps), among candidates of at least kSplitMinN.  Uniform lengths
-// with at least one candidate per workgroup split none; a few long
-// candidates among many short ones (uneven islands) split the long ones.
-// `order` lists the candidates longest first, so they are its first
-// entries.  Every workgroup computes the same count (thread 0, serially).
+// candidates weigh more than n^2 says), a half of a split candidate as
+// h = c t / 2, c the measured cost of a pair relative to one workgroup (the
+// exchanges, and half as many work items per workgroup).  Only candidates of
+// the longest length n0 (the first L entries of `order`, longest first) are
+// split, and only if n0 >= kSplitMinN.  Workgroups take tickets in order: the
+// 2s halves first, then the L - s whole long candidates, then the rest.  On G
+// workgroups the halves leave r = 2s mod G workgroups one half later than the
+// others, and the whole long candidates go round-robin to the earliest free,
+// so the long work ends at
+//   M = max(a h + (k + [q > 0]) t0, [r > 0] ((a + 1) h + (k + [q > G - r]) t0)),
+// a = floor(2s / G), k = floor((L - s) / G), q = (L - s) mod G; the rest of
+// the work W_r fills the idle time before M and only the excess, spread over
+// G, adds to it.  s minimises that makespan (smallest s on ties).  More long
+// candidates than workgroups (L > G) then split just enough to end the
+// second round of whole candidates with the first halves (s = L - G);
+// uniform lengths with one candidate per workgroup or more split none; a few
+// long candidates among many short ones split them all.  Every workgroup
+// computes the same count (thread 0, serially, O(P)).
 constexpr int kSplitMinN = 4096;
 constexpr double kLinCost = 11000.0;
 constexpr double kPairCost = 1.5;
+__host__ __device__ inline double split_makespan(int s, int L, int G, double t0, double h,
+                                                 double rest) {
+  const long long A = 2LL * s, B = (long long)L - s;
+  const long long a = A / G, r = A % G, k = B / G, q = B % G;
+  double M = (double)a * h + (double)(k + (q > 0 ? 1 : 0)) * t0;
+  if (r > 0) {
+    const double m2 = (double)(a + 1) * h + (double)(k + (q > G - r ? 1 : 0)) * t0;
+    if (m2 > M) M = m2;
+  }
+  const double idle = (double)G * M - ((double)A * h + (double)B * t0);
+  if (rest > idle) M += (rest - idle) / (double)G;
+  return M;
+}
 template <typename T>
 __device__ int split_count(const LsArgs<T>& A, int* s_out) {
   if (!A.split || !A.order || !A.ns) return 0;
   if (threadIdx.x == 0) {
-    double W = 0.0;
-    for (int p = 0; p < A.P; ++p) {
-      const double n = (double)A.ns[p];
-      W += n * (n + kLinCost);
-    }
-    const double G = (double)gridDim.x;
-    const double n0 = (double)A.ns[A.order[0]];
-    const double t0 = n0 * (n0 + kLinCost);
-    double best = t0 > W / G ? t0 : W / G;
+    const int n0 = A.ns[A.order[0]];
     int bs = 0;
-    double pre = 0.0;
-    for (int s = 1; s <= A.P; ++s) {
-      const double ns1 = (double)A.ns[A.order[s - 1]];
-      if (ns1 < kSplitMinN) break;
-      pre += ns1 * (ns1 + kLinCost);
-      const double next = s < A.P ? (double)A.ns[A.order[s]] : 0.0;
-      double m = kPairCost * t0 / 2.0;
-      const double tn = next * (next + kLinCost);
-      if (tn > m) m = tn;
-      const double avg = (W + (kPairCost - 1.0) * pre) / G;
-      if (avg > m) m = avg;
-      if (m < best) { best = m; bs = s; }
+    if (n0 >= kSplitMinN) {
+      const double t0 = (double)n0 * ((double)n0 + kLinCost);
+      int L = 0;
+      double rest = 0.0;
+      for (int p = 0; p < A.P; ++p) {
+        const int n = A.ns[A.order[p]];
+        if (n == n0) ++L;
+        else rest += (double)n * ((double)n + kLinCost);
+      }
+      const int G = (int)gridDim.x;
+      const double h = kPairCost * t0 / 2.0;
+      double best = split_makespan(0, L, G, t0, h, rest);
+      for (int sc = 1; sc <= L; ++sc) {
+        const double m = split_makespan(sc, L, G, t0, h, rest);
+        if (m < best) { best = m; bs = sc; }
+      }
     }
     *s_out = bs;
   }

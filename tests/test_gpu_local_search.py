@@ -432,3 +432,32 @@ def test_split_pairs_equal_one_workgroup_bitwise(S, W):
         out[split] = t
     assert torch.equal(out[0], out[1])
     assert not torch.equal(out[1], dev(pop))
+
+
+def test_split_more_long_candidates_than_workgroups(S, W):
+    """320 candidates at n = 8400 on 256 one-candidate-a-CU workgroups: whole
+    candidates would need two rounds, so the makespan model splits L - G = 64
+    of them (their 128 halves end with the first round's whole candidates and
+    free the workgroups the last 64 whole candidates need); the result is
+    bit-identical to the unsplit launch."""
+    print("[ls] split with more long candidates than workgroups", flush=True)
+    P, n = 320, 8400
+    G = torch.cuda.get_device_properties(0).multi_processor_count
+    if S.lds_bytes(0, n, 1024) * 2 <= 160 * 1024 or G >= P:
+        pytest.skip("needs one n = 8400 workgroup a CU and fewer CUs than candidates")
+    pop = W.structured_prior(P, n, seed=77)
+    nsd = dev(np.full(P, n, np.int32), torch.int32)
+    order = torch.empty(P, dtype=torch.int32, device="cuda")
+    S.order_by_length(nsd, order)
+    kw = dict(iters=20, p_start=4.0, p_end=1e4, lr_start=0.085, lr_end=0.0015, order=order)
+    out = {}
+    for split in (0, 1):
+        t = dev(pop)
+        ws = torch.empty(S.ragged_workspace_bytes(P, n), dtype=torch.uint8, device="cuda")
+        S.local_search_ragged(t, nsd, n, split=split, workspace=ws, **kw)
+        torch.cuda.synchronize()
+        assert S.pair_timeouts(ws) == 0
+        expect = P - G if split else 0
+        assert S.split_count(ws) == expect
+        out[split] = t
+    assert torch.equal(out[0], out[1])
