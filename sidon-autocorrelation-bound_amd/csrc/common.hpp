@@ -480,7 +480,11 @@ __device__ __forceinline__ int autoconv_cost(int j, int N16) {
 // Toeplitz steps d in [da, db) of autoconvolution tile K0 = 16j.  In the
 // zone d >= 8j, column t keeps its B operand only while d < dp(K0 + t) =
 // 8j + ceil(t/2); a lane past that reads four zeros from the left padding
-// instead, so zone steps run in the same pipelined loop.
+// instead.  The steps before the zone (all columns live) run as one
+// pipelined sequence with constant-stride addresses, the zone steps (at most
+// eight) as a second one with the per-lane select; the second continues the
+// first's even / odd accumulator assignment, so the sums are the ones of a
+// single sequence over [da, db).
 template <typename T, int D = Pipe<T>::kDepth, int SS = kSegStride>
 __device__ __forceinline__ typename Acc4<T>::type autoconv_tile(const T* __restrict__ sA, int K0,
                                                                int da, int db) {
@@ -489,13 +493,29 @@ __device__ __forceinline__ typename Acc4<T>::type autoconv_tile(const T* __restr
   const int t = lane & 15, kk = lane >> 4;
   const int pa = pad_addr<SS>(16 * da + t - 4 * kk);
   const int pb = pad_addr<SS>(16 * (K0 + t - da)) + 4 * kk;
-  const int live = (K0 >> 1) + ((t + 1) >> 1) - da;  // steps i < live use this column
-  const int zero = pad_addr<SS>(-kPadSeg * kSeg);       // first padding segment: always zero
-  return run_steps<T, D>(db - da, [&](int i, T (&x)[4], T (&y)[4]) {
+  const int dz = K0 >> 1;                               // first zone offset, 8j
+  const int za = da > dz ? da : (dz < db ? dz : db);    // zone steps are [za, db)
+  const int na = za - da;                               // steps before the zone
+  typename Acc4<T>::type acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+  if (na > 0)
+    run_steps_into<T, D>(na, [&](int i, T (&x)[4], T (&y)[4]) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) x[q] = sA[pa + i * S - q];
-    load4(sA + (i < live ? pb - i * S : zero), y);
-  });
+      for (int q = 0; q < 4; ++q) x[q] = sA[pa + i * S - q];
+      load4(sA + pb - i * S, y);
+    }, acc0, acc1);
+  if (db > za) {
+    const int qa = pa + na * S, qb = pb - na * S;
+    const int live = dz + ((t + 1) >> 1) - za;          // zone steps i < live use this column
+    const int zero = pad_addr<SS>(-kPadSeg * kSeg);     // first padding segment: always zero
+    auto ld = [&](int i, T (&x)[4], T (&y)[4]) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) x[q] = sA[qa + i * S - q];
+      load4(sA + (i < live ? qb - i * S : zero), y);
+    };
+    if (na & 1) run_steps_into<T, D>(db - za, ld, acc1, acc0);
+    else run_steps_into<T, D>(db - za, ld, acc0, acc1);
+  }
+  return acc0 + acc1;
 }
 
 // ------------------------------------------ dynamic, deterministic split --
