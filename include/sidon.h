@@ -157,14 +157,15 @@ int sidon_local_search_ragged(float* pop, int stride, const int* ns, int P, int 
  * candidate), `depth` operand steps in flight in the matrix-core tile loops
  * (2 or 4) and `per_cu` candidates a CU (2 only with block 512 and
  * n <= 4096: two workgroups held to 128 registers a lane share a CU; else 1).
- * block 0 restores the built-in rule (512/4/2 for 512 <= n <= 4096,
- * 1024/4/1 above, 256/4/1 below).  Set by the autotuner.  The rule is
+ * block 0 restores the built-in rule (512/2/2 for 512 <= n < 1024,
+ * 512/4/2 up to 4096, 1024/4/1 above, 256/4/1 below).  Set by the autotuner.  The rule is
  * process-wide host state, read when a launch is enqueued (a captured graph
  * keeps the configuration it was captured with). */
 int sidon_set_ls_tile(int n, int block, int depth, int per_cu);
 int sidon_get_ls_tile(int n, int* block, int* depth, int* per_cu);
-/* The workgroup-size-only forms: set_ls_block keeps the built-in depth and
- * sharing for that size. */
+/* The workgroup-size-only forms: set_ls_block takes the built-in
+ * configuration when it has that workgroup size, else depth 4 with two
+ * candidates a CU where 512 threads allow it. */
 int sidon_set_ls_block(int n, int block);
 int sidon_get_ls_block(int n);
 

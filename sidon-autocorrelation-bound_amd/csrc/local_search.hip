@@ -521,11 +521,14 @@ TileRule g_rules[64];
 int g_nrules = 0;
 
 // The built-in rule: 512 threads from n = 512 up to 4096, two candidates a
-// CU (measured faster than 256 and 1024 at n = 600 ... 4096); 1024 above,
-// where one candidate fills a CU's LDS; four operand steps in flight.
+// CU (measured faster than 256 and 1024 at n = 600 ... 4096), four operand
+// steps in flight from n = 1024 and two below (at n = 600 the register-capped
+// four-step form spills and runs 3 % slower); 1024 threads above 4096, where
+// one candidate fills a CU's LDS.
 LsTile default_tile(int n) {
   if (n > 4096) return LsTile{1024, 4, 1};
-  if (n >= 512) return LsTile{512, 4, 2};
+  if (n >= 1024) return LsTile{512, 4, 2};
+  if (n >= 512) return LsTile{512, 2, 2};
   return LsTile{256, 4, 1};
 }
 
@@ -724,7 +727,10 @@ extern "C" int sidon_get_ls_tile(int n, int* block, int* depth, int* per_cu) {
 extern "C" int sidon_set_ls_block(int n, int block) {
   if (block == 0) return sidon_set_ls_tile(n, 0, 0, 0);
   if (block != 256 && block != 512 && block != 1024) return SIDON_ERR_ARG;
-  // the built-in depth and sharing for that workgroup size
+  // the built-in configuration when it has this workgroup size, else four
+  // steps in flight and two candidates a CU where 512 threads allow it
+  const LsTile d = default_tile(n);
+  if (d.block == block) return sidon_set_ls_tile(n, d.block, d.depth, d.per_cu);
   const int per_cu = block == 512 && n <= 4096 ? 2 : 1;
   return sidon_set_ls_tile(n, block, 4, per_cu);
 }

@@ -121,9 +121,15 @@ def test_tile_rules_are_host_state():
     finally:
         S.set_ls_tile(2100, 0)
     assert S.get_ls_tile(2100) == (512, 4, 2)
-    S.set_ls_block(600, 512)                  # keeps the built-in depth and sharing
+    assert S.get_ls_tile(600) == (512, 2, 2)
+    S.set_ls_block(600, 512)                  # the built-in configuration of that size
     try:
-        assert S.get_ls_tile(600) == (512, 4, 2)
+        assert S.get_ls_tile(600) == (512, 2, 2)
+    finally:
+        S.set_ls_block(600, 0)
+    S.set_ls_block(600, 1024)
+    try:
+        assert S.get_ls_tile(600) == (1024, 4, 1)
     finally:
         S.set_ls_block(600, 0)
     tiles = [S.ls_tile_config(i) for i in range(S.LS_TILES)]
