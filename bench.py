@@ -58,7 +58,7 @@ def parse():
                     help="comma-separated lengths: run the autotuner at each (P candidates "
                          "of that length, 50-step trials) and print its tables")
     ap.add_argument("--fw-only", action="store_true",
-                    help="time the Frank-Wolfe / Chambolle-Pock kernel alone (config 5's "
+                    help="time the Frank-Wolfe / Chambolle-Pock kernel alone (the Frank-Wolfe ladder's "
                          "levels) and print its JSON")
     ap.add_argument("--polish-only", action="store_true",
                     help="time the whole-device fp64 polish alone and print its JSON")
@@ -191,7 +191,7 @@ def f_evals(S, W, torch, with_cpu: bool, seed: int):
 
 def fw_legs(S, W, torch, seed: int, rounds: int = 1, cp_iters: int = 200, P: int = 256):
     """Frank-Wolfe rounds solved by Chambolle-Pock over P structured-prior
-    candidates at config 5's levels n = 1050, 2100, 4200, 8400 (a bounded
+    candidates at the Frank-Wolfe ladder's levels n = 1050, 2100, 4200, 8400 (a bounded
     sample: `rounds` rounds of `cp_iters` iterations, not the 50 x 2000 of a
     full level), HIP events around the launch.  Matrix work the kernel ran is
     its block-step count (4 x 2048 FLOP each); dense-equivalent work is 4 n^2

@@ -823,7 +823,7 @@ static inline int rank_less(const double* fit, int x, int y) {
   return fit[x] < fit[y] || (fit[x] == fit[y] && x < y);
 }
 
-/* main.cu:676-735 with op fixed to 2 (the configs' Gaussian multiplicative
+/* main.cu:676-735 with op fixed to 2 (the uneven-islands workload's Gaussian multiplicative
  * mutation).  Tournament draws come from Philox keyed by (seed, gen, slot). */
 void so_select_plan(double* fit, int P, int NI, int elite, uint64_t seed, uint64_t gen,
                     int migrate_every, float sigma, int* p1, int* p2, int* op,

@@ -145,7 +145,7 @@ int so_op_uses_mean(int op);
 void so_support_sig(const float* a, int n, float tol, uint64_t* sig, int* nnz);
 uint64_t so_sig_mix(int i);
 
-/* Island selection plan (main.cu:676-735 with the configs' fixed operator):
+/* Island selection plan (main.cu:676-735 with the uneven-islands workload's fixed operator):
  * per island rank by fp64 fit (ties by index), ring migration every
  * migrate_every generations (best of island i replaces the worst of island
  * i+1; fit and the slot's vector follow), elites cloned (op 20), the rest

@@ -123,7 +123,7 @@ def test_unreadable_resume_is_ignored(tmp_path):
 
 
 def test_uneven_islands_graph():
-    """Config 3 in small: uneven lengths, generations replayed from a HIP graph."""
+    """Uneven islands in small: uneven lengths, generations replayed from a HIP graph."""
     out = run(["--uneven", "--P", "64", "--islands", "4", "--iters", "40", "--gens", "2",
                "--lengths", "128,256,600", "--no-tune"])
     rec = json.loads(out.strip().splitlines()[-1])
@@ -132,7 +132,7 @@ def test_uneven_islands_graph():
 
 
 def test_ladder_splits_preserve_F_exactly():
-    """Config 5 in small: Frank-Wolfe rounds per level, every n -> 2n split keeps
+    """The Frank-Wolfe ladder in small: Frank-Wolfe rounds per level, every n -> 2n split keeps
     the correctly rounded F bit for bit (and certifies the rounding)."""
     out = run(["--fw-ladder", "--pop", "32", "--ladder", "256,512,1024", "--fw-rounds", "1",
                "--fw-iters", "200"])

@@ -43,7 +43,7 @@ def test_plain_config_vectors_bitwise(S, O, W):
 
 
 def test_plain_config_unrounded_fp64(S, O, W):
-    """Config 1 as stated: 64 x 512 fp64 vectors (a_i = 0 w.p. 0.33 else
+    """The plain workload as stated: 64 x 512 fp64 vectors (a_i = 0 w.p. 0.33 else
     U(0,1)), not rounded to fp32, through the fp64-input evaluation
     (kEval<double,double>).  fp64 products are not exact, and the kernel adds
     each with one fused rounding where hostF rounds the product first, so F

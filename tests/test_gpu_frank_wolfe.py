@@ -36,7 +36,7 @@ def test_frank_wolfe_vs_oracle(S, O, W, n):
 
 @pytest.mark.parametrize("n", [4200, 8400])
 def test_frank_wolfe_vs_oracle_large_n(S, O, W, n):
-    """Config 5's levels 4200 and 8400: one Frank-Wolfe round of 50
+    """The ladder's levels 4200 and 8400: one Frank-Wolfe round of 50
     Chambolle-Pock iterations against the fp64 oracle, per height.  The GPU's
     line search compares fp32 maxima: when the oracle's best two trial steps
     are within 1e-5 of each other either choice is right, and the GPU result

@@ -21,8 +21,8 @@ def rel(a, b):
 
 
 @pytest.fixture(scope="module")
-def c4_rows(W):
-    """Config 4's batch: 64 structured-prior rows at n = 8400, each with five
+def planted_rows(W):
+    """The adjoint check's batch: 64 structured-prior rows at n = 8400, each with five
     b_k planted within 1e-6 (relative, fp64, after rounding to fp32) of the
     max; rows where planting reached fewer than five ties are passed over."""
     base = W.structured_prior(96, 8400, seed=77)
@@ -41,15 +41,15 @@ def c4_rows(W):
 # measured over the 64 rows on MI355X (2.1e-6, 4.1e-5, 6.4e-4): fp32 b carries
 # ~1e-7 relative error, which moves (b/M)^(p-1) near the planted ties by
 # ~(p-1) 1e-7
-C4_BOUND = {8.0: 7e-6, 64.0: 1.2e-4, 1024.0: 2e-3}
+ADJOINT_BOUND = {8.0: 7e-6, 64.0: 1.2e-4, 1024.0: 2e-3}
 
 
 @pytest.mark.parametrize("p", [8.0, 64.0, 1024.0])
-def test_adjoint_planted_ties_n8400(S, O, c4_rows, p):
+def test_adjoint_planted_ties_n8400(S, O, planted_rows, p):
     """Adjoint check at n = 8400, batch 64, through the autograd harness: J_p to
     1e-6 and every gradient component against the fp64 oracle, per height."""
     print(f"[grad] n=8400 batch 64 p={p}", flush=True)
-    A = c4_rows
+    A = planted_rows
     t = dev(A).requires_grad_(True)
     J = S.lp_objective(t, p)
     J.sum().backward()
@@ -62,7 +62,7 @@ def test_adjoint_planted_ties_n8400(S, O, c4_rows, p):
         assert abs(Jg[r] - Jr) < 1e-6 * Jr, (r, Jg[r], Jr)
         e = float(np.abs(g[r] - gr).max() / np.abs(gr).max())
         worst = max(worst, e)
-        assert e < C4_BOUND[p], (r, e)
+        assert e < ADJOINT_BOUND[p], (r, e)
     print(f"   worst per-height error over 64 rows: {worst:.3e} of max|g|")
 
 

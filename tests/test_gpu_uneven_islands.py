@@ -1,4 +1,4 @@
-"""Config 3 at its real shape: 512 candidates with n drawn uniformly from
+"""Uneven islands at full size: 512 candidates with n drawn uniformly from
 {600, 1200, 2100, 4096, 8400}, 8 islands of 64, a generation = 200 fused
 projected-Adam steps + log-normal mutation (sigma = 0.05) + binary tournaments
 on fp64 F, replayed from the captured HIP graphs.
@@ -53,7 +53,7 @@ def start(W):
     return pop, ns
 
 
-def test_config3_lengths(start, W):
+def test_uneven_lengths(start, W):
     _, ns = start
     assert set(ns.tolist()) == set(W.UNEVEN_NS) and ns.max() == 8400
 
@@ -118,14 +118,14 @@ class Eager:
 @pytest.fixture(scope="module")
 def runs(S, start):
     pop, ns = start
-    print("[config3] eager generations, stage by stage", flush=True)
+    print("[uneven] eager generations, stage by stage", flush=True)
     eager = Eager(S, pop, ns)
     recs = {}
     for g in range(GENS):
         recs[g] = eager.generation(record=g in CHECKED)
         torch.cuda.synchronize()
         print(f"   generation {g} done", flush=True)
-    print("[config3] graph replays", flush=True)
+    print("[uneven] graph replays", flush=True)
     graph = S.Generation(dev(pop), dev(ns, torch.int32), islands=NI, iters=ITERS, sigma=SIGMA,
                          elite=ELITE, migrate_every=MIG, seed=SEED, graph=True, **SCHED)
     for g in range(GENS):
@@ -135,7 +135,7 @@ def runs(S, start):
     return eager, graph, recs
 
 
-def test_config3_graph_equals_eager(runs):
+def test_uneven_graph_equals_eager(runs):
     """(Generation 0 splits its longest candidates over two workgroups when
     the makespan model says so; no pair wait gives up.)"""
     eager, graph, recs = runs
@@ -153,7 +153,7 @@ def _classes(ns):
     return [int(np.flatnonzero(ns == n)[0]) for n in sorted(set(ns.tolist()))]
 
 
-def test_config3_local_search_vs_oracle(runs, O):
+def test_uneven_local_search_vs_oracle(runs, O):
     """200 steps of the anneal, fp32 on the GPU against the fp64 oracle from
     the same start, one candidate of every length class in generation 0."""
     _, _, recs = runs
@@ -177,18 +177,18 @@ def test_config3_local_search_vs_oracle(runs, O):
         assert (got >= 0).all() and abs(got.sum() - 1) < 1e-5
     # 200 fp32 steps of the anneal to p = 1e4 are not a per-height match for
     # fp64 (a direction that passes through zero takes either Adam sign, and
-    # the trajectories part after that); their F must agree to C3_F_BOUND
+    # the trajectories part after that); their F must agree to UNEVEN_F_BOUND
     print(f"   worst F difference {worst:.2e}")
-    assert worst < C3_F_BOUND
+    assert worst < UNEVEN_F_BOUND
 
 
 # 200 fp32 steps against fp64: about 3x the worst F difference measured on
 # MI355X (4.5e-4, n = 8400)
-C3_F_BOUND = 1.5e-3
+UNEVEN_F_BOUND = 1.5e-3
 
 
 @pytest.mark.parametrize("g", CHECKED)
-def test_config3_fitness_is_hostF(runs, O, g):
+def test_uneven_fitness_is_hostF(runs, O, g):
     """fp64 F of all 512 local-search results equals hostF bit for bit."""
     _, _, recs = runs
     r = recs[g]
@@ -199,7 +199,7 @@ def test_config3_fitness_is_hostF(runs, O, g):
 
 
 @pytest.mark.parametrize("g", CHECKED)
-def test_config3_selection_plan(runs, O, g):
+def test_uneven_selection_plan(runs, O, g):
     _, _, recs = runs
     r = recs[g]
     want = O.select_plan(r["fit"], NI, ELITE, SEED, r["gen"], migrate_every=MIG, sigma=SIGMA)
@@ -219,7 +219,7 @@ def test_config3_selection_plan(runs, O, g):
 
 
 @pytest.mark.parametrize("g", CHECKED)
-def test_config3_offspring(runs, O, W, g):
+def test_uneven_offspring(runs, O, W, g):
     _, _, recs = runs
     r = recs[g]
     src, ns = r["after_sel"], r["ns_sel"]
