@@ -434,6 +434,37 @@ def test_split_pairs_equal_one_workgroup_bitwise(S, W):
     assert not torch.equal(out[1], dev(pop))
 
 
+def test_split_pairs_on_the_two_candidate_kernel(S, W):
+    """Populations whose longest length is at most 4096 run on the 512-thread
+    kernel with two candidates a CU, so the two halves of a split candidate
+    may share a CU: three n = 4096 candidates among 61 short ones split (the
+    makespan model), and the result equals the unsplit launch bit for bit."""
+    print("[ls] split pairs on the two-candidate kernel", flush=True)
+    ns = np.full(64, 600, np.int32)
+    ns[[5, 17, 40]] = 4096
+    nmax = 4096
+    assert S.get_ls_tile(nmax)[2] == 2
+    pop = np.zeros((ns.size, nmax), np.float32)
+    for p, n in enumerate(ns):
+        pop[p, :n] = W.structured_prior(1, int(n), seed=700 + p)[0]
+    nsd = dev(ns, torch.int32)
+    order = torch.empty(ns.size, dtype=torch.int32, device="cuda")
+    S.order_by_length(nsd, order)
+    kw = dict(iters=40, p_start=4.0, p_end=1e4, lr_start=0.085, lr_end=0.0015, order=order)
+    out = {}
+    for split in (0, 1):
+        t = dev(pop)
+        ws = torch.empty(S.ragged_workspace_bytes(ns.size, nmax), dtype=torch.uint8,
+                         device="cuda")
+        S.local_search_ragged(t, nsd, nmax, split=split, workspace=ws, **kw)
+        torch.cuda.synchronize()
+        assert S.pair_timeouts(ws) == 0
+        assert S.split_count(ws) == (3 if split else 0)
+        out[split] = t
+    assert torch.equal(out[0], out[1])
+    assert not torch.equal(out[1], dev(pop))
+
+
 def test_split_more_long_candidates_than_workgroups(S, W):
     """320 candidates at n = 8400 on 256 one-candidate-a-CU workgroups: whole
     candidates would need two rounds, so the makespan model splits L - G = 64
