@@ -134,6 +134,37 @@ def cpu_baseline(n: int, seconds: float, gpu_pop, gpu_F):
                       f"iterations, {dt:.1f} s"}, fcheck
 
 
+def ls_by_n(S, W, torch, seed: int, P: int = 512, iters: int = 50):
+    """The batched local search at n = 512 ... 8400: P structured-prior
+    candidates of one length, one launch of `iters` steps of the anneal
+    (p 4 -> 1e4) after a warm-up, HIP events around it; candidate-steps/s and
+    executed matrix work (the kernel's block-step count x 8192 FLOP) against
+    the FP32 peak, in the launch configuration the library picks for n."""
+    out = []
+    stream = torch.cuda.current_stream()
+    sched = (4.0, 1e4, 0.085, 0.0015)
+    for n in (512, 1024, 2048, 4096, 8400):
+        start = torch.as_tensor(W.structured_prior(P, n, seed=seed + 7 * n), device="cuda")
+        pop = start.clone()
+        S.local_search(pop, 5, *sched)                  # warm-up
+        pop.copy_(start)
+        torch.cuda.synchronize()
+        S.ls_block_steps(reset=True)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        S.local_search(pop, iters, *sched)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        sec = ev0.elapsed_time(ev1) / 1e3
+        steps = S.ls_block_steps(reset=True)
+        tf = steps * 8192.0 / sec / 1e12
+        out.append({"n": n, "P": P, "iters": iters, "ms": sec * 1e3,
+                    "candidate_steps_per_s": P * iters / sec, "executed_tflops": tf,
+                    "frac_fp32_peak": tf / PEAK_FP32,
+                    "tile": dict(zip(("block", "depth", "per_cu"), S.get_ls_tile(n)))})
+    return out
+
+
 def f_evals(S, W, torch, with_cpu: bool, seed: int):
     """fp64 F-evaluations per second at n = 512, 4096, 8400: sidon_eval_f64 over
     a batch of 512 * (8400 // n)^2 structured-prior vectors (HIP events), beside the
@@ -456,6 +487,7 @@ def main() -> int:
         cpu, fcheck = cpu_baseline(n, args.cpu_seconds, pop.cpu().numpy(), F.cpu().numpy())
         out["cpu_baseline"] = cpu
         out["f_check"] = fcheck
+    out["ls_by_n"] = ls_by_n(S, W, torch, args.seed)
     out["f_evals"] = f_evals(S, W, torch, not args.no_cpu, args.seed)
     out["host"] = host
     print(json.dumps(out))
