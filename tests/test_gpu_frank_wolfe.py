@@ -34,7 +34,11 @@ def test_frank_wolfe_vs_oracle(S, O, W, n):
             assert np.abs(got[p] - want).max() < 5e-3 * want.max()
 
 
-@pytest.mark.parametrize("n", [4200, 8400])
+# the longest candidate the Frank-Wolfe workgroup's LDS holds (20-float segments)
+FW_MAX_N = 8720
+
+
+@pytest.mark.parametrize("n", [4200, 8400, FW_MAX_N])
 def test_frank_wolfe_vs_oracle_large_n(S, O, W, n):
     """The ladder's levels 4200 and 8400: one Frank-Wolfe round of 50
     Chambolle-Pock iterations against the fp64 oracle, per height.  The GPU's
@@ -93,3 +97,13 @@ def test_skip_and_ladder(S, O, W):
             assert (amb == 0).all()
             assert torch.equal(Fr, Fe1)
             t = r
+
+
+def test_frank_wolfe_past_the_lds_is_refused(S):
+    """One height past FW_MAX_N the solve returns SIDON_ERR_LDS without a
+    launch (src/main.cu:173's "does not fit")."""
+    assert S.lds_bytes(3, FW_MAX_N, 1024) <= 160 * 1024 < S.lds_bytes(3, FW_MAX_N + 1, 1024)
+    t = torch.full((1, FW_MAX_N + 1), 1.0 / (FW_MAX_N + 1), dtype=torch.float32, device="cuda")
+    with pytest.raises(S.SidonError) as e:
+        S.frank_wolfe(t, 1, 5)
+    assert e.value.status == S.SIDON_ERR_LDS

@@ -62,8 +62,13 @@ def ls_errors(got, want, gmin):
     return float(err.max()), keep
 
 
+# the longest fp32 candidate one workgroup holds (1024 threads, 20-float segments)
+LS_MAX_N = 9296
+
+
 @pytest.mark.parametrize("n,block", [(128, None), (600, None), (2100, None), (4096, None),
-                                     (8400, None), (1200, 1024), (2100, 256)])
+                                     (8400, None), (LS_MAX_N, None), (1200, 1024),
+                                     (2100, 256)])
 def test_fp32_short_trajectory_vs_fp64(S, O, W, n, block):
     """10 fp32 steps track the fp64 oracle (same start, kernel semantics), per
     height, excluding only heights whose direction came within VANISH * M of
@@ -492,3 +497,20 @@ def test_split_more_long_candidates_than_workgroups(S, W):
         assert S.split_count(ws) == expect
         out[split] = t
     assert torch.equal(out[0], out[1])
+
+
+def test_lengths_past_the_lds_are_refused(S):
+    """One height past the longest candidate a workgroup's LDS holds, the fp32
+    and fp64 searches return SIDON_ERR_LDS before any launch (the reference's
+    "does not fit", src/main.cu:173), and the lengths themselves still run."""
+    assert 0 < S.lds_bytes(0, LS_MAX_N, 1024) <= 160 * 1024 < S.lds_bytes(0, LS_MAX_N + 1, 1024)
+    for n, dtype, fn in ((LS_MAX_N, torch.float32, S.local_search),
+                         (4544, torch.float64, S.local_search)):
+        t = torch.full((1, n + 1), 1.0 / (n + 1), dtype=dtype, device="cuda")
+        with pytest.raises(S.SidonError) as e:
+            fn(t, 2, 4.0, 8.0, 0.01, 0.005)
+        assert e.value.status == S.SIDON_ERR_LDS
+        ok = torch.full((1, n), 1.0 / n, dtype=dtype, device="cuda")
+        fn(ok, 2, 4.0, 8.0, 0.01, 0.005)
+        torch.cuda.synchronize()
+        assert bool(torch.isfinite(ok).all()) and abs(float(ok.sum()) - 1) < 1e-5
