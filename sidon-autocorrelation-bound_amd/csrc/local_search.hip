@@ -345,7 +345,8 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
 // second round of whole candidates with the first halves (s = L - G);
 // uniform lengths with one candidate per workgroup or more split none; a few
 // long candidates among many short ones split them all.  Every workgroup
-// computes the same count (thread 0, serially, O(P)).
+// computes the same count (the sums over the workgroup's threads, the
+// minimisation by thread 0, O(L)).
 constexpr int kSplitMinN = 4096;
 constexpr double kLinCost = 11000.0;
 constexpr double kPairCost = 1.5;
@@ -362,26 +363,33 @@ __host__ __device__ inline double split_makespan(int s, int L, int G, double t0,
   if (rest > idle) M += (rest - idle) / (double)G;
   return M;
 }
-template <typename T>
-__device__ int split_count(const LsArgs<T>& A, int* s_out) {
+template <typename T, int BD>
+__device__ int split_count(const LsArgs<T>& A, int* s_out, unsigned char* smem) {
   if (!A.split || !A.order || !A.ns) return 0;
+  constexpr int NW = BD / kWave;
+  // L and the rest's cost over the workgroup's threads (a fixed-order block
+  // sum: every workgroup forms the same values); smem is free until the
+  // first candidate is loaded
+  double* red = reinterpret_cast<double*>(smem);
+  const int n0 = A.ns[A.order[0]];
+  double L = 0.0, rest = 0.0;
+  for (int p = (int)threadIdx.x; p < A.P; p += BD) {
+    const int n = A.ns[A.order[p]];
+    if (n == n0) L += 1.0;
+    else rest += (double)n * ((double)n + kLinCost);
+  }
+  L = block_sum<double, NW>(L, red);
+  rest = block_sum<double, NW>(rest, red + NW);
   if (threadIdx.x == 0) {
-    const int n0 = A.ns[A.order[0]];
     int bs = 0;
     if (n0 >= kSplitMinN) {
       const double t0 = (double)n0 * ((double)n0 + kLinCost);
-      int L = 0;
-      double rest = 0.0;
-      for (int p = 0; p < A.P; ++p) {
-        const int n = A.ns[A.order[p]];
-        if (n == n0) ++L;
-        else rest += (double)n * ((double)n + kLinCost);
-      }
+      const int Li = (int)L;
       const int G = (int)gridDim.x;
       const double h = kPairCost * t0 / 2.0;
-      double best = split_makespan(0, L, G, t0, h, rest);
-      for (int sc = 1; sc <= L; ++sc) {
-        const double m = split_makespan(sc, L, G, t0, h, rest);
+      double best = split_makespan(0, Li, G, t0, h, rest);
+      for (int sc = 1; sc <= Li; ++sc) {
+        const double m = split_makespan(sc, Li, G, t0, h, rest);
         if (m < best) { best = m; bs = sc; }
       }
     }
@@ -389,7 +397,7 @@ __device__ int split_count(const LsArgs<T>& A, int* s_out) {
   }
   __syncthreads();
   const int r = *s_out;
-  __syncthreads();
+  __syncthreads();  // every thread has read the count and the reduction slots
   return r;
 }
 
@@ -405,7 +413,7 @@ __device__ __forceinline__ void ls_body(const LsArgs<T>& A, unsigned char* smem,
     ls_candidate<T, BD, QMAX, SS, D>(A, cand, -1, smem, broken);
     return;
   }
-  const int split = split_count<T>(A, &s_item);
+  const int split = split_count<T, BD>(A, &s_item, smem);
   if (A.split_out && blockIdx.x == 0 && threadIdx.x == 0) *A.split_out = split;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(A.queue, 1);
@@ -519,6 +527,7 @@ struct LsTile { int block, depth, per_cu; };
 struct TileRule { int n; LsTile tile; };
 TileRule g_rules[64];
 int g_nrules = 0;
+std::mutex g_rules_mu;  // the table is process-wide; launches read it from any thread
 
 // The built-in rule: 512 threads from n = 512 up to 4096, two candidates a
 // CU (measured faster than 256 and 1024 at n = 600 ... 4096), four operand
@@ -533,6 +542,7 @@ LsTile default_tile(int n) {
 }
 
 LsTile pick_tile(int nmax) {
+  std::lock_guard<std::mutex> lk(g_rules_mu);
   for (int r = 0; r < g_nrules; ++r)
     if (g_rules[r].n == nmax) return g_rules[r].tile;
   return default_tile(nmax);
@@ -704,6 +714,7 @@ extern "C" int sidon_set_ls_tile(int n, int block, int depth, int per_cu) {
   const bool clear = block == 0;
   const LsTile tile{block, depth, per_cu};
   if (!clear && !tile_valid(tile)) return SIDON_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_rules_mu);
   for (int r = 0; r < g_nrules; ++r)
     if (g_rules[r].n == n) {
       if (clear) { g_rules[r] = g_rules[--g_nrules]; return SIDON_OK; }
