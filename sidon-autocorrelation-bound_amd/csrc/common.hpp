@@ -657,13 +657,22 @@ __device__ __forceinline__ void autoconv_finish_lane(const T* __restrict__ sA,
   for (int v = 0; v < 18; ++v) xv[v] = sA[xs + v + ((r1 + v) >> 4) * (SS - kSeg)];
 #pragma unroll
   for (int u = 0; u < 15; ++u) yv[u] = sA[ys - u];
+  // output r takes u < cnt(r) = ((k + 1) >> 1) - (i0 + r) terms: cnt(0) = cnt(1) = C,
+  // cnt(2) = cnt(3) = C - 1 (k = 16K + 4kk + r, 16K + 4kk even); the surplus terms
+  // multiply a zeroed y (heights are >= 0, so they add exactly +0 as before)
+  const int C = ((16 * K + 4 * kk) >> 1) - i0;
+  T ya[15], yb[15];
+#pragma unroll
+  for (int u = 0; u < 15; ++u) {
+    ya[u] = u < C ? yv[u] : T(0);
+    yb[u] = u < C - 1 ? yv[u] : T(0);
+  }
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int k = 16 * K + 4 * kk + r;
-    const int cnt = ((k + 1) >> 1) - (i0 + r);
     T s = acc[r];
 #pragma unroll
-    for (int u = 0; u < 15; ++u) s += u < cnt ? xv[r + u] * yv[u] : T(0);
+    for (int u = 0; u < 15; ++u) s += xv[r + u] * (r < 2 ? ya[u] : yb[u]);
     T b = s + s;
     if (!(k & 1)) {
       const T x = sA[pad_addr<SS>(k >> 1)];
