@@ -156,16 +156,18 @@ int sidon_local_search_ragged(float* pop, int stride, const int* ns, int P, int 
  * workgroup size `block` (256, 512 or 1024 threads: 4, 8 or 16 waves per
  * candidate), `depth` operand steps in flight in the matrix-core tile loops
  * (2 or 4) and `per_cu` candidates a CU (2 only with block 512 and
- * n <= 4096: two workgroups held to 128 registers a lane share a CU; else 1).
- * block 0 restores the built-in rule (512/2/2 for 512 <= n < 1024,
- * 512/4/2 up to 4096, 1024/4/1 above, 256/4/1 below).  Set by the autotuner.  The rule is
+ * n <= 4096, 4 only with block 256 and n <= 2304: workgroups held to 128
+ * registers a lane share a CU; else 1).  block 0 restores the built-in rule
+ * (256/2/4 below n = 1024, 512/4/2 up to 4096, 1024/4/1 above).  Set by the
+ * autotuner.  The rule is
  * process-wide host state, read when a launch is enqueued (a captured graph
  * keeps the configuration it was captured with). */
 int sidon_set_ls_tile(int n, int block, int depth, int per_cu);
 int sidon_get_ls_tile(int n, int* block, int* depth, int* per_cu);
 /* The workgroup-size-only forms: set_ls_block takes the built-in
- * configuration when it has that workgroup size, else depth 4 with two
- * candidates a CU where 512 threads allow it. */
+ * configuration when it has that workgroup size, else depth 4 with the
+ * CU-sharing form (per_cu 2 or 4) where the built-in rule's lengths allow
+ * it. */
 int sidon_set_ls_block(int n, int block);
 int sidon_get_ls_block(int n);
 
@@ -324,9 +326,9 @@ int sidon_fw_block_steps(unsigned long long* steps, int reset);
 /* ---- autotuner ---------------------------------------------------------- */
 
 /* The configurations the autotuner tries, i = 0 .. SIDON_LS_TILES-1:
- * 256/2/1, 256/4/1, 512/2/1, 512/4/1, 512/2/2, 512/4/2, 1024/2/1, 1024/4/1
- * (block/depth/per_cu as in sidon_set_ls_tile). */
-#define SIDON_LS_TILES 8
+ * 256/2/1, 256/4/1, 256/2/4, 256/4/4, 512/2/1, 512/4/1, 512/2/2, 512/4/2,
+ * 1024/2/1, 1024/4/1 (block/depth/per_cu as in sidon_set_ls_tile). */
+#define SIDON_LS_TILES 10
 int sidon_ls_tile_config(int i, int* block, int* depth, int* per_cu);
 
 /* Times the fused local search at length n in every configuration above on

@@ -312,13 +312,14 @@ def get_ls_block(n: int) -> int:
     return lib().sidon_get_ls_block(n)
 
 
-LS_TILES = 8
+LS_TILES = 10
 
 
 def set_ls_tile(n: int, block: int, depth: int = 4, per_cu: int = 1) -> None:
     """Launch configuration of the fp32 local search at length n: workgroup
     size, operand steps in flight (2 or 4), candidates a CU (2 only with 512
-    threads and n <= 4096).  block 0 restores the built-in rule."""
+    threads and n <= 4096, 4 only with 256 threads and n <= 2304).  block 0
+    restores the built-in rule."""
     _check(lib().sidon_set_ls_tile(n, block, depth, per_cu), "sidon_set_ls_tile")
 
 

@@ -141,8 +141,8 @@ extern "C" int sidon_generation_destroy(sidon_generation* g) {
 // waves per candidate (workgroup size), operand steps in flight, candidates a
 // CU.  The built-in rule at every length is one of them.
 static const int kTiles[SIDON_LS_TILES][3] = {
-    {256, 2, 1}, {256, 4, 1}, {512, 2, 1}, {512, 4, 1},
-    {512, 2, 2}, {512, 4, 2}, {1024, 2, 1}, {1024, 4, 1}};
+    {256, 2, 1}, {256, 4, 1}, {256, 2, 4}, {256, 4, 4}, {512, 2, 1},
+    {512, 4, 1}, {512, 2, 2}, {512, 4, 2}, {1024, 2, 1}, {1024, 4, 1}};
 
 extern "C" int sidon_ls_tile_config(int i, int* block, int* depth, int* per_cu) {
   if (i < 0 || i >= SIDON_LS_TILES) return SIDON_ERR_ARG;

@@ -440,11 +440,12 @@ __global__ __launch_bounds__(BD) void k_local_search(LsArgs<T> A) {
 }
 
 // The same kernel held to 128 registers a lane (four waves a SIMD): two
-// 512-thread candidates then share a CU where their LDS allows it (n <= 4096),
-// and one candidate's elementwise phases overlap the other's matrix work.
+// 512-thread or four 256-thread candidates then share a CU where their LDS
+// allows it, and one candidate's elementwise phases and barrier waits overlap
+// the others' matrix work.
 template <typename T, int BD, int QMAX, int SS, int D>
 __global__ __launch_bounds__(BD) __attribute__((amdgpu_waves_per_eu(4, 4))) void
-k_local_search_pair(LsArgs<T> A) {
+k_local_search_shared(LsArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_item, s_broken;
   ls_body<T, BD, QMAX, SS, D>(A, smem, s_item, &s_broken);
@@ -519,8 +520,9 @@ namespace {
 
 // Launch configuration of the fp32 local search at one length: waves per
 // candidate (workgroup size), operand steps in flight in the tile loops, and
-// candidates a CU (2: the 512-thread workgroup held to 128 registers a lane,
-// so two share a CU where their LDS allows; 1: one candidate a CU).
+// candidates a CU (2 or 4: the 512- or 256-thread workgroup held to 128
+// registers a lane, so two or four share a CU where their LDS allows; 1: the
+// kernel without that cap).
 struct LsTile { int block, depth, per_cu; };
 
 // Per-length overrides set by the autotuner (n -> configuration).
@@ -529,16 +531,19 @@ TileRule g_rules[64];
 int g_nrules = 0;
 std::mutex g_rules_mu;  // the table is process-wide; launches read it from any thread
 
-// The built-in rule: 512 threads from n = 512 up to 4096, two candidates a
-// CU (measured faster than 256 and 1024 at n = 600 ... 4096), four operand
-// steps in flight from n = 1024 and two below (at n = 600 the register-capped
-// four-step form spills and runs 3 % slower); 1024 threads above 4096, where
-// one candidate fills a CU's LDS.
+// The built-in rule, a function of n alone (measured, the anneal p 4 -> 1e4):
+// four 256-thread candidates a CU with two operand steps in flight below
+// n = 1024 (faster than every other configuration whether a launch holds two
+// or four candidates a CU: 1.1-1.7x at n = 128, 1.03-1.26x at 600); two
+// 512-thread candidates a CU with four steps up to 4096 (at 1200-1600 four
+// 256-thread candidates a CU win only when the launch has four a CU to give
+// them, which the autotuner sees and the rule does not); 1024 threads above,
+// where one candidate fills a CU's LDS.
+constexpr int kQuadMaxN = 1023;
 LsTile default_tile(int n) {
   if (n > 4096) return LsTile{1024, 4, 1};
-  if (n >= 1024) return LsTile{512, 4, 2};
-  if (n >= 512) return LsTile{512, 2, 2};
-  return LsTile{256, 4, 1};
+  if (n > kQuadMaxN) return LsTile{512, 4, 2};
+  return LsTile{256, 2, 4};
 }
 
 LsTile pick_tile(int nmax) {
@@ -551,13 +556,14 @@ LsTile pick_tile(int nmax) {
 bool tile_valid(const LsTile& c) {
   if (c.depth != 2 && c.depth != 4) return false;
   switch (c.block) {
-    case 256: case 1024: return c.per_cu == 1;
+    case 256: return c.per_cu == 1 || c.per_cu == 4;
     case 512: return c.per_cu == 1 || c.per_cu == 2;
+    case 1024: return c.per_cu == 1;
     default: return false;
   }
 }
 
-template <typename T, int BD, int QMAX, bool PAIR, int SS, int D>
+template <typename T, int BD, int QMAX, bool SHARED, int SS, int D>
 int launch_ls_layout(const LsArgs<T>& A, int nmax, hipStream_t stream) {
   if (nmax > BD * QMAX) return SIDON_ERR_UNSUPPORTED;
   int rc = SIDON_OK;
@@ -569,15 +575,15 @@ int launch_ls_layout(const LsArgs<T>& A, int nmax, hipStream_t stream) {
     return SIDON_ERR_LDS;
   }
   auto kfn = [] {
-    if constexpr (PAIR) return k_local_search_pair<T, BD, QMAX, SS, D>;
+    if constexpr (SHARED) return k_local_search_shared<T, BD, QMAX, SS, D>;
     else return k_local_search<T, BD, QMAX, SS, D>;
   }();
   SIDON_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kfn),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int grid = A.P;
   if (A.queue) {
-    // persistent: as many workgroups as are resident at once (two a CU for
-    // the register-capped 512-thread kernel where LDS allows), at most P
+    // persistent: as many workgroups as are resident at once (two or four a
+    // CU for the register-capped kernel where LDS allows), at most P
     int per_cu = 0;
     SIDON_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &per_cu, reinterpret_cast<const void*>(kfn), BD, (size_t)lds));
@@ -593,22 +599,22 @@ int launch_ls_layout(const LsArgs<T>& A, int nmax, hipStream_t stream) {
 
 // The wide (bank-conflict-free) segment layout where it fits the LDS, else
 // the narrow one.
-template <typename T, int BD, int QMAX, bool PAIR, int D>
+template <typename T, int BD, int QMAX, bool SHARED, int D>
 int launch_ls(const LsArgs<T>& A, int nmax, hipStream_t stream) {
   int rc = SIDON_OK;
   DeviceState* ds = device_state(&rc);
   if (!ds) return rc;
   if constexpr (sizeof(T) == 4)
     if (ls_lds_bytes<T, kSegStrideWide>(nmax, BD) <= ds->lds_max)
-      return launch_ls_layout<T, BD, QMAX, PAIR, kSegStrideWide, D>(A, nmax, stream);
-  return launch_ls_layout<T, BD, QMAX, PAIR, kSegStride, D>(A, nmax, stream);
+      return launch_ls_layout<T, BD, QMAX, SHARED, kSegStrideWide, D>(A, nmax, stream);
+  return launch_ls_layout<T, BD, QMAX, SHARED, kSegStride, D>(A, nmax, stream);
 }
 
 // fp32 configurations by operand depth
-template <int BD, int QMAX, bool PAIR>
+template <int BD, int QMAX, bool SHARED>
 int launch_ls_depth(const LsArgs<float>& A, int nmax, int depth, hipStream_t stream) {
-  if (depth == 2) return launch_ls<float, BD, QMAX, PAIR, 2>(A, nmax, stream);
-  if (depth == 4) return launch_ls<float, BD, QMAX, PAIR, 4>(A, nmax, stream);
+  if (depth == 2) return launch_ls<float, BD, QMAX, SHARED, 2>(A, nmax, stream);
+  if (depth == 4) return launch_ls<float, BD, QMAX, SHARED, 4>(A, nmax, stream);
   return SIDON_ERR_UNSUPPORTED;
 }
 
@@ -655,7 +661,12 @@ int dispatch_ls(LsArgs<T>& A, int nmax, const LsTile& tile, void* ws, hipStream_
     }
   } else {
     switch (tile.block) {
-      case 256: return launch_ls_depth<256, 16, false>(A, nmax, tile.depth, stream);
+      case 256:
+        if (tile.per_cu == 4) {
+          if (nmax > 256 * 9) return SIDON_ERR_UNSUPPORTED;
+          return launch_ls_depth<256, 9, true>(A, nmax, tile.depth, stream);
+        }
+        return launch_ls_depth<256, 16, false>(A, nmax, tile.depth, stream);
       case 512:
         if (tile.per_cu == 2) {
           if (nmax > 512 * 8) return SIDON_ERR_UNSUPPORTED;
@@ -739,10 +750,10 @@ extern "C" int sidon_set_ls_block(int n, int block) {
   if (block == 0) return sidon_set_ls_tile(n, 0, 0, 0);
   if (block != 256 && block != 512 && block != 1024) return SIDON_ERR_ARG;
   // the built-in configuration when it has this workgroup size, else four
-  // steps in flight and two candidates a CU where 512 threads allow it
+  // steps in flight and the CU-sharing form where the rule's lengths allow it
   const LsTile d = default_tile(n);
   if (d.block == block) return sidon_set_ls_tile(n, d.block, d.depth, d.per_cu);
-  const int per_cu = block == 512 && n <= 4096 ? 2 : 1;
+  const int per_cu = block == 512 && n <= 4096 ? 2 : block == 256 && n <= kQuadMaxN ? 4 : 1;
   return sidon_set_ls_tile(n, block, 4, per_cu);
 }
 

@@ -66,23 +66,18 @@ def ls_errors(got, want, gmin):
 LS_MAX_N = 9296
 
 
-@pytest.mark.parametrize("n,block", [(128, None), (600, None), (2100, None), (4096, None),
-                                     (8400, None), (LS_MAX_N, None), (1200, 1024),
-                                     (2100, 256)])
-def test_fp32_short_trajectory_vs_fp64(S, O, W, n, block):
-    """10 fp32 steps track the fp64 oracle (same start, kernel semantics), per
-    height, excluding only heights whose direction came within VANISH * M of
-    zero."""
-    print(f"[ls] fp32 trajectory n={n} block={block}", flush=True)
+def _trajectory_vs_fp64(S, O, W, n, tile):
+    """10 fp32 steps of the launch configuration `tile` (None: the built-in
+    rule) against the fp64 oracle from the same start."""
     a = W.structured_prior(2, n, seed=n).astype(np.float64)
     t = dev(a)
-    if block:
-        S.set_ls_block(n, block)
+    if tile:
+        S.set_ls_tile(n, *tile)
     try:
         S.local_search(t, 10, 4.0, 40.0, 0.05, 0.01)
     finally:
-        if block:
-            S.set_ls_block(n, 0)
+        if tile:
+            S.set_ls_tile(n, 0)
     got = t.cpu().numpy().astype(np.float64)
     for p in range(2):
         want, Fw, _, _, gmin = O.local_search_gmin(a[p].astype(np.float32).astype(np.float64),
@@ -93,6 +88,29 @@ def test_fp32_short_trajectory_vs_fp64(S, O, W, n, block):
         Fg = O.F_hostorder(got[p])
         assert abs(Fg - Fw) < 2e-6 * Fw, (Fg, Fw)
         assert (got[p] >= 0).all() and abs(got[p].sum() - 1) < 1e-5
+
+
+@pytest.mark.parametrize("n,tile", [(128, None), (600, None), (1023, None), (1200, None),
+                                    (2100, None), (4096, None), (8400, None), (LS_MAX_N, None),
+                                    (1200, (1024, 4, 1)), (2100, (256, 4, 1)),
+                                    (1600, (256, 4, 4)), (2304, (256, 4, 4))])
+def test_fp32_short_trajectory_vs_fp64(S, O, W, n, tile):
+    """10 fp32 steps track the fp64 oracle (same start, kernel semantics), per
+    height, excluding only heights whose direction came within VANISH * M of
+    zero; the built-in configuration at each length, and others at the limits
+    of their lengths (2304: the longest four-a-CU candidate)."""
+    print(f"[ls] fp32 trajectory n={n} tile={tile}", flush=True)
+    _trajectory_vs_fp64(S, O, W, n, tile)
+
+
+@pytest.mark.parametrize("i", range(10))
+def test_every_tuner_configuration_vs_fp64(S, O, W, i):
+    """Each configuration the autotuner may choose, at n = 600 (all fit):
+    10 fp32 steps against the fp64 oracle."""
+    assert S.LS_TILES == 10
+    tile = S.ls_tile_config(i)
+    print(f"[ls] tuner configuration {tile} n=600", flush=True)
+    _trajectory_vs_fp64(S, O, W, 600, tile)
 
 
 # about 4x the worst error measured on MI355X (5e-7)
@@ -156,8 +174,8 @@ def test_deterministic_and_ragged_equals_uniform(S, W):
     assert torch.equal(tr[:, :n], t1)
 
 
-@pytest.mark.parametrize("n,block", [(600, 256), (600, 512), (2100, 512), (2100, 1024),
-                                     (4096, 512)])
+@pytest.mark.parametrize("n,block", [(600, 256), (600, 512), (1000, 256), (2100, 512),
+                                     (2100, 1024), (4096, 512)])
 def test_repeated_launches_bitwise_every_block(S, W, n, block):
     """Regression: the weight-segment bits were once cleared in the same barrier
     interval in which other waves set them, so a fast wave's bits could be lost
@@ -177,6 +195,37 @@ def test_repeated_launches_bitwise_every_block(S, W, n, block):
     finally:
         S.set_ls_block(n, 0)
     assert torch.equal(runs[0], runs[1]) and torch.equal(runs[0], runs[2])
+
+
+def test_ragged_short_lengths_four_a_cu(S, W):
+    """A persistent launch whose longest candidate is short enough for four
+    256-thread workgroups a CU: more candidates than resident workgroups,
+    uneven n, each equal to its own launch in the same configuration."""
+    print("[ls] ragged short lengths, four a CU", flush=True)
+    rng = np.random.default_rng(5)
+    ns = rng.choice(np.array([17, 128, 300, 600, 777, 1000], np.int32), size=1200)
+    ns[7] = 1000
+    nmax = int(ns.max())
+    tile = S.get_ls_tile(nmax)
+    assert tile == (256, 2, 4)
+    pop = np.zeros((ns.size, nmax), np.float32)
+    for p, n in enumerate(ns):
+        pop[p, :n] = W.structured_prior(1, int(n), seed=300 + p)[0]
+    tr = dev(pop)
+    order = torch.empty(ns.size, dtype=torch.int32, device="cuda")
+    nsd = dev(ns, torch.int32)
+    S.order_by_length(nsd, order)
+    S.local_search_ragged(tr, nsd, nmax, 8, 4.0, 100.0, 0.05, 0.01, order=order)
+    got = tr.cpu().numpy()
+    for n in np.unique(ns):
+        rows = np.flatnonzero(ns == n)
+        single = dev(pop[rows, :n])
+        S.set_ls_tile(int(n), *tile)
+        try:
+            S.local_search(single, 8, 4.0, 100.0, 0.05, 0.01)
+        finally:
+            S.set_ls_tile(int(n), 0)
+        np.testing.assert_array_equal(got[rows, :n], single.cpu().numpy())
 
 
 def test_ragged_uneven_lengths(S, W):

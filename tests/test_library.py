@@ -50,6 +50,8 @@ def test_argument_validation_without_device():
     assert L.sidon_set_ls_block(4096, 300) == S.SIDON_ERR_ARG
     assert L.sidon_set_ls_tile(4096, 512, 3, 1) == S.SIDON_ERR_ARG    # depth 2 or 4
     assert L.sidon_set_ls_tile(4096, 1024, 4, 2) == S.SIDON_ERR_ARG   # two a CU: 512 only
+    assert L.sidon_set_ls_tile(600, 256, 4, 2) == S.SIDON_ERR_ARG     # four a CU: 256 only
+    assert L.sidon_set_ls_tile(600, 512, 2, 4) == S.SIDON_ERR_ARG
     assert L.sidon_ls_tile_config(S.LS_TILES, None, None, None) == S.SIDON_ERR_ARG
     # the persistent launch needs a caller-owned workspace of the stated size
     assert L.sidon_local_search_ragged(fake, 64, fake, 4, 64, None, None, None, 10, 4.0, 8.0,
@@ -110,7 +112,11 @@ def test_tile_rules_are_host_state():
     """The launch-configuration rules live on the host: the built-in rule, an
     override per length, and its removal (no GPU call)."""
     S = importlib.import_module("sidon-autocorrelation-bound_amd")
-    assert S.get_ls_tile(300) == (256, 4, 1)
+    assert S.get_ls_tile(300) == (256, 2, 4)
+    assert S.get_ls_tile(1023) == (256, 2, 4)
+    assert S.get_ls_tile(1024) == (512, 4, 2)
+    assert S.get_ls_tile(4096) == (512, 4, 2)
+    assert S.get_ls_tile(4097) == (1024, 4, 1)
     assert S.get_ls_tile(2100) == (512, 4, 2)
     assert S.get_ls_tile(8400) == (1024, 4, 1)
     S.set_ls_tile(2100, 1024, 2, 1)
@@ -121,12 +127,22 @@ def test_tile_rules_are_host_state():
     finally:
         S.set_ls_tile(2100, 0)
     assert S.get_ls_tile(2100) == (512, 4, 2)
-    assert S.get_ls_tile(600) == (512, 2, 2)
-    S.set_ls_block(600, 512)                  # the built-in configuration of that size
+    assert S.get_ls_tile(600) == (256, 2, 4)
+    S.set_ls_block(600, 256)                  # the built-in configuration of that size
     try:
-        assert S.get_ls_tile(600) == (512, 2, 2)
+        assert S.get_ls_tile(600) == (256, 2, 4)
     finally:
         S.set_ls_block(600, 0)
+    S.set_ls_block(600, 512)                  # another size: depth 4, CU-sharing form
+    try:
+        assert S.get_ls_tile(600) == (512, 4, 2)
+    finally:
+        S.set_ls_block(600, 0)
+    S.set_ls_block(2100, 256)                 # past the four-a-CU lengths: one a CU
+    try:
+        assert S.get_ls_tile(2100) == (256, 4, 1)
+    finally:
+        S.set_ls_block(2100, 0)
     S.set_ls_block(600, 1024)
     try:
         assert S.get_ls_tile(600) == (1024, 4, 1)
@@ -134,5 +150,5 @@ def test_tile_rules_are_host_state():
         S.set_ls_block(600, 0)
     tiles = [S.ls_tile_config(i) for i in range(S.LS_TILES)]
     assert len(set(tiles)) == S.LS_TILES
-    for n in (300, 600, 2100, 4096, 8400):      # the built-in rule is a trial
+    for n in (300, 600, 1200, 2100, 4096, 8400):      # the built-in rule is a trial
         assert S.get_ls_tile(n) in tiles
