@@ -228,17 +228,22 @@ __device__ __forceinline__ void fill_window(double* win, const double* __restric
 
 // Adds the UW units' 16 x 16 partial tiles of a workgroup in unit order and
 // stores the sum (256 doubles at `o`, one per thread, coalesced).  `red` is
-// UW * 256 doubles of LDS that no wave reads any more; every thread calls it.
-// Lane (kk, t) of a unit holds rows kk + 4r of column t: outputs 16 t + kk + 4 r.
+// UW * kRedTile doubles of LDS that no wave reads any more; every thread calls
+// it.  Lane (kk, t) of a unit holds rows kk + 4r of column t: outputs
+// 16 t + kk + 4 r.  Columns sit 17 doubles apart in `red`, so the sixteen
+// columns a store instruction writes fall on distinct banks (at 16 doubles
+// apart they shared two).
+constexpr int kRedCol = 17, kRedTile = 16 * kRedCol;
 __device__ __forceinline__ void group_sum_store(double* red, const f64x4& acc, double* o) {
   const int lane = (int)threadIdx.x & (kWave - 1), wv = (int)threadIdx.x / kWave;
-  double* p = red + 256 * wv + 16 * (lane & 15) + (lane >> 4);
+  double* p = red + kRedTile * wv + kRedCol * (lane & 15) + (lane >> 4);
   p[0] = acc[0]; p[4] = acc[1]; p[8] = acc[2]; p[12] = acc[3];
   __syncthreads();
   const int k = threadIdx.x;  // UW * kWave == 256 threads, one output each
-  double r = red[k];
+  const int q = kRedCol * (k >> 4) + (k & 15);
+  double r = red[q];
 #pragma unroll
-  for (int u = 1; u < UW; ++u) r += red[256 * u + k];
+  for (int u = 1; u < UW; ++u) r += red[kRedTile * u + q];
   o[k] = r;
 }
 static_assert(UW * kWave == 256, "one thread per tile output in group_sum_store");
@@ -291,7 +296,7 @@ __global__ __launch_bounds__(UW * kWave) void k_polish_conv_units(Shape s, Ws w,
   constexpr int LENA = CH * UW + 1, LENB = 16 + CH * UW - 1;
   __shared__ __align__(16) double wa[win_len(LENA)];
   __shared__ __align__(16) double wb[bwin_len(LENB)];
-  static_assert(win_len(LENA) >= 256 * UW, "the group sum reuses wa");
+  static_assert(win_len(LENA) >= kRedTile * UW, "the group sum reuses wa");
   const int cand = blockIdx.z, g = blockIdx.y, j = blockIdx.x;
   int lo, hi;
   autoconv_drange(j, s.N16, lo, hi);
@@ -432,7 +437,7 @@ __global__ __launch_bounds__(UW * kWave) void k_polish_corr_units(Shape s, Ws w,
   constexpr int NT = UW * kWave, RC = (LENC + NT - 1) / NT;
   __shared__ __align__(16) double wc[LENC + 1];
   __shared__ __align__(16) double wa[bwin_len(LENA)];
-  static_assert(bwin_len(LENA) >= 256 * UW, "the group sum reuses wa");
+  static_assert(bwin_len(LENA) >= kRedTile * UW, "the group sum reuses wa");
   const int cand = blockIdx.z, g = blockIdx.y, j = blockIdx.x;
   const double* xs = w.x + w.sn * cand;
   const double* c = w.bc + w.sb * cand;
