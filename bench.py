@@ -134,25 +134,33 @@ def cpu_baseline(n: int, seconds: float, gpu_pop, gpu_F):
                       f"iterations, {dt:.1f} s"}, fcheck
 
 
+# the longest fp32 length whose correlation output fits the LDS beside a and
+# the weights; longer ones keep it in global memory (the moments' buffer)
+LS_ONCHIP_N = 9296
+
+
 def ls_by_n(S, W, torch, seed: int, P: int = 512, iters: int = 50):
-    """The batched local search at n = 512 ... 8400: P structured-prior
+    """The batched local search at n = 512 ... 13216: P structured-prior
     candidates of one length, one launch of `iters` steps of the anneal
     (p 4 -> 1e4) after a warm-up, HIP events around it; candidate-steps/s and
     executed matrix work (the kernel's block-step count x 8192 FLOP) against
-    the FP32 peak, in the launch configuration the library picks for n."""
+    the FP32 peak, in the launch configuration the library picks for n
+    (10000 and 13216: the correlation output in global memory, 24- and
+    16-float segments)."""
     out = []
     stream = torch.cuda.current_stream()
     sched = (4.0, 1e4, 0.085, 0.0015)
-    for n in (512, 1024, 2048, 4096, 8400):
+    for n in (512, 1024, 2048, 4096, 8400, 10000, 13216):
         start = torch.as_tensor(W.structured_prior(P, n, seed=seed + 7 * n), device="cuda")
         pop = start.clone()
-        S.local_search(pop, 5, *sched)                  # warm-up
+        mom = torch.empty_like(pop) if n > LS_ONCHIP_N else None
+        S.local_search(pop, 5, *sched, mom=mom)         # warm-up
         pop.copy_(start)
         torch.cuda.synchronize()
         S.ls_block_steps(reset=True)
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
-        S.local_search(pop, iters, *sched)
+        S.local_search(pop, iters, *sched, mom=mom)
         ev1.record(stream)
         torch.cuda.synchronize()
         sec = ev0.elapsed_time(ev1) / 1e3

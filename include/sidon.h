@@ -87,6 +87,12 @@ int sidon_eval_exact(const float* pop, int n, int P, double* F, double* M, doubl
  * (pStart, lrStart) to (pEnd, lrEnd).  Adam moments restart at zero every call;
  * mom/vel (may be NULL) receive the final moments.  skip[p] != 0 leaves
  * candidate p untouched; mode (may be NULL) overrides modeDefault per candidate.
+ * Lengths whose correlation output does not fit one workgroup's LDS beside a
+ * and the weights (n > 9296 on a 160 KiB workgroup) keep it in global memory:
+ * mom (or, when mom is NULL, vel) is then the launch's scratch for it until
+ * the final moments are written, and one of them must be given
+ * (SIDON_ERR_LDS otherwise; the reference always passes both).  The longest
+ * fp32 length is 13216 (sidon_lds_bytes(0, n, 1024) <= sidon_lds_limit()).
  * Replaces runLocalSearch (src/main.cu:152-159) / kLocalSearch<float>
  * (src/kernels.cuh:177-289). */
 int sidon_local_search(float* pop, float* mom, float* vel, const int* skip, const int* mode,
@@ -139,7 +145,9 @@ int sidon_polish_f64_noalloc(double* pop, double* mom, double* vel, int n, int K
  * (none when every workgroup has a candidate of the longest length to run);
  * the result is bit-identical to one workgroup's.  `workspace` (caller-owned, device,
  * sidon_local_search_ragged_workspace(P, stride) bytes) holds the queue
- * counter, the pairs' progress flags and exchange buffers; the launch clears
+ * counter, the pairs' progress flags and exchange buffers, and for strides
+ * above the on-chip form (9296) each candidate's correlation output (nmax
+ * above it: no pairs, every candidate runs that form); the launch clears
  * what must start at zero with a kernel on `stream` (no memset node).
  * Launches that may overlap need distinct workspaces.  After the launch the
  * int at byte 4 of the workspace counts pair waits that gave up (a partner
@@ -176,6 +184,8 @@ int sidon_get_ls_block(int n);
 /* J_p(a) = 2n ||a*a||_p / (sum a)^2 and its true gradient
  * dJ/da_i = (4 n R^{1/p}/S^2)(G_i/R - M/S), recomputing b (never stored).
  * a is [B x n]; J (length B) and grad ([B x n]) may be NULL independently.
+ * Above the on-chip form the correlation accumulates in grad itself
+ * (fp32 to n = 13216).
  * No reference counterpart (kLocalSearch forms only the scaled direction,
  * src/kernels.cuh:165-172, :244). */
 int sidon_lp_grad(const float* a, int n, int B, float p, float* J, float* grad,

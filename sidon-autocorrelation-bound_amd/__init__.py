@@ -250,7 +250,10 @@ def local_search(pop: torch.Tensor, iters: int, p_start: float, p_end: float, lr
                  mode_arr: torch.Tensor | None = None, skip: torch.Tensor | None = None,
                  mom: torch.Tensor | None = None, vel: torch.Tensor | None = None,
                  stream=None) -> None:
-    """In-place fused local search on every row of pop (float32 or float64)."""
+    """In-place fused local search on every row of pop (float32 or float64).
+    float32 rows longer than the on-chip form (lds_bytes(0, n, 1024) grows past
+    the G-in-LDS layout at n = 9296) keep the correlation output in global
+    memory and need `mom` or `vel`: the launch uses it as that scratch."""
     _need_gpu()
     P, n = pop.shape
     if pop.dtype == torch.float32:

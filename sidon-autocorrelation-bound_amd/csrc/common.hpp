@@ -138,14 +138,19 @@ template <int SS = kSegStride, typename T>
 __device__ __forceinline__ void pad_put(T* s, int x, T v) {
   const int p = pad_addr<SS>(x);
   s[p] = v;
-  // copy at the head of the next segment
-  if (((x + kPadSeg * kSeg) & 15) >= 12) s[p + (SS - kSeg)] = v;
+  // copy at the head of the next segment (SS = 16: the layout is contiguous,
+  // the "copy" is the element itself)
+  if constexpr (SS > kSeg)
+    if (((x + kPadSeg * kSeg) & 15) >= 12) s[p + (SS - kSeg)] = v;
 }
 __host__ __device__ __forceinline__ int nseg16(int n) { return (n + kSeg - 1) / kSeg; }
-// floats (elements) of the padded vector buffer
+// floats (elements) of the padded vector buffer.  SS = 16 is the plain
+// contiguous layout (pad_addr(x) = x + 260): the four leading slots of the
+// overlapped layouts then shift the last segment four elements past
+// (N16 + 32) * 16, so it takes four more.
 template <int SS = kSegStride>
 __host__ __device__ __forceinline__ int pad_len(int n) {
-  return (nseg16(n) + 2 * kPadSeg) * SS;
+  return (nseg16(n) + 2 * kPadSeg) * SS + (SS == kSeg ? 4 : 0);
 }
 // elements of the weight / autoconvolution buffer (normal layout, zero tail)
 __host__ __device__ __forceinline__ int cbuf_len(int n) { return kSeg * (2 * nseg16(n) + 16); }
@@ -416,6 +421,39 @@ __device__ __forceinline__ void put4(T* out, int col, const A4& acc, bool whole)
     atomicAdd(o, acc[0]); atomicAdd(o + RS, acc[1]);
     atomicAdd(o + 2 * RS, acc[2]); atomicAdd(o + 3 * RS, acc[3]);
   }
+}
+
+// put4 into a candidate's row in global memory (outputs i < n only; rows past
+// n belong to the next candidate): whole tiles by device-coherent stores,
+// halves of a shared tile by device-scope atomic adds onto an exact zero.
+// Readers in the same workgroup wait for these (s_waitcnt vmcnt(0) before
+// their barrier) and read with coherent_load, so no stale L1 line is seen.
+template <typename T, typename A4>
+__device__ __forceinline__ void put4_global(T* out, int col, const A4& acc, bool whole, int n) {
+  const int kk = (opaque((int)threadIdx.x) & (kWave - 1)) >> 4;
+  constexpr int RS = Acc4<T>::kRowStride;
+  const int i0 = 16 * col + Acc4<T>::kRowMul * kk;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + RS * r;
+    if (i < n) {
+      if (whole) __hip_atomic_store(out + i, acc[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      else unsafeAtomicAdd(out + i, acc[r]);
+    }
+  }
+}
+template <typename T>
+__device__ __forceinline__ void coherent_store(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T coherent_load(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// This wave's global stores and atomics have completed (before a barrier
+// that publishes them to the other waves of the workgroup).
+__device__ __forceinline__ void wait_global_writes() {
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 // -------------------------------------------- MFMA autoconvolution (split) --
@@ -770,7 +808,8 @@ __device__ __forceinline__ void corr_items(int n, unsigned* items, int* scratch,
 // >= 2n-1 up to cbuf_len(n); a padded.  `out` must be exactly zero before
 // the call.  With `segmask` (the weight segments' non-zero bits) steps that
 // read only zero weights are skipped.
-template <typename T, int D = Pipe<T>::kDepth, int SS = kSegStride>
+// With GOUT, `out` is the candidate's row in global memory (put4_global).
+template <typename T, int D = Pipe<T>::kDepth, int SS = kSegStride, bool GOUT = false>
 __device__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* out, int n,
                           int* ctr, const unsigned* items, const unsigned* segmask,
                           unsigned* steps = nullptr, int half = -1) {
@@ -792,7 +831,8 @@ __device__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* 
         acc = corr_tile<T, D, SS>(c, sA, I0, I0 + u0, I0 + u1);
         done += u1 - u0;
       }
-      put4(out, I0 + t, acc, u0 == 0 && u1 == cost);
+      if constexpr (GOUT) put4_global(out, I0 + t, acc, u0 == 0 && u1 == cost, n);
+      else put4(out, I0 + t, acc, u0 == 0 && u1 == cost);
     }
     it = ticket_value(take_ticket(ctr));
   }

@@ -671,6 +671,10 @@ int search(const Args& A) {
   float* dst = dalloc<float>(P);
   int* skip = dalloc<int>(P);
   int* mode = dalloc<int>(P);
+  // Adam moments out of each launch, which above the on-chip form (long
+  // levels) also serve as the launch's scratch for the correlation output
+  // (the reference allocates g.mom for every level, main.cu:107-144)
+  float* mom = dalloc<float>((size_t)P * nmax);
   std::vector<int> hmode(P);
   for (int p = 0; p < P; ++p) hmode[p] = (p / PI) % 4 == 3 ? 1 : 0;  // mirror descent island
   CHECK_HIP(hipMemcpy(mode, hmode.data(), P * sizeof(int), hipMemcpyHostToDevice));
@@ -728,7 +732,7 @@ int search(const Args& A) {
       auto tier_run = [&](int t, int iters, float pa, float pb, float la, float lb) {
         for (int p = 0; p < P; ++p) hskip[p] = tier[p] != t;
         CHECK_HIP(hipMemcpy(skip, hskip.data(), P * sizeof(int), hipMemcpyHostToDevice));
-        CHECK_SIDON(sidon_local_search(pop, nullptr, nullptr, skip, mode, n, P, iters, pa, pb,
+        CHECK_SIDON(sidon_local_search(pop, mom, nullptr, skip, mode, n, P, iters, pa, pb,
                                        la, lb, 0.9f, 0.999f, 0, s));
         for (int p = 0; p < P; ++p) lsSteps += (long long)(hskip[p] == 0) * iters;
         flop += 4.0 * n2 * (double)iters * (double)(t == 0 ? nCold : t == 1 ? nMed : P - nCold - nMed);
@@ -857,7 +861,7 @@ int search(const Args& A) {
     }
   }
   sidon_policy_destroy(pol);
-  CHECK_HIP(hipFree(pop)); CHECK_HIP(hipFree(nxt));
+  CHECK_HIP(hipFree(pop)); CHECK_HIP(hipFree(nxt)); CHECK_HIP(hipFree(mom));
   if (A.polish_iters > 0 && !bestA.empty()) polish(A, bestA, bestF);
   const double hf = host_F(bestA);
   save_vec(A.out + ".txt", bestA, std::max(bestF, hf));

@@ -267,13 +267,13 @@ int launch_fw_layout(const FwArgs& A, int P, hipStream_t stream) {
   int rc = SIDON_OK;
   DeviceState* ds = device_state(&rc);
   if (!ds) return rc;
-  if (corr_tiles(A.n) > GT * (BD / kWave) || conv_tiles(A.n) > kMaxConvTiles)
-    return SIDON_ERR_UNSUPPORTED;
   const long long lds = fw_lds_bytes<BD, SS>(A.n);
-  if (lds > ds->lds_max) {
+  if (lds > ds->lds_max) {  // first: a length that does not fit is SIDON_ERR_LDS
     set_last_error_msg("frank_wolfe: n needs more LDS than a workgroup may use");
     return SIDON_ERR_LDS;
   }
+  if (corr_tiles(A.n) > GT * (BD / kWave) || conv_tiles(A.n) > kMaxConvTiles)
+    return SIDON_ERR_UNSUPPORTED;
   auto kfn = k_frank_wolfe<BD, GT, SS>;
   SIDON_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kfn),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

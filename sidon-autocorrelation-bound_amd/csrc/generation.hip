@@ -199,6 +199,21 @@ extern "C" int sidon_tune_local_search(float* scratch, const float* reference, i
   (void)hipEventDestroy(e1);
   (void)sidon_set_ls_tile(n, 0, 0, 0);
   if (rc) return rc;
+  bool any = false;
+  for (int c = 0; c < SIDON_LS_TILES; ++c) any = any || ms_of[c] >= 0.0f;
+  if (!any && sidon_lds_bytes(0, n, 1024) >= 0 &&
+      sidon_lds_bytes(0, n, 1024) <= (long long)sidon_lds_limit()) {
+    // above the on-chip form the launch has one configuration (1024 threads,
+    // four steps in flight, the correlation output in global memory): nothing
+    // to choose, and the trials (no moments' buffer) cannot run it
+    if (best_block) *best_block = 1024;
+    if (best_depth) *best_depth = 4;
+    if (best_per_cu) *best_per_cu = 1;
+    if (best_ms) *best_ms = -1.0f;
+    if (trial_ms)
+      for (int c = 0; c < SIDON_LS_TILES; ++c) trial_ms[c] = -1.0f;
+    return SIDON_OK;
+  }
   int dflt = -1, best = -1;
   for (int c = 0; c < SIDON_LS_TILES; ++c) {
     if (kTiles[c][0] == dblock && kTiles[c][1] == ddepth && kTiles[c][2] == dper) dflt = c;

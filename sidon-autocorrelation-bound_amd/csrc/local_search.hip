@@ -55,6 +55,10 @@ struct LsArgs {
   int* split_out;       // the number of candidates split (written by workgroup 0)
   T* xchg;              // per candidate xstride elements: the b_k of each half's tiles
   long long xstride;
+  // lengths whose correlation output does not fit the LDS beside a and the
+  // weights: each candidate's G in global memory, gstride elements a row
+  T* gscr;
+  long long gstride;
 };
 
 // ------------------------------------------------------- paired halves --
@@ -80,15 +84,6 @@ struct LsArgs {
 constexpr unsigned long long kPairWaitTicks = 2000000000ull;  // 20 s
 constexpr int kPairFinal = 1 << 30;
 
-template <typename T>
-__device__ __forceinline__ void coherent_store(T* p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ T coherent_load(const T* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 __device__ __forceinline__ void pair_release(int* flag, int value) {
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wave's coherent stores are done
   __syncthreads();
@@ -112,20 +107,28 @@ __device__ __forceinline__ void pair_acquire(const int* flag, int value, int* er
   __syncthreads();
 }
 
-// Bytes of dynamic LDS for one candidate of length n.
-template <typename T, int SS = kSegStride>
+// Bytes of dynamic LDS for one candidate of length n (GG: the correlation
+// output G in global memory, not in LDS).
+template <typename T, int SS = kSegStride, bool GG = false>
 __host__ __device__ inline long long ls_lds_bytes(int n, int block) {
   // + reduction slots (2 per wave), the running max of b, three counters
   // (work items of each product, matrix steps run), the work items of both
   // products and the weight segments' bits (32-bit each; T is at least as
   // wide)
   // (conv_tiles(n) more: the item rank of each autoconvolution tile)
-  const long long elems = (long long)pad_len<SS>(n) + cbuf_len(n) + gbuf_len(n) + 2 * (block / kWave) +
-                          1 + 3 + 3 * conv_tiles(n) + 2 * corr_tiles(n) + segmask_words(n);
+  const long long elems = (long long)pad_len<SS>(n) + cbuf_len(n) + (GG ? 0 : gbuf_len(n)) +
+                          2 * (block / kWave) + 1 + 3 + 3 * conv_tiles(n) + 2 * corr_tiles(n) +
+                          segmask_words(n);
   return elems * (long long)sizeof(T);
 }
 
-template <typename T, int BD, int QMAX, int SS, int D>
+// GG: the correlation output G lives in the candidate's row of A.gscr
+// (global memory; lengths whose G does not fit the LDS beside a and the
+// weights).  The correlation writes it there (put4_global), the update reads
+// and re-zeroes it with coherent accesses, and parks the unnormalised new
+// height in a's own LDS slot (only its owner thread reads that slot during the
+// update) until S is known.  No pairs in this form (half is always -1).
+template <typename T, int BD, int QMAX, int SS, int D, bool GG = false>
 __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned char* smem,
                              int* broken) {
   constexpr int NW = BD / kWave;
@@ -133,13 +136,15 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
   const int n = A.ns ? A.ns[cand] : A.n;
   if (A.skip && A.skip[cand]) return;
   if (n <= 0) return;
+  if constexpr (GG) half = -1;
   const int mode = A.mode ? A.mode[cand] : A.modeDefault;
   T* __restrict__ a = A.pop + (size_t)cand * (size_t)A.stride;
 
   T* sA = reinterpret_cast<T*>(smem);
   T* sC = sA + pad_len<SS>(n);
   T* sG = sC + cbuf_len(n);
-  T* slots = sG + gbuf_len(n);  // 2 * NW
+  T* slots = sG + (GG ? 0 : gbuf_len(n));  // 2 * NW
+  T* gG = GG ? A.gscr + (size_t)cand * (size_t)A.gstride : nullptr;
   T* bmax = slots + 2 * NW;     // max_k b_k of the step (raised by the autoconvolution)
   int* ctr = reinterpret_cast<int*>(bmax + 1);  // [0] conv items, [1] corr items
   unsigned* nsteps = reinterpret_cast<unsigned*>(ctr + 2);  // matrix block steps run
@@ -162,7 +167,12 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
   for (int r = tid; r < NTc; r += BD) crank[citems[r] >> (2 * kItemBits)] = r;
   for (int i = tid; i < LA; i += BD) sA[i] = T(0);
   for (int i = tid; i < LC; i += BD) sC[i] = T(0);
-  for (int i = tid; i < LG; i += BD) sG[i] = T(0);
+  if constexpr (GG) {
+    for (int i = tid; i < n; i += BD) coherent_store(gG + i, T(0));
+    wait_global_writes();  // zero before any wave's correlation adds onto it
+  } else {
+    for (int i = tid; i < LG; i += BD) sG[i] = T(0);
+  }
   for (int w = tid; w < SW; w += BD) segmask[w] = 0u;
   __syncthreads();
   for (int i = tid; i < n; i += BD) pad_put<SS>(sA, i, a[i]);
@@ -236,7 +246,12 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
     b2t *= beta2;
     const T bc1 = T(1) / (T(1) - b1t), bc2 = T(1) / (T(1) - b2t);
 
-    corr_mfma<T, D, SS>(sC, sA, sG, n, ctr + 1, gitems, segmask, nsteps, half);
+    if constexpr (GG) {
+      corr_mfma<T, D, SS, true>(sC, sA, gG, n, ctr + 1, gitems, segmask, nsteps, half);
+      wait_global_writes();  // G complete in memory before the barrier
+    } else {
+      corr_mfma<T, D, SS>(sC, sA, sG, n, ctr + 1, gitems, segmask, nsteps, half);
+    }
     const int ut = opaque(tid);  // element addresses below are built per step, not hoisted
     __syncthreads();
 
@@ -251,7 +266,14 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
       const int i = ut + BD * q;
       if (i < n && (half < 0 || corr_tile_owner(i >> 8) == half)) {
         const T ai = sA[pad_addr<SS>(i)];
-        const T gi = dquot(sG[i], R) - M;
+        T Gi;
+        if constexpr (GG) {
+          Gi = coherent_load(gG + i);
+          coherent_store(gG + i, T(0));  // zero again for the next step's correlation
+        } else {
+          Gi = sG[i];
+        }
+        const T gi = dquot(Gi, R) - M;
         T xx;
         if (mode == 0) {
           const T mm = beta1 * m[q] + (T(1) - beta1) * gi;
@@ -267,7 +289,10 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
           if (e < -T(4)) e = -T(4);
           xx = ai * dexp(e);
         }
-        sG[i] = xx;  // the owner thread parks the new height in place of G_i
+        // the owner thread parks the new height in place of G_i (GG: in its
+        // own slot of a, which only this thread reads until S is known)
+        if constexpr (GG) pad_put<SS>(sA, i, xx);
+        else sG[i] = xx;
         sloc += xx;
         if (half >= 0) coherent_store(a + i, xx);
       }
@@ -292,10 +317,14 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) {
       const int i = ut + BD * q;
-      if (i < n) pad_put<SS>(sA, i, reset ? T(1) / nT : sG[i] * inv);
+      if (i < n) {
+        const T xi = GG ? sA[pad_addr<SS>(i)] : sG[i];
+        pad_put<SS>(sA, i, reset ? T(1) / nT : xi * inv);
+      }
     }
     // sC needs no clearing: the autoconvolution stores every tile whole
-    for (int i = tid; i < LG; i += BD) sG[i] = T(0);
+    if constexpr (GG) wait_global_writes();  // G's zeros land before the next correlation
+    else for (int i = tid; i < LG; i += BD) sG[i] = T(0);
     __syncthreads();
   }
 
@@ -404,16 +433,16 @@ __device__ int split_count(const LsArgs<T>& A, int* s_out, unsigned char* smem) 
 // One candidate per workgroup (grid = candidates), or workgroups pulling
 // tickets from a device queue (persistent): ticket t < 2 s is half t % 2 of
 // candidate order[t / 2] (s = split_count), later tickets whole candidates.
-template <typename T, int BD, int QMAX, int SS, int D>
+template <typename T, int BD, int QMAX, int SS, int D, bool GG = false>
 __device__ __forceinline__ void ls_body(const LsArgs<T>& A, unsigned char* smem, int& s_item,
                                         int* broken) {
   if (threadIdx.x == 0) *broken = 0;
   if (!A.queue) {
     const int cand = A.order ? A.order[blockIdx.x] : (int)blockIdx.x;
-    ls_candidate<T, BD, QMAX, SS, D>(A, cand, -1, smem, broken);
+    ls_candidate<T, BD, QMAX, SS, D, GG>(A, cand, -1, smem, broken);
     return;
   }
-  const int split = split_count<T, BD>(A, &s_item, smem);
+  const int split = GG ? 0 : split_count<T, BD>(A, &s_item, smem);
   if (A.split_out && blockIdx.x == 0 && threadIdx.x == 0) *A.split_out = split;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(A.queue, 1);
@@ -428,7 +457,7 @@ __device__ __forceinline__ void ls_body(const LsArgs<T>& A, unsigned char* smem,
     } else {
       cand = A.order ? A.order[item - split] : item - split;
     }
-    ls_candidate<T, BD, QMAX, SS, D>(A, cand, half, smem, broken);
+    ls_candidate<T, BD, QMAX, SS, D, GG>(A, cand, half, smem, broken);
   }
 }
 
@@ -437,6 +466,16 @@ __global__ __launch_bounds__(BD) void k_local_search(LsArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_item, s_broken;
   ls_body<T, BD, QMAX, SS, D>(A, smem, s_item, &s_broken);
+}
+
+// Lengths whose correlation output does not fit the LDS: G in global memory
+// (A.gscr), the segment stride as wide as the LDS allows (24, 20, or the
+// plain layout 16, whose B-operand reads are served in four passes).
+template <typename T, int BD, int QMAX, int SS, int D>
+__global__ __launch_bounds__(BD) void k_local_search_gg(LsArgs<T> A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int s_item, s_broken;
+  ls_body<T, BD, QMAX, SS, D, true>(A, smem, s_item, &s_broken);
 }
 
 // The same kernel held to 128 registers a lane (four waves a SIMD): two
@@ -461,17 +500,20 @@ struct GradArgs {
   T* grad;
 };
 
-template <typename T, int BD, int SS>
+// GG: lengths whose correlation output does not fit the LDS; the
+// correlation accumulates straight into the candidate's gradient row.
+template <typename T, int BD, int SS, bool GG = false>
 __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   constexpr int NW = BD / kWave;
   constexpr int D = (BD >= 1024 || sizeof(T) == 8) ? 2 : 4;
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = A.n, cand = blockIdx.x;
   const T* a = A.a + (size_t)cand * n;
+  T* g = A.grad ? A.grad + (size_t)cand * n : nullptr;
   T* sA = reinterpret_cast<T*>(smem);
   T* sC = sA + pad_len<SS>(n);
   T* sG = sC + cbuf_len(n);
-  T* slots = sG + gbuf_len(n);
+  T* slots = sG + (GG ? 0 : gbuf_len(n));
   T* bmax = slots + 2 * NW;
   int* ctr = reinterpret_cast<int*>(bmax + 1);
   unsigned* citems = reinterpret_cast<unsigned*>(ctr + 3);
@@ -485,7 +527,14 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   corr_items(n, gitems, reinterpret_cast<int*>(sC), tid, BD);
   for (int i = tid; i < LA; i += BD) sA[i] = T(0);
   for (int i = tid; i < LC; i += BD) sC[i] = T(0);
-  for (int i = tid; i < LG; i += BD) sG[i] = T(0);
+  if constexpr (GG) {
+    if (g) {
+      for (int i = tid; i < n; i += BD) coherent_store(g + i, T(0));
+      wait_global_writes();
+    }
+  } else {
+    for (int i = tid; i < LG; i += BD) sG[i] = T(0);
+  }
   __syncthreads();
   T sl = T(0);
   for (int i = tid; i < n; i += BD) { const T x = a[i]; pad_put<SS>(sA, i, x); sl += x; }
@@ -506,13 +555,17 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   const T R = block_sum<T, NW>(rloc, slots + NW);
   const T Rp = dexp(log(R) / A.p);
   if (A.J && tid == 0) A.J[cand] = T(2) * T(n) * M * Rp / (S * S);
-  if (!A.grad) return;
-  corr_mfma<T, D, SS>(sC, sA, sG, n, ctr + 1, gitems, segmask);
+  if (!g) return;
+  if constexpr (GG) {
+    corr_mfma<T, D, SS, true>(sC, sA, g, n, ctr + 1, gitems, segmask);
+    wait_global_writes();
+  } else {
+    corr_mfma<T, D, SS>(sC, sA, sG, n, ctr + 1, gitems, segmask);
+  }
   __syncthreads();
   const T scale = T(4) * T(n) * Rp / (S * S);
   const T MS = M / S;
-  T* g = A.grad + (size_t)cand * n;
-  for (int i = tid; i < n; i += BD) g[i] = scale * (sG[i] / R - MS);
+  for (int i = tid; i < n; i += BD) g[i] = scale * ((GG ? coherent_load(g + i) : sG[i]) / R - MS);
 }
 
 // ------------------------------------------------------------- host side --
@@ -563,19 +616,20 @@ bool tile_valid(const LsTile& c) {
   }
 }
 
-template <typename T, int BD, int QMAX, bool SHARED, int SS, int D>
+template <typename T, int BD, int QMAX, bool SHARED, int SS, int D, bool GG = false>
 int launch_ls_layout(const LsArgs<T>& A, int nmax, hipStream_t stream) {
   if (nmax > BD * QMAX) return SIDON_ERR_UNSUPPORTED;
   int rc = SIDON_OK;
   DeviceState* ds = device_state(&rc);
   if (!ds) return rc;
-  const long long lds = ls_lds_bytes<T, SS>(nmax, BD);
+  const long long lds = ls_lds_bytes<T, SS, GG>(nmax, BD);
   if (lds > ds->lds_max) {
     set_last_error_msg("local search: n needs more LDS than a workgroup may use");
     return SIDON_ERR_LDS;
   }
   auto kfn = [] {
-    if constexpr (SHARED) return k_local_search_shared<T, BD, QMAX, SS, D>;
+    if constexpr (GG) return k_local_search_gg<T, BD, QMAX, SS, D>;
+    else if constexpr (SHARED) return k_local_search_shared<T, BD, QMAX, SS, D>;
     else return k_local_search<T, BD, QMAX, SS, D>;
   }();
   SIDON_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kfn),
@@ -610,6 +664,29 @@ int launch_ls(const LsArgs<T>& A, int nmax, hipStream_t stream) {
   return launch_ls_layout<T, BD, QMAX, SHARED, kSegStride, D>(A, nmax, stream);
 }
 
+// Lengths above the on-chip form: G in global memory (A.gscr), 1024 threads,
+// four operand steps in flight, the widest segment stride that fits.
+constexpr int kGgQmax = 13;  // n <= 13312; the plain layout's LDS ends at 13216
+template <typename T>
+bool ls_needs_gg(int nmax, long long lds_max) {
+  return ls_lds_bytes<T, kSegStride>(nmax, 1024) > lds_max;
+}
+int launch_ls_gg(const LsArgs<float>& A, int nmax, hipStream_t stream) {
+  int rc = SIDON_OK;
+  DeviceState* ds = device_state(&rc);
+  if (!ds) return rc;
+  if (!A.gscr) {
+    set_last_error_msg("local search: n above the on-chip form needs mom or vel "
+                       "(the launch's scratch for the correlation output)");
+    return SIDON_ERR_LDS;
+  }
+  if (ls_lds_bytes<float, kSegStrideWide, true>(nmax, 1024) <= ds->lds_max)
+    return launch_ls_layout<float, 1024, kGgQmax, false, kSegStrideWide, 4, true>(A, nmax, stream);
+  if (ls_lds_bytes<float, kSegStride, true>(nmax, 1024) <= ds->lds_max)
+    return launch_ls_layout<float, 1024, kGgQmax, false, kSegStride, 4, true>(A, nmax, stream);
+  return launch_ls_layout<float, 1024, kGgQmax, false, kSeg, 4, true>(A, nmax, stream);
+}
+
 // fp32 configurations by operand depth
 template <int BD, int QMAX, bool SHARED>
 int launch_ls_depth(const LsArgs<float>& A, int nmax, int depth, hipStream_t stream) {
@@ -622,10 +699,17 @@ int launch_ls_depth(const LsArgs<float>& A, int nmax, int depth, hipStream_t str
 // timeout count, then 16 progress ints per candidate, then per candidate the
 // b_k exchange of a split pair (256 per autoconvolution tile, and the two
 // halves' maxima).
+// Strides above the on-chip form (on a 160 KiB workgroup) add each
+// candidate's correlation output row (stride floats).
 constexpr size_t kWsHeader = 256;
 inline long long xchg_stride(int stride) { return 256LL * conv_tiles(stride) + 64; }
-inline size_t ragged_ws_bytes(int P, int stride) {
+inline bool ragged_ws_has_g(int stride) { return ls_needs_gg<float>(stride, 160LL * 1024); }
+inline size_t ragged_ws_g_offset(int P, int stride) {
   return kWsHeader + (size_t)P * 64 + (size_t)P * (size_t)xchg_stride(stride) * sizeof(float);
+}
+inline size_t ragged_ws_bytes(int P, int stride) {
+  return ragged_ws_g_offset(P, stride) +
+         (ragged_ws_has_g(stride) ? (size_t)P * (size_t)stride * sizeof(float) : 0);
 }
 
 // Clears the queue counter, the timeout count and the candidates' progress
@@ -652,6 +736,20 @@ int dispatch_ls(LsArgs<T>& A, int nmax, const LsTile& tile, void* ws, hipStream_
     A.pair_flags = wi + kWsHeader / sizeof(int);
     A.xchg = reinterpret_cast<T*>(static_cast<char*>(ws) + kWsHeader + (size_t)A.P * 64);
     A.xstride = xchg_stride((int)A.stride);
+    if (sizeof(T) == 4 && ragged_ws_has_g((int)A.stride)) {
+      A.gscr = reinterpret_cast<T*>(static_cast<char*>(ws) +
+                                    ragged_ws_g_offset(A.P, (int)A.stride));
+      A.gstride = A.stride;
+    }
+  }
+  if constexpr (sizeof(T) == 4) {  // above the on-chip form: one fixed configuration
+    int rc = SIDON_OK;
+    DeviceState* ds = device_state(&rc);
+    if (!ds) return rc;
+    if (ls_needs_gg<float>(nmax, ds->lds_max)) {
+      A.split = 0;
+      return launch_ls_gg(A, nmax, stream);
+    }
   }
   if constexpr (sizeof(T) == 8) {  // fp64: fixed configurations, two steps in flight
     switch (tile.block) {
@@ -687,17 +785,17 @@ int check_schedule(int iters, T pS, T pE, T lS, T lE) {
   return SIDON_OK;
 }
 
-template <typename T, int BD, int SS>
+template <typename T, int BD, int SS, bool GG = false>
 int launch_grad_layout(const GradArgs<T>& A, hipStream_t stream) {
   int rc = SIDON_OK;
   DeviceState* ds = device_state(&rc);
   if (!ds) return rc;
-  const long long lds = ls_lds_bytes<T, SS>(A.n, BD);
+  const long long lds = ls_lds_bytes<T, SS, GG>(A.n, BD);
   if (lds > ds->lds_max) {
     set_last_error_msg("lp_grad: n needs more LDS than a workgroup may use");
     return SIDON_ERR_LDS;
   }
-  auto kfn = k_lp_grad<T, BD, SS>;
+  auto kfn = k_lp_grad<T, BD, SS, GG>;
   SIDON_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kfn),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(kfn, dim3(A.B), dim3(BD), (size_t)lds, stream, A);
@@ -713,6 +811,19 @@ int launch_grad(const GradArgs<T>& A, hipStream_t stream) {
   if constexpr (sizeof(T) == 4)
     if (ls_lds_bytes<T, kSegStrideWide>(A.n, BD) <= ds->lds_max)
       return launch_grad_layout<T, BD, kSegStrideWide>(A, stream);
+  if (ls_lds_bytes<T, kSegStride>(A.n, BD) <= ds->lds_max)
+    return launch_grad_layout<T, BD, kSegStride>(A, stream);
+  // the correlation output does not fit beside a and the weights: it goes to
+  // the gradient rows (none when only J is asked for), with the widest
+  // segment stride that fits (lengths that long run 1024 threads)
+  if constexpr (BD == 1024) {
+    if constexpr (sizeof(T) == 4)
+      if (ls_lds_bytes<T, kSegStrideWide, true>(A.n, BD) <= ds->lds_max)
+        return launch_grad_layout<T, BD, kSegStrideWide, true>(A, stream);
+    if (ls_lds_bytes<T, kSegStride, true>(A.n, BD) <= ds->lds_max)
+      return launch_grad_layout<T, BD, kSegStride, true>(A, stream);
+    return launch_grad_layout<T, BD, kSeg, true>(A, stream);
+  }
   return launch_grad_layout<T, BD, kSegStride>(A, stream);
 }
 
@@ -768,6 +879,10 @@ extern "C" int sidon_local_search(float* pop, float* mom, float* vel, const int*
   if (P == 0 || iters == 0) return SIDON_OK;
   LsArgs<float> A{pop, n, nullptr, n, P, mom, vel, skip, mode, modeDefault, nullptr, nullptr,
                   iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2};
+  // above the on-chip form the moments' buffer is the correlation output's
+  // scratch until the final moments are written
+  A.gscr = mom ? mom : vel;
+  A.gstride = n;
   return dispatch_ls<float>(A, n, pick_tile(n), nullptr, stream);
 }
 
@@ -830,8 +945,18 @@ extern "C" long long sidon_ls_lds_bytes_impl(int kernel, int n, int block) {
   if (n < 1) return -1;
   switch (kernel) {
     case 0: case 4: {  // the layout a 160 KiB workgroup takes: wide where it fits
+      constexpr long long cap = 160LL * 1024;
       const long long wide = ls_lds_bytes<float, kSegStrideWide>(n, block);
-      return wide <= 160LL * 1024 ? wide : ls_lds_bytes<float>(n, block);
+      if (wide <= cap) return wide;
+      const long long narrow = ls_lds_bytes<float>(n, block);
+      if (narrow <= cap) return narrow;
+      // above the on-chip form: the correlation output in global memory
+      // (the local search always runs 1024 threads there)
+      const int bd = kernel == 0 ? 1024 : block;
+      const long long g24 = ls_lds_bytes<float, kSegStrideWide, true>(n, bd);
+      if (g24 <= cap) return g24;
+      const long long g20 = ls_lds_bytes<float, kSegStride, true>(n, bd);
+      return g20 <= cap ? g20 : ls_lds_bytes<float, kSeg, true>(n, bd);
     }
     case 1: return ls_lds_bytes<double>(n, block);
     default: return -1;

@@ -178,3 +178,22 @@ def test_search_any_ladder_ratio_and_fresh_seeds(O, tmp_path):
     v = O.verify_file(str(base) + ".txt")
     assert v["rc"] == 0 and v["n"] == final["n"]
     assert abs(final["best_F"] - final["host_recheck"]) <= 1e-12 * final["best_F"]
+
+
+def test_search_past_the_on_chip_length(O, tmp_path):
+    """A ladder into the lengths whose correlation output lives in global
+    memory (6608 -> 13216 by the cell split): the search runs there with its
+    moments' buffer as the launch's scratch, Frank-Wolfe is skipped where it
+    does not fit (src/main.cu:633-639), and the output verifies."""
+    base = tmp_path / "long"
+    out = run(["--pop", "32", "--islands", "4", "--elite", "2", "--ladder", "6608,13216",
+               "--ls-iters", "20", "--cold-iters", "40", "--fw-rounds", "1", "--fw-iters", "50",
+               "--gens", "2", "--seconds", "60", "--polish-iters", "200", "--polish-top", "2",
+               "--out", str(base), "--seed", "5"])
+    recs = [json.loads(line) for line in out.splitlines() if line.startswith("{")]
+    assert [r["level"] for r in recs if "level" in r] == [6608, 13216]
+    final = recs[-1]
+    assert final["n"] == 13216
+    v = O.verify_file(str(base) + ".txt")
+    assert v["rc"] == 0 and v["n"] == 13216
+    assert abs(final["best_F"] - final["host_recheck"]) <= 1e-12 * final["best_F"]

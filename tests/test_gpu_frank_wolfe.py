@@ -99,11 +99,13 @@ def test_skip_and_ladder(S, O, W):
             t = r
 
 
-def test_frank_wolfe_past_the_lds_is_refused(S):
-    """One height past FW_MAX_N the solve returns SIDON_ERR_LDS without a
-    launch (src/main.cu:173's "does not fit")."""
-    assert S.lds_bytes(3, FW_MAX_N, 1024) <= 160 * 1024 < S.lds_bytes(3, FW_MAX_N + 1, 1024)
-    t = torch.full((1, FW_MAX_N + 1), 1.0 / (FW_MAX_N + 1), dtype=torch.float32, device="cuda")
+@pytest.mark.parametrize("n", [FW_MAX_N + 1, 13216])
+def test_frank_wolfe_past_the_lds_is_refused(S, n):
+    """Past FW_MAX_N the solve returns SIDON_ERR_LDS without a launch
+    (src/main.cu:173's "does not fit"), also at lengths the local search
+    still runs (13216), so a driver skips Frank-Wolfe there."""
+    assert S.lds_bytes(3, FW_MAX_N, 1024) <= 160 * 1024 < S.lds_bytes(3, n, 1024)
+    t = torch.full((1, n), 1.0 / n, dtype=torch.float32, device="cuda")
     with pytest.raises(S.SidonError) as e:
         S.frank_wolfe(t, 1, 5)
     assert e.value.status == S.SIDON_ERR_LDS

@@ -67,11 +67,12 @@ def test_adjoint_planted_ties_n8400(S, O, planted_rows, p):
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 7, 15, 16, 17, 31, 33, 63, 65, 255, 257, 1000, 2047,
-                               4095])
+                               4095, 9297, 12177, 13216])
 def test_lp_grad_fp32_ragged_lengths(S, O, n):
     """fp32 J_p and gradient at p = 8 at every tile edge the kernel has (the
     symmetric autoconvolution's zone and finish, Hankel tile tails), per
-    height against the fp64 oracle."""
+    height against the fp64 oracle; 9297, 12177 and 13216 with the
+    correlation accumulated in the gradient rows (24- and 16-float layouts)."""
     print(f"[grad] fp32 ragged n={n}", flush=True)
     rng = np.random.default_rng(n)
     A = rng.uniform(0.1, 1.0, size=(3, n))

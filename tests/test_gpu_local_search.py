@@ -62,19 +62,26 @@ def ls_errors(got, want, gmin):
     return float(err.max()), keep
 
 
-# the longest fp32 candidate one workgroup holds (1024 threads, 20-float segments)
-LS_MAX_N = 9296
+# the longest fp32 candidate whose correlation output one workgroup's LDS
+# holds beside a and the weights (1024 threads, 20-float segments)
+LS_ONCHIP_N = 9296
+# the longest with the correlation output in global memory (the plain
+# 16-float segment layout); the 24- and 20-float layouts of that form end at
+# 11280 and 12176
+LS_MAX_N = 13216
 
 
 def _trajectory_vs_fp64(S, O, W, n, tile):
     """10 fp32 steps of the launch configuration `tile` (None: the built-in
-    rule) against the fp64 oracle from the same start."""
+    rule) against the fp64 oracle from the same start.  Above the on-chip
+    form the moments' buffer is passed: it is the launch's scratch."""
     a = W.structured_prior(2, n, seed=n).astype(np.float64)
     t = dev(a)
+    mom = torch.zeros_like(t) if n > LS_ONCHIP_N else None
     if tile:
         S.set_ls_tile(n, *tile)
     try:
-        S.local_search(t, 10, 4.0, 40.0, 0.05, 0.01)
+        S.local_search(t, 10, 4.0, 40.0, 0.05, 0.01, mom=mom)
     finally:
         if tile:
             S.set_ls_tile(n, 0)
@@ -91,14 +98,18 @@ def _trajectory_vs_fp64(S, O, W, n, tile):
 
 
 @pytest.mark.parametrize("n,tile", [(128, None), (600, None), (1023, None), (1200, None),
-                                    (2100, None), (4096, None), (8400, None), (LS_MAX_N, None),
+                                    (2100, None), (4096, None), (8400, None), (LS_ONCHIP_N, None),
+                                    (LS_ONCHIP_N + 1, None), (11281, None), (12177, None),
+                                    (LS_MAX_N, None),
                                     (1200, (1024, 4, 1)), (2100, (256, 4, 1)),
                                     (1600, (256, 4, 4)), (2304, (256, 4, 4))])
 def test_fp32_short_trajectory_vs_fp64(S, O, W, n, tile):
     """10 fp32 steps track the fp64 oracle (same start, kernel semantics), per
     height, excluding only heights whose direction came within VANISH * M of
     zero; the built-in configuration at each length, and others at the limits
-    of their lengths (2304: the longest four-a-CU candidate)."""
+    of their lengths (2304: the longest four-a-CU candidate; 9297, 11281,
+    12177: the first lengths of the correlation-in-global-memory form's 24-,
+    20- and 16-float layouts, 13216 its last)."""
     print(f"[ls] fp32 trajectory n={n} tile={tile}", flush=True)
     _trajectory_vs_fp64(S, O, W, n, tile)
 
@@ -551,15 +562,51 @@ def test_split_more_long_candidates_than_workgroups(S, W):
 def test_lengths_past_the_lds_are_refused(S):
     """One height past the longest candidate a workgroup's LDS holds, the fp32
     and fp64 searches return SIDON_ERR_LDS before any launch (the reference's
-    "does not fit", src/main.cu:173), and the lengths themselves still run."""
+    "does not fit", src/main.cu:173), and the lengths themselves still run.
+    Above the on-chip form the fp32 search needs the moments' buffer (or the
+    second moments') as its scratch and refuses to run without either."""
     assert 0 < S.lds_bytes(0, LS_MAX_N, 1024) <= 160 * 1024 < S.lds_bytes(0, LS_MAX_N + 1, 1024)
-    for n, dtype, fn in ((LS_MAX_N, torch.float32, S.local_search),
-                         (4544, torch.float64, S.local_search)):
+    for n, dtype in ((LS_MAX_N, torch.float32), (4544, torch.float64)):
         t = torch.full((1, n + 1), 1.0 / (n + 1), dtype=dtype, device="cuda")
         with pytest.raises(S.SidonError) as e:
-            fn(t, 2, 4.0, 8.0, 0.01, 0.005)
+            S.local_search(t, 2, 4.0, 8.0, 0.01, 0.005, mom=torch.zeros_like(t))
         assert e.value.status == S.SIDON_ERR_LDS
         ok = torch.full((1, n), 1.0 / n, dtype=dtype, device="cuda")
-        fn(ok, 2, 4.0, 8.0, 0.01, 0.005)
+        S.local_search(ok, 2, 4.0, 8.0, 0.01, 0.005, vel=torch.zeros_like(ok))
         torch.cuda.synchronize()
         assert bool(torch.isfinite(ok).all()) and abs(float(ok.sum()) - 1) < 1e-5
+    big = torch.full((1, LS_ONCHIP_N + 1), 1.0 / (LS_ONCHIP_N + 1), device="cuda")
+    with pytest.raises(S.SidonError) as e:
+        S.local_search(big, 2, 4.0, 8.0, 0.01, 0.005)
+    assert e.value.status == S.SIDON_ERR_LDS
+    assert float(big[0, 0]) == float(np.float32(1.0 / (LS_ONCHIP_N + 1)))   # nothing ran
+
+
+def test_correlation_in_global_memory_equals_lds_bitwise(S, W):
+    """A ragged launch whose stride is above the on-chip form runs every
+    candidate with the correlation output in global memory; at 600 and 4096
+    heights that must give the bits of the on-chip kernel with the same
+    configuration (1024 threads, four steps in flight), and the long
+    candidates the bits of the uniform launch at their length."""
+    print("[ls] correlation output in global memory vs LDS, bitwise", flush=True)
+    lengths = [600, 4096, 10000, 10000, 4096, 600]
+    stride = 10000
+    P = len(lengths)
+    pop = np.zeros((P, stride), np.float32)
+    for p, n in enumerate(lengths):
+        pop[p, :n] = W.structured_prior(1, n, seed=70 + p)[0]
+    kw = dict(iters=20, p_start=4.0, p_end=300.0, lr_start=0.05, lr_end=0.005)
+    t = dev(pop)
+    S.local_search_ragged(t, dev(np.array(lengths, np.int32), torch.int32), stride, **kw)
+    got = t.cpu().numpy()
+    for n in (600, 4096):
+        S.set_ls_tile(n, 1024, 4, 1)
+    try:
+        for p, n in enumerate(lengths):
+            u = dev(pop[p:p + 1, :n])
+            S.local_search(u, mom=torch.zeros_like(u), **kw)
+            assert np.array_equal(u.cpu().numpy()[0], got[p, :n]), (p, n)
+            assert (got[p, n:] == 0).all()
+    finally:
+        for n in (600, 4096):
+            S.set_ls_tile(n, 0)
