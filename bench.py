@@ -64,6 +64,9 @@ def parse():
                     help="time the whole-device fp64 polish alone and print its JSON")
     ap.add_argument("--f-evals-only", action="store_true",
                     help="run only the F-evaluation leg and print its rows as one JSON line")
+    ap.add_argument("--phase-clocks", action="store_true",
+                    help="load the profiling build of the library (make clocks) and print the "
+                         "local search's per-phase shader-clock breakdown at --n/--P/--iters")
     ap.add_argument("--profile-copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates the HBM "
                          "byte counters of a profiler run)")
@@ -171,6 +174,57 @@ def ls_by_n(S, W, torch, seed: int, P: int = 512, iters: int = 50):
                     "frac_fp32_peak": tf / PEAK_FP32,
                     "tile": dict(zip(("block", "depth", "per_cu"), S.get_ls_tile(n)))})
     return out
+
+
+def phase_clock_leg(S, W, torch, args):
+    """The headline launch (P candidates at n, `iters` steps of the anneal) on
+    the profiling build of the library: shader-clock cycles per candidate-step
+    of each phase (barrier to barrier, as each workgroup's first wave sees
+    them), the idle wave-cycles at the end of the two product phases, and the
+    matrix-core cycles the executed block steps need (four MFMAs of 32 cycles
+    per block step on one SIMD; a workgroup's 16 waves share four SIMDs).
+    The launch configuration is the library's built-in one for n."""
+    import numpy as np
+    n, P, iters = args.n, args.P, args.iters
+    sched = (4.0, 1e4, 0.085, 0.0015)
+    start = torch.as_tensor(W.structured_prior(P, n, seed=args.seed), device="cuda")
+    pop = start.clone()
+    mom = torch.empty_like(pop) if n > LS_ONCHIP_N else None
+    S.local_search(pop, iters, *sched, mom=mom)      # warm-up
+    torch.cuda.synchronize()
+    S.ls_phase_clocks(reset=True)
+    S.ls_block_steps(reset=True)
+    stream = torch.cuda.current_stream()
+    times = []
+    for _ in range(args.steps):
+        pop.copy_(start)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        S.local_search(pop, iters, *sched, mom=mom)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        times.append(ev0.elapsed_time(ev1))
+    clk = S.ls_phase_clocks(reset=True)
+    blocks = S.ls_block_steps(reset=True)
+    steps = clk[6]
+    waves = S.get_ls_tile(n)[0] // 64
+    per = lambda x: x / steps
+    phases = {"autoconvolution": per(clk[0]), "weights": per(clk[1]), "correlation": per(clk[2]),
+              "update": per(clk[3])}
+    total = sum(phases.values())
+    matrix = blocks * 128.0 / 4.0 / steps            # cycles of matrix work per candidate-step
+    return {"n": n, "P": P, "iters": iters, "launches": args.steps,
+            "ms_per_launch": float(np.median(times)),
+            "tile": dict(zip(("block", "depth", "per_cu"), S.get_ls_tile(n))),
+            "candidate_steps": steps, "candidates": clk[7],
+            "cycles_per_candidate_step": phases, "total": total,
+            "idle_wave_cycles_per_candidate_step": {"autoconvolution_end": per(clk[4]),
+                                                    "correlation_end": per(clk[5]),
+                                                    "waves": waves},
+            "matrix_cycles_per_candidate_step": matrix,
+            "block_steps_per_candidate_step": blocks / steps,
+            "matrix_share_of_step": matrix / total}
 
 
 def f_evals(S, W, torch, with_cpu: bool, seed: int):
@@ -332,6 +386,8 @@ def main() -> int:
 
     S = importlib.import_module("sidon-autocorrelation-bound_amd")
     W = importlib.import_module("sidon-autocorrelation-bound_amd.workloads")
+    if args.phase_clocks:
+        S.use_phase_clock_library()
     S.init(0)
     host = {"nproc": os.cpu_count(),
             "threads": int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)}
@@ -343,6 +399,9 @@ def main() -> int:
         del src, dst
     if args.tune_only:
         print(json.dumps({"autotune": tune_table(S, args), "host": host}))
+        return 0
+    if args.phase_clocks:
+        print(json.dumps({"phase_clocks": phase_clock_leg(S, W, torch, args), "host": host}))
         return 0
     if args.fw_only:
         print(json.dumps({"frank_wolfe": fw_legs(S, W, torch, args.seed), "host": host}))
@@ -378,10 +437,13 @@ def main() -> int:
     start = torch.as_tensor(W.structured_prior(P, n, seed=args.seed), device="cuda")
     pop = start.clone()
     stream = torch.cuda.current_stream()
+    # above the on-chip form the launch keeps its correlation output in the
+    # moments' buffer
+    mom = torch.empty_like(pop) if n > LS_ONCHIP_N else None
 
     for _ in range(args.warmup):
         pop.copy_(start)
-        S.local_search(pop, iters, *sched)
+        S.local_search(pop, iters, *sched, mom=mom)
     torch.cuda.synchronize()
 
     times = []
@@ -392,7 +454,7 @@ def main() -> int:
         ev1 = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         ev0.record(stream)
-        S.local_search(pop, iters, *sched)
+        S.local_search(pop, iters, *sched, mom=mom)
         ev1.record(stream)
         torch.cuda.synchronize()
         times.append(ev0.elapsed_time(ev1) / 1e3)
@@ -409,8 +471,10 @@ def main() -> int:
         ev1 = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         ev0.record(stream)
+        pmom = torch.empty_like(plain) if mom is not None else None
         for p in range(k):
-            S.local_search(plain[p:p + 1], iters, *sched)
+            S.local_search(plain[p:p + 1], iters, *sched,
+                           mom=pmom[p:p + 1] if pmom is not None else None)
         ev1.record(stream)
         torch.cuda.synchronize()
         plain_value = k * iters / (ev0.elapsed_time(ev1) / 1e3)

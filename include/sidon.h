@@ -328,6 +328,17 @@ int sidon_measure_fp32_ceilings(float* scratch, int scratch_len, double* valu_tf
  * step that a concurrently running launch adds.  Synchronous.  For roofline
  * accounting; no reference counterpart. */
 int sidon_ls_block_steps(unsigned long long* steps, int reset);
+/* Profiling builds of the library only (`make clocks`: libsidon_amd_clocks.so,
+ * the local search compiled with SIDON_PHASE_CLOCKS); the product library
+ * returns SIDON_ERR_UNSUPPORTED.  out[8]: shader-clock cycles of the local
+ * search's step phases as its first wave sees them, barrier to barrier,
+ * summed over steps and workgroups ([0] autoconvolution with finish and max,
+ * [1] weights and R, [2] correlation, [3] update, S and renormalisation), the
+ * wave-cycles spent between a wave's last work item and the end of the
+ * autoconvolution [4] / correlation [5] barrier, then the steps [6] and
+ * candidates [7] counted.  Synchronous; reset != 0 zeroes the sums.  No
+ * reference counterpart. */
+int sidon_ls_phase_clocks(unsigned long long* out, int reset);
 /* The same count for sidon_frank_wolfe: Toeplitz steps of the dual updates,
  * of the M and line-search autoconvolutions, and the correlation steps run
  * (steps over all-zero multipliers are skipped).  No reference counterpart. */

@@ -41,7 +41,7 @@ EXPORTS = (
     "sidon_generation_create", "sidon_generation_run", "sidon_generation_current",
     "sidon_generation_enqueue", "sidon_generation_destroy", "sidon_tune_local_search",
     "sidon_measure_fp32_ceilings",
-    "sidon_ls_block_steps", "sidon_fw_block_steps",
+    "sidon_ls_block_steps", "sidon_ls_phase_clocks", "sidon_fw_block_steps",
     "sidon_policy_create", "sidon_policy_destroy", "sidon_policy_adopt", "sidon_policy_new_level",
     "sidon_policy_observe", "sidon_policy_plan", "sidon_policy_credit", "sidon_policy_stats",
     "sidon_policy_tiers",
@@ -140,12 +140,32 @@ def _declare(L):
                                      C.POINTER(_i), C.POINTER(_f), C.POINTER(_f), _vp], _i),
         "sidon_measure_fp32_ceilings": ([_vp, _i, C.POINTER(_d), C.POINTER(_d), _vp], _i),
         "sidon_ls_block_steps": ([C.POINTER(C.c_ulonglong), _i], _i),
+        "sidon_ls_phase_clocks": ([C.POINTER(C.c_ulonglong), _i], _i),
         "sidon_fw_block_steps": ([C.POINTER(C.c_ulonglong), _i], _i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = res
+
+
+def use_phase_clock_library() -> None:
+    """Load the profiling build of the library (libsidon_amd_clocks.so, `make
+    clocks`: the local search records per-phase shader-clock sums) instead of
+    the product one.  Must come before the first call into the library."""
+    global LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("the library is already loaded")
+    LIB_PATH = HERE / "libsidon_amd_clocks.so"
+
+
+def ls_phase_clocks(reset: bool = False) -> list:
+    """The profiling build's phase sums (sidon_ls_phase_clocks): cycles of the
+    autoconvolution, weights, correlation and update phases, idle wave-cycles
+    at the end of the two product phases, steps, candidates."""
+    out = (C.c_ulonglong * 8)()
+    _check(lib().sidon_ls_phase_clocks(out, int(reset)), "sidon_ls_phase_clocks")
+    return [int(x) for x in out]
 
 
 def lib():

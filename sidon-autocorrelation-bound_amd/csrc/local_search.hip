@@ -31,6 +31,64 @@ __global__ void k_take_block_steps(int reset) {
   g_block_steps_taken = reset ? atomicExch(&g_block_steps, 0ull) : atomicAdd(&g_block_steps, 0ull);
 }
 
+// ------------------------------------------------------- phase clocks --
+// Profiling builds only (SIDON_PHASE_CLOCKS, the library's `clocks` make
+// target; the product library compiles none of this): shader-clock cycles of
+// each phase of a step, from barrier to barrier as wave 0 sees them, summed
+// over steps and workgroups, and the wave-cycles the waves spend at the end
+// of each product phase between finishing their last work item and leaving
+// the barrier.  Slots: [0] autoconvolution + finish + max, [1] weights + R,
+// [2] correlation, [3] update + S + renormalisation, [4] / [5] idle
+// wave-cycles at the end of the autoconvolution / correlation, [6] steps,
+// [7] candidates.
+constexpr int kPhaseSlots = 8;
+#ifdef SIDON_PHASE_CLOCKS
+__device__ unsigned long long g_phase_clk[kPhaseSlots];
+__device__ unsigned long long g_phase_clk_taken[kPhaseSlots];
+__global__ void k_take_phase_clocks(int reset) {
+  for (int i = 0; i < kPhaseSlots; ++i)
+    g_phase_clk_taken[i] = reset ? atomicExch(&g_phase_clk[i], 0ull)
+                                 : atomicAdd(&g_phase_clk[i], 0ull);
+}
+struct PhaseClock {
+  unsigned long long t = 0, acc[4] = {0, 0, 0, 0};
+  unsigned long long* idle;  // LDS: [0] autoconvolution end, [1] correlation end
+  unsigned long long done = 0;
+  __device__ void start(unsigned long long* lds_idle) {
+    idle = lds_idle;
+    if (threadIdx.x == 0) { idle[0] = 0; idle[1] = 0; }
+    t = clock64();
+  }
+  __device__ void mark(int ph) {
+    const unsigned long long now = clock64();
+    acc[ph] += now - t;
+    t = now;
+  }
+  __device__ void item_end() { done = clock64(); }  // this wave's last item is done
+  __device__ void after_barrier(int which) {       // wave-cycles spent waiting
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(&idle[which], clock64() - done);
+  }
+  __device__ void finish(int steps) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int i = 0; i < 4; ++i) atomicAdd(&g_phase_clk[i], acc[i]);
+      atomicAdd(&g_phase_clk[4], idle[0]);
+      atomicAdd(&g_phase_clk[5], idle[1]);
+      atomicAdd(&g_phase_clk[6], (unsigned long long)steps);
+      atomicAdd(&g_phase_clk[7], 1ull);
+    }
+  }
+};
+#else
+struct PhaseClock {
+  __device__ void start(unsigned long long*) {}
+  __device__ void mark(int) {}
+  __device__ void item_end() {}
+  __device__ void after_barrier(int) {}
+  __device__ void finish(int) {}
+};
+#endif
+
 template <typename T>
 struct LsArgs {
   T* pop;
@@ -182,6 +240,12 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
   T m[QMAX], v[QMAX];
 #pragma unroll
   for (int q = 0; q < QMAX; ++q) { m[q] = T(0); v[q] = T(0); }
+  PhaseClock clk;
+#ifdef SIDON_PHASE_CLOCKS
+  __shared__ unsigned long long s_idle[2];
+  clk.start(s_idle);
+#endif
+  int steps_run = 0;
   __syncthreads();
 
   const int nb = 2 * n - 1;
@@ -198,7 +262,9 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
 
     // sC[k] = b_k, *bmax = M (this half's tiles only when paired)
     autoconv_mfma<T, D, SS>(sA, sC, n, ctr, citems, nsteps, bmax, half);
+    clk.item_end();
     __syncthreads();
+    clk.after_barrier(0);
     if (half >= 0) {  // exchange b_k of the owned tiles and the tile maxima
       for (int k = tid; k < 256 * NTc; k += BD)
         if (conv_item_owner(crank[k >> 8]) == half) coherent_store(xb + k, sC[k]);
@@ -217,6 +283,8 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
     const int st = opaque(tid);  // per-step copy: nothing derived from it is hoisted
     const T M = *bmax;
     if (!(M > T(0))) break;
+    clk.mark(0);
+    ++steps_run;
 
     const T pm1 = pexp - T(1);
     // fp32 weights flush to zero once (p-1) log2(b/M) < -126, i.e. b < M exp(-126 ln2 / (p-1));
@@ -241,6 +309,7 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
       mark_segments(segmask, k - (st & (kWave - 1)), c != T(0));
     }
     const T R = block_sum<T, NW>(rloc, slots + NW);  // barrier also publishes segmask
+    clk.mark(1);
 
     b1t *= beta1;
     b2t *= beta2;
@@ -253,7 +322,10 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
       corr_mfma<T, D, SS>(sC, sA, sG, n, ctr + 1, gitems, segmask, nsteps, half);
     }
     const int ut = opaque(tid);  // element addresses below are built per step, not hoisted
+    clk.item_end();
     __syncthreads();
+    clk.after_barrier(1);
+    clk.mark(2);
 
     if (tid == 0) { ctr[1] = 0; *bmax = T(0); }  // every wave is past the correlation (and M)
     // the correlation is done with the segment bits; clearing them here, three
@@ -326,7 +398,9 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
     if constexpr (GG) wait_global_writes();  // G's zeros land before the next correlation
     else for (int i = tid; i < LG; i += BD) sG[i] = T(0);
     __syncthreads();
+    clk.mark(3);
   }
+  clk.finish(steps_run);
 
   if (half >= 0)  // the partner has read this half's last heights from the row
     pair_release(myflag, kPairFinal), pair_acquire(peerflag, kPairFinal, A.pair_err, broken);
@@ -961,6 +1035,25 @@ extern "C" long long sidon_ls_lds_bytes_impl(int kernel, int n, int block) {
     case 1: return ls_lds_bytes<double>(n, block);
     default: return -1;
   }
+}
+
+extern "C" int sidon_ls_phase_clocks(unsigned long long* out, int reset) {
+  if (!out) return SIDON_ERR_ARG;
+#ifdef SIDON_PHASE_CLOCKS
+  int rc = SIDON_OK;
+  if (!device_state(&rc)) return rc;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
+  hipLaunchKernelGGL(k_take_phase_clocks, dim3(1), dim3(1), 0, 0, reset);
+  SIDON_HIP_TRY(hipGetLastError());
+  SIDON_HIP_TRY(hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase_clk_taken),
+                                    sizeof(unsigned long long) * kPhaseSlots));
+  return SIDON_OK;
+#else
+  (void)reset;
+  set_last_error_msg("phase clocks: this library was built without SIDON_PHASE_CLOCKS");
+  return SIDON_ERR_UNSUPPORTED;
+#endif
 }
 
 extern "C" int sidon_ls_block_steps(unsigned long long* out, int reset) {
