@@ -41,8 +41,8 @@ __global__ void k_take_block_steps(int reset) {
 // [2] correlation, [3] update + S + renormalisation, [4] / [5] idle
 // wave-cycles at the end of the autoconvolution / correlation, [6] steps,
 // [7] candidates.
-constexpr int kPhaseSlots = 8;
 #ifdef SIDON_PHASE_CLOCKS
+constexpr int kPhaseSlots = 8;
 __device__ unsigned long long g_phase_clk[kPhaseSlots];
 __device__ unsigned long long g_phase_clk_taken[kPhaseSlots];
 __global__ void k_take_phase_clocks(int reset) {
