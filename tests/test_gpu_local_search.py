@@ -90,10 +90,11 @@ def _trajectory_vs_fp64(S, O, W, n, tile):
         want, Fw, _, _, gmin = O.local_search_gmin(a[p].astype(np.float32).astype(np.float64),
                                                    10, 4.0, 40.0, 0.05, 0.01)
         worst, keep = ls_errors(got[p], want, gmin)
-        print(f"   row {p}: max rel err {worst:.3e} over {keep.sum()}/{n} heights")
-        assert worst < LS10_BOUND, worst
         Fg = O.F_hostorder(got[p])
-        assert abs(Fg - Fw) < 2e-6 * Fw, (Fg, Fw)
+        print(f"   row {p}: max rel err {worst:.3e} over {keep.sum()}/{n} heights, "
+              f"F rel diff {abs(Fg - Fw) / Fw:.3e}")
+        assert worst < LS10_BOUND, worst
+        assert abs(Fg - Fw) < FP32_F_BOUND * Fw, (Fg, Fw)
         assert (got[p] >= 0).all() and abs(got[p].sum() - 1) < 1e-5
 
 
@@ -126,6 +127,9 @@ def test_every_tuner_configuration_vs_fp64(S, O, W, i):
 
 # about 4x the worst error measured on MI355X (5e-7)
 LS10_BOUND = 2e-6
+# F of the fp32 search's vectors against the fp64 trajectory's F: the bound the
+# fp32 search is held to (1e-6 relative; measured up to 7.4e-7, at n = 1200)
+FP32_F_BOUND = 1e-6
 
 
 def test_selftest_improvement(S, O):
@@ -311,9 +315,11 @@ def test_high_p_skips_zero_weights_and_tracks_oracle(S, O, W):
         want, Fw, _, _, gmin = O.local_search_gmin(a[p].astype(np.float32).astype(np.float64), 4,
                                                    1e4, 1e4, 0.02, 0.02)
         worst, keep = ls_errors(got[p], want, gmin)
-        print(f"   row {p}: max rel err {worst:.3e} over {keep.sum()}/{n} heights")
+        Fg = O.F_hostorder(got[p])
+        print(f"   row {p}: max rel err {worst:.3e} over {keep.sum()}/{n} heights, "
+              f"F rel diff {abs(Fg - Fw) / Fw:.3e}")
         assert worst < LS10_BOUND, worst
-        assert abs(O.F_hostorder(got[p]) - Fw) < 2e-6 * Fw
+        assert abs(Fg - Fw) < FP32_F_BOUND * Fw
 
 
 # One fp32 step from zero moments at every residue of n mod 16 the tile code
