@@ -67,6 +67,9 @@ def parse():
     ap.add_argument("--phase-clocks", action="store_true",
                     help="load the profiling build of the library (make clocks) and print the "
                          "local search's per-phase shader-clock breakdown at --n/--P/--iters")
+    ap.add_argument("--no-moments", action="store_true",
+                    help="--phase-clocks without the moments' buffers (the correlation output in "
+                         "LDS)")
     ap.add_argument("--profile-copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates the HBM "
                          "byte counters of a profiler run)")
@@ -156,14 +159,14 @@ def ls_by_n(S, W, torch, seed: int, P: int = 512, iters: int = 50):
     for n in (512, 1024, 2048, 4096, 8400, 10000, 13216):
         start = torch.as_tensor(W.structured_prior(P, n, seed=seed + 7 * n), device="cuda")
         pop = start.clone()
-        mom = torch.empty_like(pop) if n > LS_ONCHIP_N else None
-        S.local_search(pop, 5, *sched, mom=mom)         # warm-up
+        mom, vel = torch.empty_like(pop), torch.empty_like(pop)
+        S.local_search(pop, 5, *sched, mom=mom, vel=vel)  # warm-up
         pop.copy_(start)
         torch.cuda.synchronize()
         S.ls_block_steps(reset=True)
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
-        S.local_search(pop, iters, *sched, mom=mom)
+        S.local_search(pop, iters, *sched, mom=mom, vel=vel)
         ev1.record(stream)
         torch.cuda.synchronize()
         sec = ev0.elapsed_time(ev1) / 1e3
@@ -189,8 +192,10 @@ def phase_clock_leg(S, W, torch, args):
     sched = (4.0, 1e4, 0.085, 0.0015)
     start = torch.as_tensor(W.structured_prior(P, n, seed=args.seed), device="cuda")
     pop = start.clone()
-    mom = torch.empty_like(pop) if n > LS_ONCHIP_N else None
-    S.local_search(pop, iters, *sched, mom=mom)      # warm-up
+    # the moments' buffers, as in the headline launch; --no-moments keeps the
+    # correlation output in LDS instead (the form without them)
+    mom, vel = (None, None) if args.no_moments else (torch.empty_like(pop), torch.empty_like(pop))
+    S.local_search(pop, iters, *sched, mom=mom, vel=vel)  # warm-up
     torch.cuda.synchronize()
     S.ls_phase_clocks(reset=True)
     S.ls_block_steps(reset=True)
@@ -201,7 +206,7 @@ def phase_clock_leg(S, W, torch, args):
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         ev0.record(stream)
-        S.local_search(pop, iters, *sched, mom=mom)
+        S.local_search(pop, iters, *sched, mom=mom, vel=vel)
         ev1.record(stream)
         torch.cuda.synchronize()
         times.append(ev0.elapsed_time(ev1))
@@ -215,6 +220,7 @@ def phase_clock_leg(S, W, torch, args):
     total = sum(phases.values())
     matrix = blocks * 128.0 / 4.0 / steps            # cycles of matrix work per candidate-step
     return {"n": n, "P": P, "iters": iters, "launches": args.steps,
+            "correlation_output": "LDS" if args.no_moments else "moments' buffer rows",
             "ms_per_launch": float(np.median(times)),
             "tile": dict(zip(("block", "depth", "per_cu"), S.get_ls_tile(n))),
             "candidate_steps": steps, "candidates": clk[7],
@@ -437,13 +443,14 @@ def main() -> int:
     start = torch.as_tensor(W.structured_prior(P, n, seed=args.seed), device="cuda")
     pop = start.clone()
     stream = torch.cuda.current_stream()
-    # above the on-chip form the launch keeps its correlation output in the
-    # moments' buffer
-    mom = torch.empty_like(pop) if n > LS_ONCHIP_N else None
+    # the moments' buffers, as the reference's runLocalSearch passes them
+    # (src/main.cu:152-159); the 1024-thread launch keeps its correlation
+    # output in the rows of the first until the final moments are written
+    mom, vel = torch.empty_like(pop), torch.empty_like(pop)
 
     for _ in range(args.warmup):
         pop.copy_(start)
-        S.local_search(pop, iters, *sched, mom=mom)
+        S.local_search(pop, iters, *sched, mom=mom, vel=vel)
     torch.cuda.synchronize()
 
     times = []
@@ -454,7 +461,7 @@ def main() -> int:
         ev1 = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         ev0.record(stream)
-        S.local_search(pop, iters, *sched, mom=mom)
+        S.local_search(pop, iters, *sched, mom=mom, vel=vel)
         ev1.record(stream)
         torch.cuda.synchronize()
         times.append(ev0.elapsed_time(ev1) / 1e3)
@@ -471,10 +478,10 @@ def main() -> int:
         ev1 = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         ev0.record(stream)
-        pmom = torch.empty_like(plain) if mom is not None else None
+        pmom, pvel = torch.empty_like(plain), torch.empty_like(plain)
         for p in range(k):
-            S.local_search(plain[p:p + 1], iters, *sched,
-                           mom=pmom[p:p + 1] if pmom is not None else None)
+            S.local_search(plain[p:p + 1], iters, *sched, mom=pmom[p:p + 1],
+                           vel=pvel[p:p + 1])
         ev1.record(stream)
         torch.cuda.synchronize()
         plain_value = k * iters / (ev0.elapsed_time(ev1) / 1e3)

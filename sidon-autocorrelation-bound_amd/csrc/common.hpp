@@ -423,11 +423,39 @@ __device__ __forceinline__ void put4(T* out, int col, const A4& acc, bool whole)
   }
 }
 
+// Device-scope (agent) accesses: coherent between workgroups on different
+// XCDs, whose L2 caches are separate (the two halves of a split candidate).
+template <typename T>
+__device__ __forceinline__ void coherent_store(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T coherent_load(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Workgroup-scope accesses to a buffer in global memory that one workgroup
+// owns (a candidate's correlation output row): coherent among the waves of
+// the workgroup, which share one CU and one L2, so the lines stay in L2
+// instead of being written through to memory as device-scope ones are.
+template <typename T>
+__device__ __forceinline__ void wg_store(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename T>
+__device__ __forceinline__ T wg_load(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename T>
+__device__ __forceinline__ void wg_add(T* p, T v) {
+  [[clang::atomic(no_remote_memory, no_fine_grained_memory, ignore_denormal_mode)]] {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+}
 // put4 into a candidate's row in global memory (outputs i < n only; rows past
-// n belong to the next candidate): whole tiles by device-coherent stores,
-// halves of a shared tile by device-scope atomic adds onto an exact zero.
-// Readers in the same workgroup wait for these (s_waitcnt vmcnt(0) before
-// their barrier) and read with coherent_load, so no stale L1 line is seen.
+// n belong to the next candidate): whole tiles by stores, halves of a shared
+// tile by atomic adds onto an exact zero, both at workgroup scope.  Readers in
+// the same workgroup wait for these (s_waitcnt vmcnt(0) before their barrier,
+// whose workgroup-scope fences order them) and read with wg_load.
 template <typename T, typename A4>
 __device__ __forceinline__ void put4_global(T* out, int col, const A4& acc, bool whole, int n) {
   const int kk = (opaque((int)threadIdx.x) & (kWave - 1)) >> 4;
@@ -437,18 +465,10 @@ __device__ __forceinline__ void put4_global(T* out, int col, const A4& acc, bool
   for (int r = 0; r < 4; ++r) {
     const int i = i0 + RS * r;
     if (i < n) {
-      if (whole) __hip_atomic_store(out + i, acc[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      else unsafeAtomicAdd(out + i, acc[r]);
+      if (whole) wg_store(out + i, acc[r]);
+      else wg_add(out + i, acc[r]);
     }
   }
-}
-template <typename T>
-__device__ __forceinline__ void coherent_store(T* p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ T coherent_load(const T* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // This wave's global stores and atomics have completed (before a barrier
 // that publishes them to the other waves of the workgroup).

@@ -181,11 +181,14 @@ __host__ __device__ inline long long ls_lds_bytes(int n, int block) {
 }
 
 // GG: the correlation output G lives in the candidate's row of A.gscr
-// (global memory; lengths whose G does not fit the LDS beside a and the
-// weights).  The correlation writes it there (put4_global), the update reads
-// and re-zeroes it with coherent accesses, and parks the unnormalised new
-// height in a's own LDS slot (only its owner thread reads that slot during the
-// update) until S is known.  No pairs in this form (half is always -1).
+// (global memory: required for lengths whose G does not fit the LDS beside a
+// and the weights, and faster for the 1024-thread launches, which take it
+// whenever the caller provides the rows).  The correlation writes it there
+// (put4_global), the update reads and re-zeroes it at workgroup scope and
+// parks the unnormalised new height in a's own LDS slot (only its owner
+// thread reads that slot during the update) until S is known.  The two halves
+// of a split candidate share the row: each writes, reads and zeroes only the
+// outputs of the correlation tiles it owns.
 template <typename T, int BD, int QMAX, int SS, int D, bool GG = false>
 __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned char* smem,
                              int* broken) {
@@ -194,7 +197,6 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
   const int n = A.ns ? A.ns[cand] : A.n;
   if (A.skip && A.skip[cand]) return;
   if (n <= 0) return;
-  if constexpr (GG) half = -1;
   const int mode = A.mode ? A.mode[cand] : A.modeDefault;
   T* __restrict__ a = A.pop + (size_t)cand * (size_t)A.stride;
 
@@ -226,7 +228,8 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
   for (int i = tid; i < LA; i += BD) sA[i] = T(0);
   for (int i = tid; i < LC; i += BD) sC[i] = T(0);
   if constexpr (GG) {
-    for (int i = tid; i < n; i += BD) coherent_store(gG + i, T(0));
+    for (int i = tid; i < n; i += BD)
+      if (half < 0 || corr_tile_owner(i >> 8) == half) wg_store(gG + i, T(0));
     wait_global_writes();  // zero before any wave's correlation adds onto it
   } else {
     for (int i = tid; i < LG; i += BD) sG[i] = T(0);
@@ -340,8 +343,8 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
         const T ai = sA[pad_addr<SS>(i)];
         T Gi;
         if constexpr (GG) {
-          Gi = coherent_load(gG + i);
-          coherent_store(gG + i, T(0));  // zero again for the next step's correlation
+          Gi = wg_load(gG + i);
+          wg_store(gG + i, T(0));  // zero again for the next step's correlation
         } else {
           Gi = sG[i];
         }
@@ -377,8 +380,13 @@ __device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned ch
       for (int q = 0; q < QMAX; ++q) {
         const int i = ut + BD * q;
         if (i < n) {
-          if (corr_tile_owner(i >> 8) != half) sG[i] = coherent_load(a + i);
-          sloc += sG[i];
+          if constexpr (GG) {
+            if (corr_tile_owner(i >> 8) != half) pad_put<SS>(sA, i, coherent_load(a + i));
+            sloc += sA[pad_addr<SS>(i)];
+          } else {
+            if (corr_tile_owner(i >> 8) != half) sG[i] = coherent_load(a + i);
+            sloc += sG[i];
+          }
         }
       }
     }
@@ -516,7 +524,7 @@ __device__ __forceinline__ void ls_body(const LsArgs<T>& A, unsigned char* smem,
     ls_candidate<T, BD, QMAX, SS, D, GG>(A, cand, -1, smem, broken);
     return;
   }
-  const int split = GG ? 0 : split_count<T, BD>(A, &s_item, smem);
+  const int split = split_count<T, BD>(A, &s_item, smem);
   if (A.split_out && blockIdx.x == 0 && threadIdx.x == 0) *A.split_out = split;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(A.queue, 1);
@@ -603,7 +611,7 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   for (int i = tid; i < LC; i += BD) sC[i] = T(0);
   if constexpr (GG) {
     if (g) {
-      for (int i = tid; i < n; i += BD) coherent_store(g + i, T(0));
+      for (int i = tid; i < n; i += BD) wg_store(g + i, T(0));
       wait_global_writes();
     }
   } else {
@@ -639,7 +647,7 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   __syncthreads();
   const T scale = T(4) * T(n) * Rp / (S * S);
   const T MS = M / S;
-  for (int i = tid; i < n; i += BD) g[i] = scale * ((GG ? coherent_load(g + i) : sG[i]) / R - MS);
+  for (int i = tid; i < n; i += BD) g[i] = scale * ((GG ? wg_load(g + i) : sG[i]) / R - MS);
 }
 
 // ------------------------------------------------------------- host side --
@@ -738,26 +746,25 @@ int launch_ls(const LsArgs<T>& A, int nmax, hipStream_t stream) {
   return launch_ls_layout<T, BD, QMAX, SHARED, kSegStride, D>(A, nmax, stream);
 }
 
-// Lengths above the on-chip form: G in global memory (A.gscr), 1024 threads,
-// four operand steps in flight, the widest segment stride that fits.
+// The 1024-thread fp32 launch with G in global memory (A.gscr) and the plain
+// 16-float segment layout: required above the on-chip form's lengths, and
+// taken at every length the 1024-thread configuration runs when the caller
+// provides the rows (measured at n = 8400: 714 k candidate-steps/s against
+// 676 k with G in LDS and 24-float segments; the correlation's epilogues and
+// the update no longer go through the LDS the matrix operands are read from).
 constexpr int kGgQmax = 13;  // n <= 13312; the plain layout's LDS ends at 13216
 template <typename T>
 bool ls_needs_gg(int nmax, long long lds_max) {
   return ls_lds_bytes<T, kSegStride>(nmax, 1024) > lds_max;
 }
-int launch_ls_gg(const LsArgs<float>& A, int nmax, hipStream_t stream) {
-  int rc = SIDON_OK;
-  DeviceState* ds = device_state(&rc);
-  if (!ds) return rc;
+int launch_ls_gg(const LsArgs<float>& A, int nmax, int depth, hipStream_t stream) {
   if (!A.gscr) {
     set_last_error_msg("local search: n above the on-chip form needs mom or vel "
                        "(the launch's scratch for the correlation output)");
     return SIDON_ERR_LDS;
   }
-  if (ls_lds_bytes<float, kSegStrideWide, true>(nmax, 1024) <= ds->lds_max)
-    return launch_ls_layout<float, 1024, kGgQmax, false, kSegStrideWide, 4, true>(A, nmax, stream);
-  if (ls_lds_bytes<float, kSegStride, true>(nmax, 1024) <= ds->lds_max)
-    return launch_ls_layout<float, 1024, kGgQmax, false, kSegStride, 4, true>(A, nmax, stream);
+  if (depth == 2)
+    return launch_ls_layout<float, 1024, kGgQmax, false, kSeg, 2, true>(A, nmax, stream);
   return launch_ls_layout<float, 1024, kGgQmax, false, kSeg, 4, true>(A, nmax, stream);
 }
 
@@ -773,11 +780,12 @@ int launch_ls_depth(const LsArgs<float>& A, int nmax, int depth, hipStream_t str
 // timeout count, then 16 progress ints per candidate, then per candidate the
 // b_k exchange of a split pair (256 per autoconvolution tile, and the two
 // halves' maxima).
-// Strides above the on-chip form (on a 160 KiB workgroup) add each
-// candidate's correlation output row (stride floats).
+// Strides above 4096 (where the built-in rule runs 1024-thread workgroups)
+// add each candidate's correlation output row (stride floats).
 constexpr size_t kWsHeader = 256;
+constexpr int kWsRowsMinStride = 4097;
 inline long long xchg_stride(int stride) { return 256LL * conv_tiles(stride) + 64; }
-inline bool ragged_ws_has_g(int stride) { return ls_needs_gg<float>(stride, 160LL * 1024); }
+inline bool ragged_ws_has_g(int stride) { return stride >= kWsRowsMinStride; }
 inline size_t ragged_ws_g_offset(int P, int stride) {
   return kWsHeader + (size_t)P * 64 + (size_t)P * (size_t)xchg_stride(stride) * sizeof(float);
 }
@@ -816,14 +824,15 @@ int dispatch_ls(LsArgs<T>& A, int nmax, const LsTile& tile, void* ws, hipStream_
       A.gstride = A.stride;
     }
   }
-  if constexpr (sizeof(T) == 4) {  // above the on-chip form: one fixed configuration
+  if constexpr (sizeof(T) == 4) {
     int rc = SIDON_OK;
     DeviceState* ds = device_state(&rc);
     if (!ds) return rc;
-    if (ls_needs_gg<float>(nmax, ds->lds_max)) {
-      A.split = 0;
-      return launch_ls_gg(A, nmax, stream);
-    }
+    // above the on-chip form: 1024 threads, G in global memory (rows needed);
+    // 1024-thread launches with rows: the same form
+    if (ls_needs_gg<float>(nmax, ds->lds_max))
+      return launch_ls_gg(A, nmax, tile.block == 1024 ? tile.depth : 4, stream);
+    if (tile.block == 1024 && A.gscr) return launch_ls_gg(A, nmax, tile.depth, stream);
   }
   if constexpr (sizeof(T) == 8) {  // fp64: fixed configurations, two steps in flight
     switch (tile.block) {
@@ -1025,12 +1034,13 @@ extern "C" long long sidon_ls_lds_bytes_impl(int kernel, int n, int block) {
       const long long narrow = ls_lds_bytes<float>(n, block);
       if (narrow <= cap) return narrow;
       // above the on-chip form: the correlation output in global memory
-      // (the local search always runs 1024 threads there)
-      const int bd = kernel == 0 ? 1024 : block;
-      const long long g24 = ls_lds_bytes<float, kSegStrideWide, true>(n, bd);
+      // (the local search always runs 1024 threads there, plain layout; the
+      // gradient the widest layout that fits)
+      if (kernel == 0) return ls_lds_bytes<float, kSeg, true>(n, 1024);
+      const long long g24 = ls_lds_bytes<float, kSegStrideWide, true>(n, block);
       if (g24 <= cap) return g24;
-      const long long g20 = ls_lds_bytes<float, kSegStride, true>(n, bd);
-      return g20 <= cap ? g20 : ls_lds_bytes<float, kSeg, true>(n, bd);
+      const long long g20 = ls_lds_bytes<float, kSegStride, true>(n, block);
+      return g20 <= cap ? g20 : ls_lds_bytes<float, kSeg, true>(n, block);
     }
     case 1: return ls_lds_bytes<double>(n, block);
     default: return -1;

@@ -592,8 +592,9 @@ def test_correlation_in_global_memory_equals_lds_bitwise(S, W):
     """A ragged launch whose stride is above the on-chip form runs every
     candidate with the correlation output in global memory; at 600 and 4096
     heights that must give the bits of the on-chip kernel with the same
-    configuration (1024 threads, four steps in flight), and the long
-    candidates the bits of the uniform launch at their length."""
+    configuration (1024 threads, four steps in flight; no moments' buffer, so
+    G stays in LDS), and the long candidates the bits of the uniform launch
+    at their length."""
     print("[ls] correlation output in global memory vs LDS, bitwise", flush=True)
     lengths = [600, 4096, 10000, 10000, 4096, 600]
     stride = 10000
@@ -610,9 +611,36 @@ def test_correlation_in_global_memory_equals_lds_bitwise(S, W):
     try:
         for p, n in enumerate(lengths):
             u = dev(pop[p:p + 1, :n])
-            S.local_search(u, mom=torch.zeros_like(u), **kw)
+            S.local_search(u, mom=torch.zeros_like(u) if n > LS_ONCHIP_N else None, **kw)
             assert np.array_equal(u.cpu().numpy()[0], got[p, :n]), (p, n)
             assert (got[p, n:] == 0).all()
     finally:
         for n in (600, 4096):
             S.set_ls_tile(n, 0)
+
+
+@pytest.mark.parametrize("n", [4097, 6000, 8400])
+def test_moments_buffer_form_equals_lds_form_bitwise(S, W, n):
+    """With a moments' buffer the 1024-thread launch keeps its correlation
+    output in the buffer's rows (and the plain segment layout); without one,
+    in LDS.  The heights and the final moments are the same bits."""
+    print(f"[ls] G in the moments' rows vs in LDS, n={n}", flush=True)
+    P = 6
+    start = dev(W.structured_prior(P, n, seed=n + 3))
+    kw = dict(iters=30, p_start=4.0, p_end=1e3, lr_start=0.05, lr_end=0.005)
+    S.set_ls_tile(n, 1024, 4, 1)
+    try:
+        a = start.clone()
+        S.local_search(a, **kw)
+        b = start.clone()
+        mom, vel = torch.zeros_like(b), torch.zeros_like(b)
+        S.local_search(b, mom=mom, vel=vel, **kw)
+        c = start.clone()
+        vel2 = torch.zeros_like(c)
+        S.local_search(c, vel=vel2, **kw)      # vel alone is the scratch
+        torch.cuda.synchronize()
+    finally:
+        S.set_ls_tile(n, 0)
+    assert torch.equal(a, b) and torch.equal(a, c)
+    assert torch.equal(vel, vel2)
+    assert bool(torch.isfinite(mom).all()) and float(mom.abs().max()) > 0

@@ -75,13 +75,18 @@ def test_lds_footprints():
     assert S.lds_bytes(3, 8400, 1024) <= 160 * 1024
     assert S.lds_bytes(2, 8400, 256) <= 160 * 1024
     assert S.lds_bytes(1, 8400, 512) > 160 * 1024   # fp64 local search does not fit at 8400
-    # above 9296 the correlation output moves to global memory: 24-float
-    # segments to 11280, 20-float to 12176, the plain layout to 13216
+    # above 9296 the correlation output moves to global memory: the local
+    # search with the plain 16-float layout to 13216, the gradient with the
+    # widest that fits (24-float segments to 11280, 20-float to 12176, 16)
     for n in (9297, 11280, 11281, 12176, 12177, 13216):
         assert 0 < S.lds_bytes(0, n, 1024) <= 160 * 1024, n
-        assert S.lds_bytes(4, n, 1024) == S.lds_bytes(0, n, 1024)
+        assert 0 < S.lds_bytes(4, n, 1024) <= 160 * 1024, n
     assert S.lds_bytes(0, 13217, 1024) > 160 * 1024
-    assert S.lds_bytes(0, 11281, 1024) < S.lds_bytes(0, 11280, 1024)   # narrower segments
+    assert S.lds_bytes(4, 13217, 1024) > 160 * 1024
+    assert S.lds_bytes(0, 11281, 1024) > S.lds_bytes(0, 11280, 1024)   # one layout throughout
+    assert S.lds_bytes(4, 11281, 1024) < S.lds_bytes(4, 11280, 1024)   # narrower segments
+    assert S.lds_bytes(4, 12176, 1024) == S.lds_bytes(0, 12176, 1024) + 4 * (
+        (12176 + 15) // 16 + 32) * 4 - 16   # 20- against 16-float segments
     assert S.lds_bytes(9, 100, 256) == -1
 
 
