@@ -482,7 +482,7 @@ __device__ __forceinline__ void wait_global_writes() {
 // call (two commuting partial sums per element at most).  Every wave calls
 // this with its wave index; returns without a barrier.
 template <typename T, int SS = kSegStride>
-__device__ void conv_mfma(const T* __restrict__ sA, const T* __restrict__ sB, T* out, int n,
+__device__ __forceinline__ void conv_mfma(const T* __restrict__ sA, const T* __restrict__ sB, T* out, int n,
                           int wave, int nwaves) {
   const int N16 = nseg16(n);
   const int NT = conv_tiles(n);
@@ -769,7 +769,7 @@ __device__ __forceinline__ int conv_item_owner(int rank) {
 __device__ __forceinline__ int corr_tile_owner(int j) { return j & 1; }
 
 template <typename T, int D = Pipe<T>::kDepth, int SS = kSegStride>
-__device__ void autoconv_mfma(const T* __restrict__ sA, T* out, int n, int* ctr,
+__device__ __forceinline__ void autoconv_mfma(const T* __restrict__ sA, T* out, int n, int* ctr,
                               const unsigned* items, unsigned* steps = nullptr,
                               T* bmax = nullptr, int half = -1) {
   const int N16 = nseg16(n);
@@ -830,7 +830,7 @@ __device__ __forceinline__ void corr_items(int n, unsigned* items, int* scratch,
 // read only zero weights are skipped.
 // With GOUT, `out` is the candidate's row in global memory (put4_global).
 template <typename T, int D = Pipe<T>::kDepth, int SS = kSegStride, bool GOUT = false>
-__device__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* out, int n,
+__device__ __forceinline__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* out, int n,
                           int* ctr, const unsigned* items, const unsigned* segmask,
                           unsigned* steps = nullptr, int half = -1) {
   const int N16 = nseg16(n);

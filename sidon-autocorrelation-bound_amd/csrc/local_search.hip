@@ -190,7 +190,7 @@ __host__ __device__ inline long long ls_lds_bytes(int n, int block) {
 // of a split candidate share the row: each writes, reads and zeroes only the
 // outputs of the correlation tiles it owns.
 template <typename T, int BD, int QMAX, int SS, int D, bool GG = false>
-__device__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned char* smem,
+__device__ __forceinline__ void ls_candidate(const LsArgs<T>& A, int cand, int half, unsigned char* smem,
                              int* broken) {
   constexpr int NW = BD / kWave;
   // D: operand steps in flight in the tile loops (a tuning choice)
@@ -550,9 +550,9 @@ __global__ __launch_bounds__(BD) void k_local_search(LsArgs<T> A) {
   ls_body<T, BD, QMAX, SS, D>(A, smem, s_item, &s_broken);
 }
 
-// Lengths whose correlation output does not fit the LDS: G in global memory
-// (A.gscr), the segment stride as wide as the LDS allows (24, 20, or the
-// plain layout 16, whose B-operand reads are served in four passes).
+// The correlation output G in global memory (A.gscr, the caller's rows),
+// with the plain 16-float segment layout (whose B-operand reads are served in
+// four passes; measured faster all the same).
 template <typename T, int BD, int QMAX, int SS, int D>
 __global__ __launch_bounds__(BD) void k_local_search_gg(LsArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -570,6 +570,14 @@ k_local_search_shared(LsArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_item, s_broken;
   ls_body<T, BD, QMAX, SS, D>(A, smem, s_item, &s_broken);
+}
+// The same with the correlation output in global memory (A.gscr).
+template <typename T, int BD, int QMAX, int SS, int D>
+__global__ __launch_bounds__(BD) __attribute__((amdgpu_waves_per_eu(4, 4))) void
+k_local_search_shared_gg(LsArgs<T> A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int s_item, s_broken;
+  ls_body<T, BD, QMAX, SS, D, true>(A, smem, s_item, &s_broken);
 }
 
 // ---------------------------------------------------------------- lp_grad --
@@ -710,7 +718,8 @@ int launch_ls_layout(const LsArgs<T>& A, int nmax, hipStream_t stream) {
     return SIDON_ERR_LDS;
   }
   auto kfn = [] {
-    if constexpr (GG) return k_local_search_gg<T, BD, QMAX, SS, D>;
+    if constexpr (GG && SHARED) return k_local_search_shared_gg<T, BD, QMAX, SS, D>;
+    else if constexpr (GG) return k_local_search_gg<T, BD, QMAX, SS, D>;
     else if constexpr (SHARED) return k_local_search_shared<T, BD, QMAX, SS, D>;
     else return k_local_search<T, BD, QMAX, SS, D>;
   }();
@@ -733,16 +742,22 @@ int launch_ls_layout(const LsArgs<T>& A, int nmax, hipStream_t stream) {
   return SIDON_OK;
 }
 
-// The wide (bank-conflict-free) segment layout where it fits the LDS, else
-// the narrow one.
+// fp32 with the caller's rows for the correlation output (A.gscr): G in
+// global memory and the plain 16-float layout (faster at every configuration
+// measured: 6-10 % for two 512-thread candidates a CU at n = 1200-4096, 4-5 %
+// for 1024 threads at 4200-8400); otherwise the wide (bank-conflict-free)
+// segment layout where it fits the LDS, else the narrow one.
 template <typename T, int BD, int QMAX, bool SHARED, int D>
 int launch_ls(const LsArgs<T>& A, int nmax, hipStream_t stream) {
   int rc = SIDON_OK;
   DeviceState* ds = device_state(&rc);
   if (!ds) return rc;
-  if constexpr (sizeof(T) == 4)
+  if constexpr (sizeof(T) == 4) {
+    if constexpr (BD < 1024)  // 1024 threads: launch_ls_gg, 13 heights a thread
+      if (A.gscr) return launch_ls_layout<T, BD, QMAX, SHARED, kSeg, D, true>(A, nmax, stream);
     if (ls_lds_bytes<T, kSegStrideWide>(nmax, BD) <= ds->lds_max)
       return launch_ls_layout<T, BD, QMAX, SHARED, kSegStrideWide, D>(A, nmax, stream);
+  }
   return launch_ls_layout<T, BD, QMAX, SHARED, kSegStride, D>(A, nmax, stream);
 }
 
@@ -780,12 +795,10 @@ int launch_ls_depth(const LsArgs<float>& A, int nmax, int depth, hipStream_t str
 // timeout count, then 16 progress ints per candidate, then per candidate the
 // b_k exchange of a split pair (256 per autoconvolution tile, and the two
 // halves' maxima).
-// Strides above 4096 (where the built-in rule runs 1024-thread workgroups)
-// add each candidate's correlation output row (stride floats).
+// Then each candidate's correlation output row (stride floats).
 constexpr size_t kWsHeader = 256;
-constexpr int kWsRowsMinStride = 4097;
 inline long long xchg_stride(int stride) { return 256LL * conv_tiles(stride) + 64; }
-inline bool ragged_ws_has_g(int stride) { return stride >= kWsRowsMinStride; }
+inline bool ragged_ws_has_g(int) { return true; }
 inline size_t ragged_ws_g_offset(int P, int stride) {
   return kWsHeader + (size_t)P * 64 + (size_t)P * (size_t)xchg_stride(stride) * sizeof(float);
 }
