@@ -356,14 +356,19 @@ int sidon_ls_tile_config(int i, int* block, int* depth, int* per_cu);
  * `scratch` (P x n, overwritten; reset from `reference` before every trial
  * when non-NULL), `repeats` timed runs of `iters` steps after one warm-up,
  * and keeps the fastest as the rule for n (sidon_get_ls_tile); the built-in
- * configuration stays unless another beats it by more than 1%.  Reports the
- * choice and its time, and, when `trial_ms` is non-NULL, the time of every
- * configuration (SIDON_LS_TILES floats, -1 where it does not fit n).  No
- * reference counterpart (tile sizes are compile-time constants there,
- * src/kernels.cuh:30-33). */
-int sidon_tune_local_search(float* scratch, const float* reference, int n, int P, int iters,
-                            int repeats, int* best_block, int* best_depth, int* best_per_cu,
-                            float* best_ms, float* trial_ms, hipStream_t stream);
+ * configuration stays unless another beats it by more than 1%.  `rows`
+ * (P x n floats, may be NULL) is passed to the trials as their moments'
+ * buffer, so they time the form that launches with a moments' buffer run
+ * (correlation output in its rows); NULL times the form without one, and
+ * above n = 9296, which has no other form, reports the one configuration
+ * untimed.  Reports the choice and its time, and, when `trial_ms` is
+ * non-NULL, the time of every configuration (SIDON_LS_TILES floats, -1
+ * where it does not fit n).  No reference counterpart (tile sizes are
+ * compile-time constants there, src/kernels.cuh:30-33). */
+int sidon_tune_local_search(float* scratch, const float* reference, float* rows, int n, int P,
+                            int iters, int repeats, int* best_block, int* best_depth,
+                            int* best_per_cu, float* best_ms, float* trial_ms,
+                            hipStream_t stream);
 
 /* ---- island search policy (host only) ---------------------------------- */
 

@@ -136,8 +136,9 @@ def _declare(L):
         "sidon_generation_current": ([_vp, C.POINTER(_i)], _i),
         "sidon_generation_enqueue": ([C.POINTER(GenerationDesc), _i, _vp], _i),
         "sidon_generation_destroy": ([_vp], _i),
-        "sidon_tune_local_search": ([_vp, _vp, _i, _i, _i, _i, C.POINTER(_i), C.POINTER(_i),
-                                     C.POINTER(_i), C.POINTER(_f), C.POINTER(_f), _vp], _i),
+        "sidon_tune_local_search": ([_vp, _vp, _vp, _i, _i, _i, _i, C.POINTER(_i),
+                                     C.POINTER(_i), C.POINTER(_i), C.POINTER(_f),
+                                     C.POINTER(_f), _vp], _i),
         "sidon_measure_fp32_ceilings": ([_vp, _i, C.POINTER(_d), C.POINTER(_d), _vp], _i),
         "sidon_ls_block_steps": ([C.POINTER(C.c_ulonglong), _i], _i),
         "sidon_ls_phase_clocks": ([C.POINTER(C.c_ulonglong), _i], _i),
@@ -578,19 +579,22 @@ def fw_block_steps(reset: bool = False) -> int:
 
 
 def tune_local_search(n: int, P: int = 64, iters: int = 20, repeats: int = 2, seed: int = 1,
-                      stream=None):
+                      with_rows: bool = True, stream=None):
     """Autotune the launch configuration of the fused local search at length n
     (workgroup size, operand steps in flight, candidates a CU) on a
-    structured-prior sample; the choice is kept by the library for n.  Returns
-    ((block, depth, per_cu), ms of the choice, {configuration: ms} of every
-    trial that fits n)."""
+    structured-prior sample; the choice is kept by the library for n.  With
+    `with_rows` the trials run with a moments' buffer (the correlation output
+    in its rows), as launches that pass one do.  Returns ((block, depth,
+    per_cu), ms of the choice, {configuration: ms} of every trial that fits
+    n)."""
     _need_gpu()
     from .workloads import structured_prior
     ref = torch.as_tensor(structured_prior(P, n, seed=seed), device="cuda")
     scratch = torch.empty_like(ref)
+    rows = torch.empty_like(ref) if with_rows else None
     blk, dep, per, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
     trials = (C.c_float * LS_TILES)()
-    _check(lib().sidon_tune_local_search(_ptr(scratch), _ptr(ref), n, P, iters, repeats,
+    _check(lib().sidon_tune_local_search(_ptr(scratch), _ptr(ref), _ptr(rows), n, P, iters, repeats,
                                          C.byref(blk), C.byref(dep), C.byref(per), C.byref(ms),
                                          trials, _stream(stream)),
            "sidon_tune_local_search")

@@ -385,13 +385,15 @@ int uneven(const Args& A) {
   if (A.tune) {  // autotune the launch configuration at the longest length (it sets the launch)
     float* scratch = dalloc<float>((size_t)64 * nmax);
     float* ref = dalloc<float>((size_t)64 * nmax);
+    float* rows = dalloc<float>((size_t)64 * nmax);  // the launches run with rows (workspace)
     std::vector<float> r((size_t)64 * nmax);
     for (int p = 0; p < 64; ++p) structured_prior(rng, nmax, &r[(size_t)p * nmax]);
     CHECK_HIP(hipMemcpy(ref, r.data(), r.size() * sizeof(float), hipMemcpyHostToDevice));
     int blk = 0, depth = 0, per_cu = 0;
     float ms = 0.0f;
-    CHECK_SIDON(sidon_tune_local_search(scratch, ref, nmax, 64, 10, 2, &blk, &depth, &per_cu, &ms,
-                                        nullptr, s));
+    CHECK_SIDON(sidon_tune_local_search(scratch, ref, rows, nmax, 64, 10, 2, &blk, &depth, &per_cu,
+                                        &ms, nullptr, s));
+    CHECK_HIP(hipFree(rows));
     std::printf("{\"autotune\": {\"n\": %d, \"block\": %d, \"depth\": %d, \"per_cu\": %d, "
                 "\"ms\": %.3f}}\n",
                 nmax, blk, depth, per_cu, ms);

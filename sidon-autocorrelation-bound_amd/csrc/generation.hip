@@ -157,10 +157,10 @@ extern "C" int sidon_ls_tile_config(int i, int* block, int* depth, int* per_cu) 
 // `reference` before each trial when given) and keeps the fastest as the rule
 // for n.  The built-in configuration stays unless another beats it by more
 // than 1% (two runs of one configuration differ by a few tenths of a percent).
-extern "C" int sidon_tune_local_search(float* scratch, const float* reference, int n, int P,
-                                       int iters, int repeats, int* best_block, int* best_depth,
-                                       int* best_per_cu, float* best_ms, float* trial_ms,
-                                       hipStream_t stream) {
+extern "C" int sidon_tune_local_search(float* scratch, const float* reference, float* rows,
+                                       int n, int P, int iters, int repeats, int* best_block,
+                                       int* best_depth, int* best_per_cu, float* best_ms,
+                                       float* trial_ms, hipStream_t stream) {
   if (!scratch || n < 1 || P < 1 || iters < 1 || repeats < 1) return SIDON_ERR_ARG;
   (void)sidon_set_ls_tile(n, 0, 0, 0);
   int dblock = 0, ddepth = 0, dper = 0;
@@ -183,7 +183,7 @@ extern "C" int sidon_tune_local_search(float* scratch, const float* reference, i
         break;
       }
       (void)hipEventRecord(e0, stream);
-      const int lrc = sidon_local_search(scratch, nullptr, nullptr, nullptr, nullptr, n, P, iters,
+      const int lrc = sidon_local_search(scratch, rows, nullptr, nullptr, nullptr, n, P, iters,
                                          4.0f, 1e4f, 0.085f, 0.0015f, 0.9f, 0.999f, 0, stream);
       (void)hipEventRecord(e1, stream);
       if (lrc == SIDON_ERR_LDS || lrc == SIDON_ERR_UNSUPPORTED) { ok = false; break; }
@@ -203,9 +203,9 @@ extern "C" int sidon_tune_local_search(float* scratch, const float* reference, i
   for (int c = 0; c < SIDON_LS_TILES; ++c) any = any || ms_of[c] >= 0.0f;
   if (!any && sidon_lds_bytes(0, n, 1024) >= 0 &&
       sidon_lds_bytes(0, n, 1024) <= (long long)sidon_lds_limit()) {
-    // above the on-chip form the launch has one configuration (1024 threads,
-    // four steps in flight, the correlation output in global memory): nothing
-    // to choose, and the trials (no moments' buffer) cannot run it
+    // above the on-chip form without rows: the launch has one configuration
+    // (1024 threads, four steps in flight, the correlation output in the
+    // caller's rows), which trials without rows cannot run
     if (best_block) *best_block = 1024;
     if (best_depth) *best_depth = 4;
     if (best_per_cu) *best_per_cu = 1;
