@@ -991,16 +991,17 @@ int bench_kernels() {
     std::uniform_real_distribution<float> U(0.2f, 1.0f);
     for (auto& x : h) x = U(rng);
     float* pop = dalloc<float>(h.size());
+    float* mom = dalloc<float>(h.size());  // moments' buffer, as the search passes it
     double* F = dalloc<double>(P);
     CHECK_HIP(hipMemcpy(pop, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
     const int it = 100;
-    CHECK_SIDON(sidon_local_search(pop, nullptr, nullptr, nullptr, nullptr, n, P, 10, 8.0f, 8.0f,
+    CHECK_SIDON(sidon_local_search(pop, mom, nullptr, nullptr, nullptr, n, P, 10, 8.0f, 8.0f,
                                    0.01f, 0.01f, 0.9f, 0.999f, 0, s));
     CHECK_HIP(hipStreamSynchronize(s));
     unsigned long long steps = 0;
     CHECK_SIDON(sidon_ls_block_steps(&steps, 1));
     CHECK_HIP(hipEventRecord(e0, s));
-    CHECK_SIDON(sidon_local_search(pop, nullptr, nullptr, nullptr, nullptr, n, P, it, 8.0f,
+    CHECK_SIDON(sidon_local_search(pop, mom, nullptr, nullptr, nullptr, n, P, it, 8.0f,
                                    200.0f, 0.05f, 0.005f, 0.9f, 0.999f, 0, s));
     CHECK_HIP(hipEventRecord(e1, s));
     CHECK_SIDON(sidon_eval_f64(pop, n, P, F, nullptr, 1e-4, s));
@@ -1019,6 +1020,7 @@ int bench_kernels() {
                 (double)P * it / (ls / 1e3), ref_flop / (ls / 1e3) / 1e12,
                 8192.0 * (double)steps / (ls / 1e3) / 1e12, ev);
     CHECK_HIP(hipFree(pop));
+    CHECK_HIP(hipFree(mom));
     CHECK_HIP(hipFree(F));
   }
   CHECK_HIP(hipEventDestroy(e0));
@@ -1031,8 +1033,10 @@ int bench_kernels() {
 void usage() {
   std::printf(
       "sidon_search [mode] [options]     (no mode: the search, with the reference's defaults)\n"
-      "  --ladder 64,128,...   multiresolution ladder, each level twice the last\n"
-      "                        (default 64..2048)\n"
+      "  --ladder 64,128,...   multiresolution ladder (default 64..2048); a level twice\n"
+      "                        the last is carried by the cell split, any other ratio by\n"
+      "                        nearest-neighbour resampling (--fw-ladder, --search-graph:\n"
+      "                        doubling only)\n"
       "  --pop N               population size (default 256)\n"
       "  --islands N           number of islands (default 8)\n"
       "  --elite N             elites kept per island (default 2)\n"
