@@ -99,7 +99,9 @@ int sidon_local_search(float* pop, float* mom, float* vel, const int* skip, cons
                        int n, int P, int iters, float pStart, float pEnd, float lrStart,
                        float lrEnd, float beta1, float beta2, int modeDefault,
                        hipStream_t stream);
-/* fp64 form: kLocalSearch<double> (src/main.cu:837-839). */
+/* fp64 form: kLocalSearch<double> (src/main.cu:837-839).  Above n = 4544 the
+ * correlation output goes to the rows of mom (else vel), which must then be
+ * given; the longest length is 6480. */
 int sidon_local_search_f64(double* pop, double* mom, double* vel, const int* skip,
                            const int* mode, int n, int P, int iters, double pStart,
                            double pEnd, double lrStart, double lrEnd, double beta1,

@@ -848,6 +848,19 @@ int dispatch_ls(LsArgs<T>& A, int nmax, const LsTile& tile, void* ws, hipStream_
     if (tile.block == 1024 && A.gscr) return launch_ls_gg(A, nmax, tile.depth, stream);
   }
   if constexpr (sizeof(T) == 8) {  // fp64: fixed configurations, two steps in flight
+    int rc = SIDON_OK;
+    DeviceState* ds = device_state(&rc);
+    if (!ds) return rc;
+    // lengths whose fp64 G does not fit the LDS beside a and the weights: the
+    // correlation output in the caller's rows, 512 threads, plain layout
+    if (ls_lds_bytes<T, kSegStride>(nmax, tile.block) > ds->lds_max) {
+      if (!A.gscr) {
+        set_last_error_msg("local search (fp64): n above the on-chip form needs mom or vel "
+                           "(the launch's scratch for the correlation output)");
+        return SIDON_ERR_LDS;
+      }
+      return launch_ls_layout<T, 512, kGgQmax, false, kSeg, 2, true>(A, nmax, stream);
+    }
     switch (tile.block) {
       case 256: return launch_ls<T, 256, 16, false, 2>(A, nmax, stream);
       case 512: return launch_ls<T, 512, 12, false, 2>(A, nmax, stream);
@@ -991,6 +1004,10 @@ extern "C" int sidon_local_search_f64(double* pop, double* mom, double* vel, con
   if (P == 0 || iters == 0) return SIDON_OK;
   LsArgs<double> A{pop, n, nullptr, n, P, mom, vel, skip, mode, modeDefault, nullptr, nullptr,
                    iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2};
+  // above the on-chip form the moments' buffer is the correlation output's
+  // scratch until the final moments are written
+  A.gscr = mom ? mom : vel;
+  A.gstride = n;
   const LsTile tile{n >= 1024 ? 512 : 256, 2, 1};
   return dispatch_ls<double>(A, n, tile, nullptr, stream);
 }
@@ -1055,7 +1072,10 @@ extern "C" long long sidon_ls_lds_bytes_impl(int kernel, int n, int block) {
       const long long g20 = ls_lds_bytes<float, kSegStride, true>(n, block);
       return g20 <= cap ? g20 : ls_lds_bytes<float, kSeg, true>(n, block);
     }
-    case 1: return ls_lds_bytes<double>(n, block);
+    case 1: {  // G in LDS where it fits, else in the caller's rows (512 threads)
+      const long long onchip = ls_lds_bytes<double>(n, block);
+      return onchip <= 160LL * 1024 ? onchip : ls_lds_bytes<double, kSeg, true>(n, 512);
+    }
     default: return -1;
   }
 }

@@ -565,14 +565,21 @@ def test_split_more_long_candidates_than_workgroups(S, W):
     assert torch.equal(out[0], out[1])
 
 
+# fp64: G fits the LDS beside a and the weights to 4544 (512 threads); with
+# the moments' rows the plain layout reaches 6480
+LS64_ONCHIP_N = 4544
+LS64_MAX_N = 6480
+
+
 def test_lengths_past_the_lds_are_refused(S):
     """One height past the longest candidate a workgroup's LDS holds, the fp32
     and fp64 searches return SIDON_ERR_LDS before any launch (the reference's
     "does not fit", src/main.cu:173), and the lengths themselves still run.
-    Above the on-chip form the fp32 search needs the moments' buffer (or the
-    second moments') as its scratch and refuses to run without either."""
+    Above the on-chip form the searches need the moments' buffer (or the
+    second moments') as their scratch and refuse to run without either."""
     assert 0 < S.lds_bytes(0, LS_MAX_N, 1024) <= 160 * 1024 < S.lds_bytes(0, LS_MAX_N + 1, 1024)
-    for n, dtype in ((LS_MAX_N, torch.float32), (4544, torch.float64)):
+    assert 0 < S.lds_bytes(1, LS64_MAX_N, 512) <= 160 * 1024 < S.lds_bytes(1, LS64_MAX_N + 1, 512)
+    for n, dtype in ((LS_MAX_N, torch.float32), (LS64_MAX_N, torch.float64)):
         t = torch.full((1, n + 1), 1.0 / (n + 1), dtype=dtype, device="cuda")
         with pytest.raises(S.SidonError) as e:
             S.local_search(t, 2, 4.0, 8.0, 0.01, 0.005, mom=torch.zeros_like(t))
@@ -581,11 +588,39 @@ def test_lengths_past_the_lds_are_refused(S):
         S.local_search(ok, 2, 4.0, 8.0, 0.01, 0.005, vel=torch.zeros_like(ok))
         torch.cuda.synchronize()
         assert bool(torch.isfinite(ok).all()) and abs(float(ok.sum()) - 1) < 1e-5
-    big = torch.full((1, LS_ONCHIP_N + 1), 1.0 / (LS_ONCHIP_N + 1), device="cuda")
-    with pytest.raises(S.SidonError) as e:
-        S.local_search(big, 2, 4.0, 8.0, 0.01, 0.005)
-    assert e.value.status == S.SIDON_ERR_LDS
-    assert float(big[0, 0]) == float(np.float32(1.0 / (LS_ONCHIP_N + 1)))   # nothing ran
+    for n, dtype in ((LS_ONCHIP_N + 1, torch.float32), (LS64_ONCHIP_N + 1, torch.float64)):
+        big = torch.full((1, n), 1.0 / n, dtype=dtype, device="cuda")
+        before = big.clone()
+        with pytest.raises(S.SidonError) as e:
+            S.local_search(big, 2, 4.0, 8.0, 0.01, 0.005)
+        assert e.value.status == S.SIDON_ERR_LDS
+        assert torch.equal(big, before)   # nothing ran
+
+
+@pytest.mark.parametrize("n", [LS64_ONCHIP_N + 1, LS64_MAX_N])
+def test_fp64_past_the_on_chip_length(S, O, W, n):
+    """The fp64 search with its correlation output in the moments' rows: 10
+    steps against the fp64 oracle (same arithmetic, summation order aside),
+    per height; and the fp64 gradient at that length against the long-double
+    oracle."""
+    print(f"[ls] fp64 with the moments' rows, n={n}", flush=True)
+    a = W.structured_prior(2, n, seed=n).astype(np.float64)
+    t = dev(a, torch.float64)
+    S.local_search(t, 10, 4.0, 40.0, 0.05, 0.01, mom=torch.zeros_like(t))
+    got = t.cpu().numpy()
+    for p in range(2):
+        want, Fw, _, _, gmin = O.local_search_gmin(a[p], 10, 4.0, 40.0, 0.05, 0.01)
+        worst, keep = ls_errors(got[p], want, gmin)
+        print(f"   row {p}: max rel err {worst:.3e} over {keep.sum()}/{n} heights")
+        assert worst < 1e-10, worst
+        assert abs(O.F_hostorder(got[p]) - Fw) < 1e-12 * Fw
+    J, g = S.lp_grad(dev(a, torch.float64), 8.0)
+    J = J.cpu().numpy()
+    g = g.cpu().numpy()
+    for r in range(2):
+        Jr, gr = O.Jp_grad(a[r], 8.0)
+        assert abs(J[r] - Jr) < 1e-12 * Jr
+        assert float(np.abs(g[r] - gr).max() / np.abs(gr).max()) < 1e-9
 
 
 def test_correlation_in_global_memory_equals_lds_bitwise(S, W):
