@@ -283,7 +283,8 @@ int launch_fw_layout(const FwArgs& A, int P, hipStream_t stream) {
 }
 
 // The wide (bank-conflict-free) segment layout where both padded vectors fit
-// the LDS with it (n up to about 7900), else the narrow one.
+// the LDS with it (n up to about 7900), else the narrow one (to 8720), else
+// the plain one (to 9856).
 template <int BD, int GT>
 int launch_fw(const FwArgs& A, int P, hipStream_t stream) {
   int rc = SIDON_OK;
@@ -291,7 +292,11 @@ int launch_fw(const FwArgs& A, int P, hipStream_t stream) {
   if (!ds) return rc;
   if (fw_lds_bytes<BD, kSegStrideWide>(A.n) <= ds->lds_max)
     return launch_fw_layout<BD, GT, kSegStrideWide>(A, P, stream);
-  return launch_fw_layout<BD, GT, kSegStride>(A, P, stream);
+  if (fw_lds_bytes<BD, kSegStride>(A.n) <= ds->lds_max)
+    return launch_fw_layout<BD, GT, kSegStride>(A, P, stream);
+  // the plain layout: same time where both fit (n = 1050-8400, bit-identical
+  // results), and it holds the two padded vectors to n = 9900
+  return launch_fw_layout<BD, GT, kSeg>(A, P, stream);
 }
 
 }  // namespace
@@ -303,7 +308,9 @@ using namespace sidon;
 template <int BD>
 long long fw_lds_choice(int n) {
   const long long wide = fw_lds_bytes<BD, kSegStrideWide>(n);
-  return wide <= 160LL * 1024 ? wide : fw_lds_bytes<BD>(n);
+  if (wide <= 160LL * 1024) return wide;
+  const long long narrow = fw_lds_bytes<BD>(n);
+  return narrow <= 160LL * 1024 ? narrow : fw_lds_bytes<BD, kSeg>(n);
 }
 
 extern "C" long long sidon_fw_lds_bytes_impl(int n, int block) {

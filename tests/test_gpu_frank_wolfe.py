@@ -34,11 +34,13 @@ def test_frank_wolfe_vs_oracle(S, O, W, n):
             assert np.abs(got[p] - want).max() < 5e-3 * want.max()
 
 
-# the longest candidate the Frank-Wolfe workgroup's LDS holds (20-float segments)
-FW_MAX_N = 8720
+# the longest candidate the Frank-Wolfe workgroup's LDS holds with 20-float
+# segments, and with the plain 16-float layout
+FW_NARROW_MAX_N = 8720
+FW_MAX_N = 9856
 
 
-@pytest.mark.parametrize("n", [4200, 8400, FW_MAX_N])
+@pytest.mark.parametrize("n", [4200, 8400, FW_NARROW_MAX_N, FW_NARROW_MAX_N + 1, FW_MAX_N])
 def test_frank_wolfe_vs_oracle_large_n(S, O, W, n):
     """The ladder's levels 4200 and 8400: one Frank-Wolfe round of 50
     Chambolle-Pock iterations against the fp64 oracle, per height.  The GPU's
