@@ -87,12 +87,13 @@ int sidon_eval_exact(const float* pop, int n, int P, double* F, double* M, doubl
  * (pStart, lrStart) to (pEnd, lrEnd).  Adam moments restart at zero every call;
  * mom/vel (may be NULL) receive the final moments.  skip[p] != 0 leaves
  * candidate p untouched; mode (may be NULL) overrides modeDefault per candidate.
- * Lengths whose correlation output does not fit one workgroup's LDS beside a
- * and the weights (n > 9296 on a 160 KiB workgroup) keep it in global memory:
- * mom (or, when mom is NULL, vel) is then the launch's scratch for it until
- * the final moments are written, and one of them must be given
- * (SIDON_ERR_LDS otherwise; the reference always passes both).  The longest
- * fp32 length is 13216 (sidon_lds_bytes(0, n, 1024) <= sidon_lds_limit()).
+ * When mom (or, when mom is NULL, vel) is given, the launch keeps the
+ * correlation output in its rows until the final moments are written (the
+ * same bits, measured faster); lengths whose correlation output does not fit
+ * one workgroup's LDS beside a and the weights (n > 9296 on a 160 KiB
+ * workgroup) need one of them (SIDON_ERR_LDS otherwise; the reference always
+ * passes both).  Neither may overlap pop (SIDON_ERR_ARG).  The longest fp32
+ * length is 13216 (sidon_lds_bytes(0, n, 1024) <= sidon_lds_limit()).
  * Replaces runLocalSearch (src/main.cu:152-159) / kLocalSearch<float>
  * (src/kernels.cuh:177-289). */
 int sidon_local_search(float* pop, float* mom, float* vel, const int* skip, const int* mode,

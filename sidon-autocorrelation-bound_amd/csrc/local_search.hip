@@ -886,6 +886,15 @@ int dispatch_ls(LsArgs<T>& A, int nmax, const LsTile& tile, void* ws, hipStream_
   }
 }
 
+// The moments' buffers may serve as the correlation output's scratch, so
+// neither may overlap the population (all three are [P x n]).
+template <typename T>
+bool rows_overlap(const T* pop, const T* buf, int n, int P) {
+  if (!buf || P <= 0) return false;
+  const size_t len = (size_t)n * (size_t)P;
+  return buf < pop + len && pop < buf + len;
+}
+
 template <typename T>
 int check_schedule(int iters, T pS, T pE, T lS, T lE) {
   if (iters < 0) return SIDON_ERR_ARG;
@@ -984,6 +993,7 @@ extern "C" int sidon_local_search(float* pop, float* mom, float* vel, const int*
                                   float pEnd, float lrStart, float lrEnd, float beta1,
                                   float beta2, int modeDefault, hipStream_t stream) {
   if (!pop || n < 1 || P < 0) return SIDON_ERR_ARG;
+  if (rows_overlap(pop, mom, n, P) || rows_overlap(pop, vel, n, P)) return SIDON_ERR_ARG;
   if (int rc = check_schedule<float>(iters, pStart, pEnd, lrStart, lrEnd)) return rc;
   if (P == 0 || iters == 0) return SIDON_OK;
   LsArgs<float> A{pop, n, nullptr, n, P, mom, vel, skip, mode, modeDefault, nullptr, nullptr,
@@ -1000,6 +1010,7 @@ extern "C" int sidon_local_search_f64(double* pop, double* mom, double* vel, con
                                       double pEnd, double lrStart, double lrEnd, double beta1,
                                       double beta2, int modeDefault, hipStream_t stream) {
   if (!pop || n < 1 || P < 0) return SIDON_ERR_ARG;
+  if (rows_overlap(pop, mom, n, P) || rows_overlap(pop, vel, n, P)) return SIDON_ERR_ARG;
   if (int rc = check_schedule<double>(iters, pStart, pEnd, lrStart, lrEnd)) return rc;
   if (P == 0 || iters == 0) return SIDON_OK;
   LsArgs<double> A{pop, n, nullptr, n, P, mom, vel, skip, mode, modeDefault, nullptr, nullptr,

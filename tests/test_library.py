@@ -40,6 +40,14 @@ def test_argument_validation_without_device():
                                 0.9, 0.999, 0, None) == S.SIDON_ERR_ARG
     assert L.sidon_local_search(fake, None, None, None, None, 64, 4, 10, -4.0, 8.0, 0.1, 0.01,
                                 0.9, 0.999, 0, None) == S.SIDON_ERR_ARG
+    # the moments' buffers may not overlap the population (they can be the launch's scratch)
+    fake2 = ctypes.c_void_p(0x1000 + 4 * 64 * 4 - 4)
+    assert L.sidon_local_search(fake, fake, None, None, None, 64, 4, 10, 4.0, 8.0, 0.1, 0.01,
+                                0.9, 0.999, 0, None) == S.SIDON_ERR_ARG
+    assert L.sidon_local_search(fake, None, fake2, None, None, 64, 4, 10, 4.0, 8.0, 0.1, 0.01,
+                                0.9, 0.999, 0, None) == S.SIDON_ERR_ARG
+    assert L.sidon_local_search_f64(fake, fake, None, None, None, 64, 4, 10, 4.0, 8.0, 0.1, 0.01,
+                                    0.9, 0.999, 0, None) == S.SIDON_ERR_ARG
     # P == 0 / iters == 0 are no-ops
     assert L.sidon_local_search(fake, None, None, None, None, 64, 0, 10, 4.0, 8.0, 0.1, 0.01,
                                 0.9, 0.999, 0, None) == S.SIDON_OK
