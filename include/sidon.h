@@ -148,9 +148,10 @@ int sidon_polish_f64_noalloc(double* pop, double* mom, double* vel, int n, int K
  * (none when every workgroup has a candidate of the longest length to run);
  * the result is bit-identical to one workgroup's.  `workspace` (caller-owned, device,
  * sidon_local_search_ragged_workspace(P, stride) bytes) holds the queue
- * counter, the pairs' progress flags and exchange buffers, and for strides
- * above the on-chip form (9296) each candidate's correlation output (nmax
- * above it: no pairs, every candidate runs that form); the launch clears
+ * counter, the pairs' progress flags and exchange buffers, and a row per
+ * candidate for its correlation output (the launch keeps it there, as
+ * sidon_local_search does in the moments' rows; nmax up to 13216); the
+ * launch clears
  * what must start at zero with a kernel on `stream` (no memset node).
  * Launches that may overlap need distinct workspaces.  After the launch the
  * int at byte 4 of the workspace counts pair waits that gave up (a partner
