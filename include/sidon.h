@@ -89,11 +89,14 @@ int sidon_eval_exact(const float* pop, int n, int P, double* F, double* M, doubl
  * candidate p untouched; mode (may be NULL) overrides modeDefault per candidate.
  * When mom (or, when mom is NULL, vel) is given, the launch keeps the
  * correlation output in its rows until the final moments are written (the
- * same bits, measured faster); lengths whose correlation output does not fit
- * one workgroup's LDS beside a and the weights (n > 9296 on a 160 KiB
- * workgroup) need one of them (SIDON_ERR_LDS otherwise; the reference always
- * passes both).  Neither may overlap pop (SIDON_ERR_ARG).  The longest fp32
- * length is 13216 (sidon_lds_bytes(0, n, 1024) <= sidon_lds_limit()).
+ * same bits, measured faster); with both given (and apart) it uses the rows
+ * of both, which moves about an eighth of the memory-side traffic of one
+ * (the halves of a split tile are stored, not added atomically).  Lengths
+ * whose correlation output does not fit one workgroup's LDS beside a and the
+ * weights (n > 9296 on a 160 KiB workgroup) need one of them (SIDON_ERR_LDS
+ * otherwise; the reference always passes both).  Neither may overlap pop
+ * (SIDON_ERR_ARG).  The longest fp32 length is 13216
+ * (sidon_lds_bytes(0, n, 1024) <= sidon_lds_limit()).
  * Replaces runLocalSearch (src/main.cu:152-159) / kLocalSearch<float>
  * (src/kernels.cuh:177-289). */
 int sidon_local_search(float* pop, float* mom, float* vel, const int* skip, const int* mode,
@@ -101,8 +104,8 @@ int sidon_local_search(float* pop, float* mom, float* vel, const int* skip, cons
                        float lrEnd, float beta1, float beta2, int modeDefault,
                        hipStream_t stream);
 /* fp64 form: kLocalSearch<double> (src/main.cu:837-839).  Above n = 4544 the
- * correlation output goes to the rows of mom (else vel), which must then be
- * given; the longest length is 6480. */
+ * correlation output goes to the rows of mom (else vel; of both when both
+ * are given), which must then be given; the longest length is 6480. */
 int sidon_local_search_f64(double* pop, double* mom, double* vel, const int* skip,
                            const int* mode, int n, int P, int iters, double pStart,
                            double pEnd, double lrStart, double lrEnd, double beta1,
@@ -148,7 +151,7 @@ int sidon_polish_f64_noalloc(double* pop, double* mom, double* vel, int n, int K
  * (none when every workgroup has a candidate of the longest length to run);
  * the result is bit-identical to one workgroup's.  `workspace` (caller-owned, device,
  * sidon_local_search_ragged_workspace(P, stride) bytes) holds the queue
- * counter, the pairs' progress flags and exchange buffers, and a row per
+ * counter, the pairs' progress flags and exchange buffers, and two rows per
  * candidate for its correlation output (the launch keeps it there, as
  * sidon_local_search does in the moments' rows; nmax up to 13216); the
  * launch clears
@@ -361,9 +364,9 @@ int sidon_ls_tile_config(int i, int* block, int* depth, int* per_cu);
  * when non-NULL), `repeats` timed runs of `iters` steps after one warm-up,
  * and keeps the fastest as the rule for n (sidon_get_ls_tile); the built-in
  * configuration stays unless another beats it by more than 1%.  `rows`
- * (P x n floats, may be NULL) is passed to the trials as their moments'
- * buffer, so they time the form that launches with a moments' buffer run
- * (correlation output in its rows); NULL times the form without one, and
+ * (2 x P x n floats, may be NULL) is passed to the trials as their moments'
+ * buffers (mom, then vel), so they time the form that launches with both
+ * run (the correlation output in their rows); NULL times the form without, and
  * above n = 9296, which has no other form, reports the one configuration
  * untimed.  Reports the choice and its time, and, when `trial_ms` is
  * non-NULL, the time of every configuration (SIDON_LS_TILES floats, -1

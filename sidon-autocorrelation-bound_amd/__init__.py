@@ -274,7 +274,8 @@ def local_search(pop: torch.Tensor, iters: int, p_start: float, p_end: float, lr
     """In-place fused local search on every row of pop (float32 or float64).
     float32 rows longer than the on-chip form (lds_bytes(0, n, 1024) grows past
     the G-in-LDS layout at n = 9296) keep the correlation output in global
-    memory and need `mom` or `vel`: the launch uses it as that scratch."""
+    memory and need `mom` or `vel`: the launch uses it as that scratch (both,
+    when both are given: the two-row form, with no atomic adds in memory)."""
     _need_gpu()
     P, n = pop.shape
     if pop.dtype == torch.float32:
@@ -583,15 +584,15 @@ def tune_local_search(n: int, P: int = 64, iters: int = 20, repeats: int = 2, se
     """Autotune the launch configuration of the fused local search at length n
     (workgroup size, operand steps in flight, candidates a CU) on a
     structured-prior sample; the choice is kept by the library for n.  With
-    `with_rows` the trials run with a moments' buffer (the correlation output
-    in its rows), as launches that pass one do.  Returns ((block, depth,
+    `with_rows` the trials run with both moments' buffers (the correlation
+    output in their rows), as launches that pass them do.  Returns ((block, depth,
     per_cu), ms of the choice, {configuration: ms} of every trial that fits
     n)."""
     _need_gpu()
     from .workloads import structured_prior
     ref = torch.as_tensor(structured_prior(P, n, seed=seed), device="cuda")
     scratch = torch.empty_like(ref)
-    rows = torch.empty_like(ref) if with_rows else None
+    rows = torch.empty((2,) + tuple(ref.shape), dtype=ref.dtype, device=ref.device) if with_rows else None
     blk, dep, per, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
     trials = (C.c_float * LS_TILES)()
     _check(lib().sidon_tune_local_search(_ptr(scratch), _ptr(ref), _ptr(rows), n, P, iters, repeats,

@@ -470,6 +470,22 @@ __device__ __forceinline__ void put4_global(T* out, int col, const A4& acc, bool
     }
   }
 }
+// The two-row form: every item stores its sums (outputs i < n), the first
+// item of a tile into `out`, the second into the second row, and the reader
+// adds the two rows.  Float atomic adds in global memory are performed past
+// the L2 (each one a 32-byte write to the memory side), stores stay in it;
+// fl(x + y) is the value 0 + x + y of two atomic adds in either order.
+template <typename T, typename A4>
+__device__ __forceinline__ void put4_rows(T* out, int col, const A4& acc, int n) {
+  const int kk = (opaque((int)threadIdx.x) & (kWave - 1)) >> 4;
+  constexpr int RS = Acc4<T>::kRowStride;
+  const int i0 = 16 * col + Acc4<T>::kRowMul * kk;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + RS * r;
+    if (i < n) wg_store(out + i, acc[r]);
+  }
+}
 // This wave's global stores and atomics have completed (before a barrier
 // that publishes them to the other waves of the workgroup).
 __device__ __forceinline__ void wait_global_writes() {
@@ -828,11 +844,14 @@ __device__ __forceinline__ void corr_items(int n, unsigned* items, int* scratch,
 // >= 2n-1 up to cbuf_len(n); a padded.  `out` must be exactly zero before
 // the call.  With `segmask` (the weight segments' non-zero bits) steps that
 // read only zero weights are skipped.
-// With GOUT, `out` is the candidate's row in global memory (put4_global).
+// With GOUT, `out` is the candidate's row in global memory (put4_global);
+// with `out2` as well, the two-row form (put4_rows): neither row needs to be
+// zero, and G_i is out[i] + out2[i] once out2 is zero where no tile's second
+// item writes.
 template <typename T, int D = Pipe<T>::kDepth, int SS = kSegStride, bool GOUT = false>
 __device__ __forceinline__ void corr_mfma(const T* __restrict__ c, const T* __restrict__ sA, T* out, int n,
                           int* ctr, const unsigned* items, const unsigned* segmask,
-                          unsigned* steps = nullptr, int half = -1) {
+                          unsigned* steps = nullptr, int half = -1, T* out2 = nullptr) {
   const int N16 = nseg16(n);
   const int NT = corr_tiles(n);
   const int cost = corr_cost(N16);
@@ -851,8 +870,12 @@ __device__ __forceinline__ void corr_mfma(const T* __restrict__ c, const T* __re
         acc = corr_tile<T, D, SS>(c, sA, I0, I0 + u0, I0 + u1);
         done += u1 - u0;
       }
-      if constexpr (GOUT) put4_global(out, I0 + t, acc, u0 == 0 && u1 == cost, n);
-      else put4(out, I0 + t, acc, u0 == 0 && u1 == cost);
+      if constexpr (GOUT) {
+        if (out2) put4_rows(u0 == 0 ? out : out2, I0 + t, acc, n);
+        else put4_global(out, I0 + t, acc, u0 == 0 && u1 == cost, n);
+      } else {
+        put4(out, I0 + t, acc, u0 == 0 && u1 == cost);
+      }
     }
     it = ticket_value(take_ticket(ctr));
   }

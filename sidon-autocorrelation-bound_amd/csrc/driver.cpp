@@ -385,7 +385,7 @@ int uneven(const Args& A) {
   if (A.tune) {  // autotune the launch configuration at the longest length (it sets the launch)
     float* scratch = dalloc<float>((size_t)64 * nmax);
     float* ref = dalloc<float>((size_t)64 * nmax);
-    float* rows = dalloc<float>((size_t)64 * nmax);  // the launches run with rows (workspace)
+    float* rows = dalloc<float>((size_t)2 * 64 * nmax);  // the launches run with rows (workspace)
     std::vector<float> r((size_t)64 * nmax);
     for (int p = 0; p < 64; ++p) structured_prior(rng, nmax, &r[(size_t)p * nmax]);
     CHECK_HIP(hipMemcpy(ref, r.data(), r.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -673,10 +673,11 @@ int search(const Args& A) {
   float* dst = dalloc<float>(P);
   int* skip = dalloc<int>(P);
   int* mode = dalloc<int>(P);
-  // Adam moments out of each launch, which above the on-chip form (long
-  // levels) also serve as the launch's scratch for the correlation output
-  // (the reference allocates g.mom for every level, main.cu:107-144)
+  // Adam moments out of each launch, which also serve as the launch's scratch
+  // for the correlation output (the reference allocates g.mom and g.vel for
+  // every level, main.cu:107-144)
   float* mom = dalloc<float>((size_t)P * nmax);
+  float* vel = dalloc<float>((size_t)P * nmax);
   std::vector<int> hmode(P);
   for (int p = 0; p < P; ++p) hmode[p] = (p / PI) % 4 == 3 ? 1 : 0;  // mirror descent island
   CHECK_HIP(hipMemcpy(mode, hmode.data(), P * sizeof(int), hipMemcpyHostToDevice));
@@ -734,7 +735,7 @@ int search(const Args& A) {
       auto tier_run = [&](int t, int iters, float pa, float pb, float la, float lb) {
         for (int p = 0; p < P; ++p) hskip[p] = tier[p] != t;
         CHECK_HIP(hipMemcpy(skip, hskip.data(), P * sizeof(int), hipMemcpyHostToDevice));
-        CHECK_SIDON(sidon_local_search(pop, mom, nullptr, skip, mode, n, P, iters, pa, pb,
+        CHECK_SIDON(sidon_local_search(pop, mom, vel, skip, mode, n, P, iters, pa, pb,
                                        la, lb, 0.9f, 0.999f, 0, s));
         for (int p = 0; p < P; ++p) lsSteps += (long long)(hskip[p] == 0) * iters;
         flop += 4.0 * n2 * (double)iters * (double)(t == 0 ? nCold : t == 1 ? nMed : P - nCold - nMed);
@@ -863,7 +864,7 @@ int search(const Args& A) {
     }
   }
   sidon_policy_destroy(pol);
-  CHECK_HIP(hipFree(pop)); CHECK_HIP(hipFree(nxt)); CHECK_HIP(hipFree(mom));
+  CHECK_HIP(hipFree(pop)); CHECK_HIP(hipFree(nxt)); CHECK_HIP(hipFree(mom)); CHECK_HIP(hipFree(vel));
   if (A.polish_iters > 0 && !bestA.empty()) polish(A, bestA, bestF);
   const double hf = host_F(bestA);
   save_vec(A.out + ".txt", bestA, std::max(bestF, hf));
@@ -991,17 +992,18 @@ int bench_kernels() {
     std::uniform_real_distribution<float> U(0.2f, 1.0f);
     for (auto& x : h) x = U(rng);
     float* pop = dalloc<float>(h.size());
-    float* mom = dalloc<float>(h.size());  // moments' buffer, as the search passes it
+    float* mom = dalloc<float>(h.size());  // moments' buffers, as the search passes them
+    float* vel = dalloc<float>(h.size());
     double* F = dalloc<double>(P);
     CHECK_HIP(hipMemcpy(pop, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
     const int it = 100;
-    CHECK_SIDON(sidon_local_search(pop, mom, nullptr, nullptr, nullptr, n, P, 10, 8.0f, 8.0f,
+    CHECK_SIDON(sidon_local_search(pop, mom, vel, nullptr, nullptr, n, P, 10, 8.0f, 8.0f,
                                    0.01f, 0.01f, 0.9f, 0.999f, 0, s));
     CHECK_HIP(hipStreamSynchronize(s));
     unsigned long long steps = 0;
     CHECK_SIDON(sidon_ls_block_steps(&steps, 1));
     CHECK_HIP(hipEventRecord(e0, s));
-    CHECK_SIDON(sidon_local_search(pop, mom, nullptr, nullptr, nullptr, n, P, it, 8.0f,
+    CHECK_SIDON(sidon_local_search(pop, mom, vel, nullptr, nullptr, n, P, it, 8.0f,
                                    200.0f, 0.05f, 0.005f, 0.9f, 0.999f, 0, s));
     CHECK_HIP(hipEventRecord(e1, s));
     CHECK_SIDON(sidon_eval_f64(pop, n, P, F, nullptr, 1e-4, s));
@@ -1021,6 +1023,7 @@ int bench_kernels() {
                 8192.0 * (double)steps / (ls / 1e3) / 1e12, ev);
     CHECK_HIP(hipFree(pop));
     CHECK_HIP(hipFree(mom));
+    CHECK_HIP(hipFree(vel));
     CHECK_HIP(hipFree(F));
   }
   CHECK_HIP(hipEventDestroy(e0));

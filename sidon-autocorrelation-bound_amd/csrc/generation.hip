@@ -183,7 +183,8 @@ extern "C" int sidon_tune_local_search(float* scratch, const float* reference, f
         break;
       }
       (void)hipEventRecord(e0, stream);
-      const int lrc = sidon_local_search(scratch, rows, nullptr, nullptr, nullptr, n, P, iters,
+      const int lrc = sidon_local_search(scratch, rows, rows ? rows + (size_t)P * (size_t)n : nullptr,
+                                         nullptr, nullptr, n, P, iters,
                                          4.0f, 1e4f, 0.085f, 0.0015f, 0.9f, 0.999f, 0, stream);
       (void)hipEventRecord(e1, stream);
       if (lrc == SIDON_ERR_LDS || lrc == SIDON_ERR_UNSUPPORTED) { ok = false; break; }

@@ -117,6 +117,9 @@ struct LsArgs {
   // weights: each candidate's G in global memory, gstride elements a row
   T* gscr;
   long long gstride;
+  // a second set of rows (same stride): the two-row form, G_i = row + row2
+  // (nullptr: one row, the split tiles' halves added by atomic adds)
+  T* gscr2;
 };
 
 // ------------------------------------------------------- paired halves --
@@ -205,6 +208,7 @@ __device__ __forceinline__ void ls_candidate(const LsArgs<T>& A, int cand, int h
   T* sG = sC + cbuf_len(n);
   T* slots = sG + (GG ? 0 : gbuf_len(n));  // 2 * NW
   T* gG = GG ? A.gscr + (size_t)cand * (size_t)A.gstride : nullptr;
+  T* gG2 = GG && A.gscr2 ? A.gscr2 + (size_t)cand * (size_t)A.gstride : nullptr;
   T* bmax = slots + 2 * NW;     // max_k b_k of the step (raised by the autoconvolution)
   int* ctr = reinterpret_cast<int*>(bmax + 1);  // [0] conv items, [1] corr items
   unsigned* nsteps = reinterpret_cast<unsigned*>(ctr + 2);  // matrix block steps run
@@ -229,7 +233,10 @@ __device__ __forceinline__ void ls_candidate(const LsArgs<T>& A, int cand, int h
   for (int i = tid; i < LC; i += BD) sC[i] = T(0);
   if constexpr (GG) {
     for (int i = tid; i < n; i += BD)
-      if (half < 0 || corr_tile_owner(i >> 8) == half) wg_store(gG + i, T(0));
+      if (half < 0 || corr_tile_owner(i >> 8) == half) {
+        wg_store(gG + i, T(0));
+        if (gG2) wg_store(gG2 + i, T(0));  // stays zero where no second item writes
+      }
     wait_global_writes();  // zero before any wave's correlation adds onto it
   } else {
     for (int i = tid; i < LG; i += BD) sG[i] = T(0);
@@ -319,7 +326,7 @@ __device__ __forceinline__ void ls_candidate(const LsArgs<T>& A, int cand, int h
     const T bc1 = T(1) / (T(1) - b1t), bc2 = T(1) / (T(1) - b2t);
 
     if constexpr (GG) {
-      corr_mfma<T, D, SS, true>(sC, sA, gG, n, ctr + 1, gitems, segmask, nsteps, half);
+      corr_mfma<T, D, SS, true>(sC, sA, gG, n, ctr + 1, gitems, segmask, nsteps, half, gG2);
       wait_global_writes();  // G complete in memory before the barrier
     } else {
       corr_mfma<T, D, SS>(sC, sA, sG, n, ctr + 1, gitems, segmask, nsteps, half);
@@ -343,8 +350,12 @@ __device__ __forceinline__ void ls_candidate(const LsArgs<T>& A, int cand, int h
         const T ai = sA[pad_addr<SS>(i)];
         T Gi;
         if constexpr (GG) {
-          Gi = wg_load(gG + i);
-          wg_store(gG + i, T(0));  // zero again for the next step's correlation
+          if (gG2) {  // the two-row form: each step's items overwrite the rows
+            Gi = wg_load(gG + i) + wg_load(gG2 + i);
+          } else {
+            Gi = wg_load(gG + i);
+            wg_store(gG + i, T(0));  // zero again for the next step's correlation
+          }
         } else {
           Gi = sG[i];
         }
@@ -795,7 +806,8 @@ int launch_ls_depth(const LsArgs<float>& A, int nmax, int depth, hipStream_t str
 // timeout count, then 16 progress ints per candidate, then per candidate the
 // b_k exchange of a split pair (256 per autoconvolution tile, and the two
 // halves' maxima).
-// Then each candidate's correlation output row (stride floats).
+// Then two sets of correlation output rows (stride floats per candidate each:
+// the two-row form of the correlation output).
 constexpr size_t kWsHeader = 256;
 inline long long xchg_stride(int stride) { return 256LL * conv_tiles(stride) + 64; }
 inline bool ragged_ws_has_g(int) { return true; }
@@ -804,7 +816,7 @@ inline size_t ragged_ws_g_offset(int P, int stride) {
 }
 inline size_t ragged_ws_bytes(int P, int stride) {
   return ragged_ws_g_offset(P, stride) +
-         (ragged_ws_has_g(stride) ? (size_t)P * (size_t)stride * sizeof(float) : 0);
+         (ragged_ws_has_g(stride) ? 2 * (size_t)P * (size_t)stride * sizeof(float) : 0);
 }
 
 // Clears the queue counter, the timeout count and the candidates' progress
@@ -835,6 +847,7 @@ int dispatch_ls(LsArgs<T>& A, int nmax, const LsTile& tile, void* ws, hipStream_
       A.gscr = reinterpret_cast<T*>(static_cast<char*>(ws) +
                                     ragged_ws_g_offset(A.P, (int)A.stride));
       A.gstride = A.stride;
+      A.gscr2 = A.gscr + (size_t)A.P * (size_t)A.stride;
     }
   }
   if constexpr (sizeof(T) == 4) {
@@ -998,10 +1011,11 @@ extern "C" int sidon_local_search(float* pop, float* mom, float* vel, const int*
   if (P == 0 || iters == 0) return SIDON_OK;
   LsArgs<float> A{pop, n, nullptr, n, P, mom, vel, skip, mode, modeDefault, nullptr, nullptr,
                   iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2};
-  // above the on-chip form the moments' buffer is the correlation output's
-  // scratch until the final moments are written
+  // above the on-chip form the moments' buffers are the correlation output's
+  // scratch until the final moments are written (both, when both are given)
   A.gscr = mom ? mom : vel;
   A.gstride = n;
+  A.gscr2 = mom && vel && !rows_overlap(mom, vel, n, P) ? vel : nullptr;
   return dispatch_ls<float>(A, n, pick_tile(n), nullptr, stream);
 }
 
@@ -1015,10 +1029,11 @@ extern "C" int sidon_local_search_f64(double* pop, double* mom, double* vel, con
   if (P == 0 || iters == 0) return SIDON_OK;
   LsArgs<double> A{pop, n, nullptr, n, P, mom, vel, skip, mode, modeDefault, nullptr, nullptr,
                    iters, pStart, pEnd, lrStart, lrEnd, beta1, beta2};
-  // above the on-chip form the moments' buffer is the correlation output's
-  // scratch until the final moments are written
+  // above the on-chip form the moments' buffers are the correlation output's
+  // scratch until the final moments are written (both, when both are given)
   A.gscr = mom ? mom : vel;
   A.gstride = n;
+  A.gscr2 = mom && vel && !rows_overlap(mom, vel, n, P) ? vel : nullptr;
   const LsTile tile{n >= 1024 ? 512 : 256, 2, 1};
   return dispatch_ls<double>(A, n, tile, nullptr, stream);
 }

@@ -623,6 +623,34 @@ def test_fp64_past_the_on_chip_length(S, O, W, n):
         assert float(np.abs(g[r] - gr).max() / np.abs(gr).max()) < 1e-9
 
 
+@pytest.mark.parametrize("dtype,n", [(torch.float32, 10000), (torch.float32, 13216),
+                                     (torch.float64, 4545), (torch.float64, 6480)])
+def test_two_rows_equal_one_row_bitwise(S, W, dtype, n):
+    """Past the on-chip form the correlation output must live in global
+    memory: in the rows of mom and vel when both are given (each tile item
+    stores its sums, the update adds the two rows), else in the one buffer's
+    rows (the halves of a split tile added atomically onto zero).  fl(x + y)
+    either way: heights and final moments are the same bits, and a vel that
+    overlaps mom falls back to the one-row form."""
+    print(f"[ls] two rows vs one row, {dtype} n={n}", flush=True)
+    P = 4
+    start = torch.as_tensor(W.structured_prior(P, n, seed=n + 11), dtype=dtype, device="cuda")
+    kw = dict(iters=10, p_start=4.0, p_end=200.0, lr_start=0.05, lr_end=0.01)
+    a, mom_a, vel_a = start.clone(), torch.zeros_like(start), torch.zeros_like(start)
+    S.local_search(a, mom=mom_a, vel=vel_a, **kw)
+    b, mom_b = start.clone(), torch.zeros_like(start)
+    S.local_search(b, mom=mom_b, **kw)
+    c, vel_c = start.clone(), torch.zeros_like(start)
+    S.local_search(c, vel=vel_c, **kw)
+    d, both = start.clone(), torch.zeros_like(start)
+    S.local_search(d, mom=both, vel=both, **kw)   # overlapping: one row
+    torch.cuda.synchronize()
+    assert torch.equal(a, b) and torch.equal(a, c) and torch.equal(a, d)
+    assert torch.equal(mom_a, mom_b) and torch.equal(vel_a, vel_c)
+    assert torch.equal(both, vel_a)                # vel is written after mom
+    assert float(a.sum(dim=1).sub(1).abs().max()) < 1e-5
+
+
 def test_correlation_in_global_memory_equals_lds_bitwise(S, W):
     """A ragged launch whose stride is above the on-chip form runs every
     candidate with the correlation output in global memory; at 600 and 4096
@@ -657,8 +685,10 @@ def test_correlation_in_global_memory_equals_lds_bitwise(S, W):
 @pytest.mark.parametrize("n", [4097, 6000, 8400])
 def test_moments_buffer_form_equals_lds_form_bitwise(S, W, n):
     """With a moments' buffer the 1024-thread launch keeps its correlation
-    output in the buffer's rows (and the plain segment layout); without one,
-    in LDS.  The heights and the final moments are the same bits."""
+    output in the buffer's rows (and the plain segment layout; with both
+    buffers, in the rows of both: the second item of a split tile stores into
+    vel's row); without one, in LDS.  The heights and the final moments are
+    the same bits."""
     print(f"[ls] G in the moments' rows vs in LDS, n={n}", flush=True)
     P = 6
     start = dev(W.structured_prior(P, n, seed=n + 3))
