@@ -321,6 +321,44 @@ def fw_legs(S, W, torch, seed: int, rounds: int = 1, cp_iters: int = 200, P: int
     return out
 
 
+def grad_leg(S, W, torch, seed: int, n: int = 8400):
+    """The backward kernel of J_p (sidon_lp_grad: b recomputed, the correlation
+    adjoint, one workgroup a row) on B structured-prior rows at n, p = 8, 64
+    and 1024: median of 5 calls after a warm-up, HIP events on the stream.
+    Executed matrix work from the kernel's block-step count (8192 FLOP each;
+    correlation steps over zero weights, most of them at p = 1024, are
+    skipped), and the dense 3 n^2 FLOP a row beside it; B = 64 (the adjoint
+    check's batch) occupies 64 of the 256 CUs, B = 256 one row a CU."""
+    import numpy as np
+    out = []
+    stream = torch.cuda.current_stream()
+    for B in (64, 256):
+        a = torch.as_tensor(W.structured_prior(B, n, seed=seed + B), device="cuda")
+        J = torch.empty(B, dtype=torch.float32, device="cuda")
+        g = torch.empty_like(a)
+        for p in (8.0, 64.0, 1024.0):
+            call = lambda: S._check(S.lib().sidon_lp_grad(a.data_ptr(), n, B, p, J.data_ptr(),
+                                                          g.data_ptr(), stream.cuda_stream),
+                                    "sidon_lp_grad")
+            call()
+            torch.cuda.synchronize()
+            S.ls_block_steps(reset=True)
+            ts = []
+            for _ in range(5):
+                ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                ev0.record(stream)
+                call()
+                ev1.record(stream)
+                torch.cuda.synchronize()
+                ts.append(ev0.elapsed_time(ev1) / 1e3)
+            s = float(np.median(ts))
+            executed = S.ls_block_steps(reset=True) / 5 * 8192.0 / s / 1e12
+            out.append({"n": n, "B": B, "p": p, "ms": s * 1e3, "gradients_per_s": B / s,
+                        "executed_tflops": executed, "frac_fp32_peak": executed / PEAK_FP32,
+                        "dense_equivalent_tflops": 3.0 * n * n * B / s / 1e12})
+    return out
+
+
 def polish_leg(S, W, torch, seed: int, n: int = 8400, K: int = 8, iters: int = 100):
     """The whole-device fp64 polish (sidon_polish_f64) of K candidates at n over
     `iters` steps of the anneal, HIP events around the launch sequence.  Dense
@@ -568,6 +606,7 @@ def main() -> int:
         out["f_check"] = fcheck
     out["ls_by_n"] = ls_by_n(S, W, torch, args.seed)
     out["f_evals"] = f_evals(S, W, torch, not args.no_cpu, args.seed)
+    out["lp_grad"] = grad_leg(S, W, torch, args.seed)
     out["host"] = host
     print(json.dumps(out))
     return 0

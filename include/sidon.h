@@ -327,8 +327,8 @@ int sidon_measure_fp32_ceilings(float* scratch, int scratch_len, double* valu_tf
                                 double* mfma_tflops, hipStream_t stream);
 
 /* Matrix block steps (four v_mfma_f32_16x16x4f32 per wave: 8192 FLOP per
- * lane-group of 64, i.e. 4 * 2048 FLOP) that the local-search kernels have
- * executed on the current device since the last call with reset != 0; the
+ * lane-group of 64, i.e. 4 * 2048 FLOP) that the local-search kernels and
+ * sidon_lp_grad[_f64] have executed on the current device since the last call with reset != 0; the
  * count excludes correlation steps skipped because their weights are all
  * zero.  The counter is device-wide: launches on every stream add to it, and
  * the read (with reset, an atomic exchange with zero on the device) loses no

@@ -562,8 +562,9 @@ def measure_fp32_ceilings(stream=None):
 
 
 def ls_block_steps(reset: bool = False) -> int:
-    """Matrix block steps (4 x 2048 FLOP each) the local-search kernels have
-    executed on the current device since the last reset (synchronous)."""
+    """Matrix block steps (4 x 2048 FLOP each) the local-search and gradient
+    kernels have executed on the current device since the last reset
+    (synchronous)."""
     _need_gpu()
     out = C.c_ulonglong()
     _check(lib().sidon_ls_block_steps(C.byref(out), int(reset)), "sidon_ls_block_steps")

@@ -18,10 +18,10 @@
 
 namespace sidon {
 
-// Matrix block steps (4 MFMAs of 16x16x4 each) run by the local-search
-// kernels since the last sidon_ls_block_steps(reset) call: the work the
-// kernels actually executed, for roofline accounting (correlation steps that
-// would read only zero weights are skipped and not counted).
+// Matrix block steps (4 MFMAs of 16x16x4 each) run by the local-search and
+// gradient kernels since the last sidon_ls_block_steps(reset) call: the work
+// the kernels actually executed, for roofline accounting (correlation steps
+// that would read only zero weights are skipped and not counted).
 __device__ unsigned long long g_block_steps = 0ull;
 __device__ unsigned long long g_block_steps_taken = 0ull;
 
@@ -617,12 +617,13 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   T* slots = sG + (GG ? 0 : gbuf_len(n));
   T* bmax = slots + 2 * NW;
   int* ctr = reinterpret_cast<int*>(bmax + 1);
+  unsigned* nsteps = reinterpret_cast<unsigned*>(ctr + 2);  // matrix block steps run
   unsigned* citems = reinterpret_cast<unsigned*>(ctr + 3);
   unsigned* gitems = citems + 2 * conv_tiles(n);
   unsigned* segmask = gitems + 2 * corr_tiles(n);
   const int tid = threadIdx.x;
   const int LA = pad_len<SS>(n), LC = cbuf_len(n), LG = gbuf_len(n);
-  if (tid == 0) { ctr[0] = 0; ctr[1] = 0; *bmax = T(0); }
+  if (tid == 0) { ctr[0] = 0; ctr[1] = 0; *nsteps = 0u; *bmax = T(0); }
   for (int w = tid; w < segmask_words(n); w += BD) segmask[w] = 0u;
   autoconv_items(n, citems, reinterpret_cast<int*>(sC), tid, BD);
   corr_items(n, gitems, reinterpret_cast<int*>(sC), tid, BD);
@@ -641,7 +642,7 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   for (int i = tid; i < n; i += BD) { const T x = a[i]; pad_put<SS>(sA, i, x); sl += x; }
   const T S = block_sum<T, NW>(sl, slots);  // barrier also publishes sA
   const int nb = 2 * n - 1;
-  autoconv_mfma<T, D, SS>(sA, sC, n, ctr, citems, nullptr, bmax);  // sC[k] = b_k, *bmax = M
+  autoconv_mfma<T, D, SS>(sA, sC, n, ctr, citems, nsteps, bmax);  // sC[k] = b_k, *bmax = M
   __syncthreads();
   const T M = *bmax;
   const T pm1 = A.p - T(1);
@@ -656,14 +657,18 @@ __global__ __launch_bounds__(BD) void k_lp_grad(GradArgs<T> A) {
   const T R = block_sum<T, NW>(rloc, slots + NW);
   const T Rp = dexp(log(R) / A.p);
   if (A.J && tid == 0) A.J[cand] = T(2) * T(n) * M * Rp / (S * S);
-  if (!g) return;
+  if (!g) {  // the autoconvolution's steps are counted (the barrier of R is past them)
+    if (tid == 0) atomicAdd(&g_block_steps, (unsigned long long)*nsteps);
+    return;
+  }
   if constexpr (GG) {
-    corr_mfma<T, D, SS, true>(sC, sA, g, n, ctr + 1, gitems, segmask);
+    corr_mfma<T, D, SS, true>(sC, sA, g, n, ctr + 1, gitems, segmask, nsteps);
     wait_global_writes();
   } else {
-    corr_mfma<T, D, SS>(sC, sA, sG, n, ctr + 1, gitems, segmask);
+    corr_mfma<T, D, SS>(sC, sA, sG, n, ctr + 1, gitems, segmask, nsteps);
   }
   __syncthreads();
+  if (tid == 0) atomicAdd(&g_block_steps, (unsigned long long)*nsteps);
   const T scale = T(4) * T(n) * Rp / (S * S);
   const T MS = M / S;
   for (int i = tid; i < n; i += BD) g[i] = scale * ((GG ? wg_load(g + i) : sG[i]) / R - MS);

@@ -96,6 +96,40 @@ def test_lp_grad_fp32_ragged_lengths(S, O, n):
         assert e < 6e-6, (r, e)            # measured up to 1.8e-6 (n = 4095)
 
 
+def _dense_steps(n: int):
+    """Matrix block steps of one pass when every weight c_k, k < 2n-1, is
+    non-zero: (the symmetric autoconvolution's tiles, block offsets
+    max(0, 16j-N16+1) <= d <= min(16j+15, N16, 8j+7); the correlation steps
+    e in [16j, 16j+N16+15) whose weight window c[16e .. 16e+30] reaches below
+    2n-1, i.e. e <= (2n-2) // 16)."""
+    N16 = (n + 15) // 16
+    NTc = (((2 * n - 1) + 15) // 16 + 15) // 16
+    conv = sum(max(0, min(16 * j + 15, N16, 8 * j + 7) - max(0, 16 * j - N16 + 1) + 1)
+               for j in range(NTc))
+    last = (2 * n - 2) // 16
+    corr = sum(max(0, min(16 * j + N16 + 14, last) - 16 * j + 1) for j in range((N16 + 15) // 16))
+    return conv, corr
+
+
+@pytest.mark.parametrize("n", [600, 8400, 13216])
+def test_grad_block_step_count_dense(S, n):
+    """Strictly positive heights at p = 4: every weight is non-zero, so the
+    gradient kernel runs the dense schedule of one autoconvolution and one
+    correlation a row (13216: the correlation in the gradient rows), and J
+    alone the autoconvolution; the counts feed the bench's executed work."""
+    print(f"[grad] block-step count n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    a = dev(rng.uniform(0.5, 1.0, size=(3, n)))
+    conv, corr = _dense_steps(n)
+    S.ls_block_steps(reset=True)
+    S.lp_grad(a, 4.0)
+    torch.cuda.synchronize()
+    assert S.ls_block_steps(reset=True) == 3 * (conv + corr)
+    S.lp_grad(a, 4.0, want_grad=False)
+    torch.cuda.synchronize()
+    assert S.ls_block_steps(reset=True) == 3 * conv
+
+
 def test_grad_fp64_kernel_and_fd(S, O, W):
     """fp64 backward kernel vs the long-double oracle gradient and central
     finite differences of the long-double J (step 1e-6 relative)."""
